@@ -1,0 +1,181 @@
+/*
+ * rodv_phmm.h -- C ABI of the MI355X PairHMM engine.
+ *
+ * Drop-in boundary for roDV's batch API
+ *   bool pairhmm::schedule::schedule_pairhmm(haplotypes, reads, result, quality,
+ *        insertion_qualities, deletion_qualities, gap_contiguous_qualities,
+ *        use_double, max_idle_ratio_float, max_idle_ratio_double, verbose)
+ * (src/core/pairhmm_schedule.h:14-25, implemented at pairhmm_schedule.cpp:116-340).
+ * The C++ shim in rodv_phmm_schedule.hpp keeps that exact signature and calls
+ * the functions below.  Plain pointers and sizes only; nothing here throws.
+ *
+ * Encodings follow the reference: bases are ASCII 'A','C','G','T','N'
+ * (common.h:42-56); qualities are raw Phred integers masked with & 127
+ * (pairhmm_impl.cpp:89-92).  Anything else is rejected with
+ * RODV_PHMM_ERR_INVALID_ARG instead of the reference's undefined behaviour.
+ */
+#ifndef RODV_PHMM_H_
+#define RODV_PHMM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RODV_PHMM_VERSION 1
+
+/* Status codes.  0 is success. */
+enum {
+    RODV_PHMM_OK = 0,
+    RODV_PHMM_ERR_INVALID_ARG = 1, /* bad pointer, length, base or config     */
+    RODV_PHMM_ERR_OOM = 2,         /* host or device allocation failed        */
+    RODV_PHMM_ERR_DEVICE = 3,      /* HIP runtime error / no device           */
+    RODV_PHMM_ERR_UNSUPPORTED = 4, /* request outside what the engine supports */
+    RODV_PHMM_ERR_EMPTY = 5        /* M == 0 or N == 0 (the reference's false) */
+};
+
+/* Per-pair output flags. */
+#define RODV_PHMM_FLAG_RESCUED 0x01u   /* FP32 raw sum < 1e-28f: FP64 result  */
+#define RODV_PHMM_FLAG_UNDERFLOW 0x02u /* FP64 raw sum == 0: log10 is -inf    */
+#define RODV_PHMM_FLAG_AMBIGUOUS 0x04u /* FP32 raw within 2^-20 of 1e-28f     */
+#define RODV_PHMM_FLAG_INTER 0x08u     /* scored with inter-kernel semantics  */
+
+/*
+ * A batch: M haplotypes and N reads, concatenated.  The four quality strings
+ * of read r start at read_offsets[r] in their arrays, like the read bases.
+ * Regions are blocks haps[hap_begin, +num_haps) x reads[read_begin, +num_reads)
+ * scored as a full cross product, exactly one schedule_pairhmm call each.
+ * num_regions == 0 means one region over all haps x all reads.
+ * Output order: region by region, hap-major inside a region
+ * (result[hap][read], pairhmm_schedule.cpp:135-138).
+ */
+typedef struct rodv_phmm_region {
+    uint32_t hap_begin, num_haps;
+    uint32_t read_begin, num_reads;
+} rodv_phmm_region;
+
+typedef struct rodv_phmm_batch {
+    uint32_t num_haps;
+    const uint8_t *hap_bases;
+    const uint64_t *hap_offsets;
+    const uint32_t *hap_lengths;
+    uint32_t num_reads;
+    const uint8_t *read_bases;
+    const uint8_t *base_quals;
+    const uint8_t *ins_quals;
+    const uint8_t *del_quals;
+    const uint8_t *gcp_quals;
+    const uint64_t *read_offsets;
+    const uint32_t *read_lengths;
+    uint32_t num_regions;
+    const rodv_phmm_region *regions;
+} rodv_phmm_batch;
+
+/* Which reference kernel's FP32 arithmetic to reproduce bit for bit. */
+enum {
+    RODV_PHMM_SEMANTICS_INTRA = 0,   /* intra::computeLikelihoods for every pair */
+    RODV_PHMM_SEMANTICS_SCHEDULE = 1 /* schedule_pairhmm routing: pairs its greedy
+                                        grouping sends to the inter kernel use the
+                                        inter priors and FMA order */
+};
+
+typedef struct rodv_phmm_config {
+    uint32_t struct_size;         /* sizeof(rodv_phmm_config)                  */
+    int32_t semantics;            /* RODV_PHMM_SEMANTICS_*                      */
+    int32_t use_double;           /* schedule_pairhmm's use_double flag         */
+    double max_idle_ratio_float;  /* schedule_pairhmm's idle bounds (0.1)       */
+    double max_idle_ratio_double;
+    uint32_t simd_width_float;    /* reference host's group widths: 16 (AVX-512) */
+    uint32_t simd_width_double;   /* or 8/4 (AVX2); used by SEMANTICS_SCHEDULE  */
+    int32_t use_graph;            /* capture the batched launch in a hipGraph   */
+    int32_t waves_per_cu;         /* persistent grid size; 0 = autotuned/default */
+    int32_t short_lanes;          /* lanes per pair for reads <= 240: 16 or 64  */
+    int32_t extended_range;       /* write finite log10 where FP64 underflows   */
+} rodv_phmm_config;
+
+/* Counters of the last execution of a plan. */
+typedef struct rodv_phmm_stats {
+    uint64_t num_pairs;
+    uint64_t cells;          /* sum over pairs of R*H                      */
+    uint64_t rescued_pairs;  /* pairs that went through the FP64 pass      */
+    uint64_t rescued_cells;
+    uint64_t launches;       /* kernel launches of the last execution       */
+    uint32_t num_buckets;    /* FP32 kernel shapes used                     */
+    double fp32_kernel_ms;   /* summed FP32 forward kernel time (profiled)  */
+    double fp64_kernel_ms;   /* FP64 rescue kernel time (profiled)          */
+    double total_ms;         /* whole execution (profiled)                  */
+} rodv_phmm_stats;
+
+typedef struct rodv_phmm_ctx rodv_phmm_ctx;
+typedef struct rodv_phmm_plan rodv_phmm_plan;
+
+/* Builds the prior tables once (thread-safe).  Replaces the lazy
+ * Context<float/double> construction of context.h:90-166. */
+int rodv_phmm_init(void);
+const char *rodv_phmm_strerror(int status);
+void rodv_phmm_config_default(rodv_phmm_config *cfg);
+
+/* One context per device; owns device tables and scratch. */
+int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out);
+void rodv_phmm_ctx_destroy(rodv_phmm_ctx *ctx);
+
+/*
+ * Whole-batch forward, synchronous: host inputs, host outputs.  out_log10
+ * holds one double per pair (the schedule_pairhmm result values); out_flags
+ * (may be NULL) one RODV_PHMM_FLAG_* byte per pair.
+ * Replaces schedule_pairhmm (pairhmm_schedule.cpp:116-340).
+ */
+int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                      const rodv_phmm_config *cfg, double *out_log10, uint8_t *out_flags);
+
+/*
+ * Two-phase form for callers that score the same batch shape repeatedly or
+ * want inputs resident in HBM: plan_create validates, buckets and uploads;
+ * plan_execute runs the batched launch on `stream` (a hipStream_t, NULL for
+ * the default stream) without host synchronisation; plan_execute_plain runs
+ * one launch per region instead; plan_read synchronises and copies results.
+ */
+int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                          const rodv_phmm_config *cfg, rodv_phmm_plan **out);
+int rodv_phmm_plan_execute(rodv_phmm_plan *plan, void *stream);
+int rodv_phmm_plan_execute_plain(rodv_phmm_plan *plan, void *stream);
+/* Eager execution with a hipEvent around every launch; fills plan stats. */
+int rodv_phmm_plan_profile(rodv_phmm_plan *plan, void *stream);
+int rodv_phmm_plan_read(rodv_phmm_plan *plan, double *out_log10, uint8_t *out_flags,
+                        float *out_raw32, double *out_raw64);
+int rodv_phmm_plan_stats(const rodv_phmm_plan *plan, rodv_phmm_stats *out);
+uint64_t rodv_phmm_plan_num_pairs(const rodv_phmm_plan *plan);
+void rodv_phmm_plan_destroy(rodv_phmm_plan *plan);
+
+/* The engine's prior tables (for parity checks against context.h:90-166):
+ * ph2pr[128] and the triangular match-to-match table, first `mm_n` entries. */
+int rodv_phmm_tables_f32(float *ph2pr, float *mm, int mm_n);
+int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n);
+
+/*
+ * Adjoint (no counterpart in the reference).  Parametrised model:
+ *   delta = 10^(-gap_open/10), gamma = 10^(-gap_cont/10),
+ *   mm = 1 - 2 delta, gapm = 1 - gamma, mx = my = delta, xx = yy = gamma,
+ *   e_i = scale[bin(q_i)] * 10^(-q_i/10) with bin(q) = min(q, 47) / 6,
+ *   prior = 1 - e_i on a match (or N), e_i / 3 otherwise.
+ * Per-row insertion/deletion/gap-continuation qualities are replaced by the
+ * two global parameters.  Computes log10 L per pair (FP32, 2^120 scaling) and
+ * grad[10] = sum_k weight[k] * d log10 L_k / d(gap_open, gap_cont, scale[0..7]),
+ * checkpointing every `ckpt_stride`-th anti-diagonal and recomputing the rest.
+ */
+typedef struct rodv_phmm_params {
+    double gap_open;
+    double gap_cont;
+    double scale[8];
+} rodv_phmm_params;
+
+int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                      const rodv_phmm_params *params, const double *weights, int ckpt_stride,
+                      double *out_log10, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RODV_PHMM_H_ */

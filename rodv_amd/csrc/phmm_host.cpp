@@ -1,0 +1,897 @@
+// phmm_host.cpp -- host side of the MI355X PairHMM engine: prior tables,
+// validation, length bucketing, device upload, launch sequencing, hipGraph
+// capture and the C ABI declared in include/rodv_phmm.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/rodv_phmm.h"
+#include "phmm_internal.h"
+
+namespace rodv {
+// defined in phmm_kernels.hip
+void *fwd_f32_kernel(int L, int MR, bool inter);
+void *fwd_f64_rescue_kernel(bool inter);
+int fwd_f64_rescue_mr();
+hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
+hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
+hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
+}  // namespace rodv
+
+using namespace rodv;
+
+namespace {
+
+// ------------------------------------------------------------------ tables
+// Restates Context<float>/Context<double> (context.h:21-166): ph2pr by
+// powf/pow, the 80,001-entry Jacobian table in double cast to NUMBER, and the
+// match-to-match table whose log-sum runs in NUMBER precision.
+
+constexpr int kMaxQual = 254;
+constexpr int kJacSize = 80001;
+constexpr int kMMSize = (kMaxQual + 1) * (kMaxQual + 2) / 2;
+
+struct HostTables {
+    float ph2pr_f[128];
+    double ph2pr_d[128];
+    std::vector<float> mm_f;
+    std::vector<double> mm_d;
+    DevTables dev;
+};
+
+HostTables *g_tables = nullptr;
+std::once_flag g_tables_once;
+
+template <typename N>
+N approx_log10_sum(N small, N big, const std::vector<N> &jac) {
+    if (small > big) std::swap(small, big);
+    if (std::isinf(small) || std::isinf(big)) return big;
+    N diff = big - small;
+    if (diff >= (N)8.0) return big;
+    N x = (N)(diff * (N)(1.0 / 0.0001));
+    int ind = x > (N)0 ? (int)(x + (N)0.5) : (int)(x - (N)0.5);
+    return big + jac[ind];
+}
+
+void build_tables() {
+    HostTables *t = new HostTables();
+    std::vector<float> jf(kJacSize);
+    std::vector<double> jd(kJacSize);
+    for (int k = 0; k < kJacSize; k++) {
+        double v = std::log10(1.0 + std::pow(10.0, -((double)k) * 0.0001));
+        jd[k] = v;
+        jf[k] = (float)v;
+    }
+    t->mm_f.resize(kMMSize);
+    t->mm_d.resize(kMMSize);
+    const double inv_ln10 = 1.0 / std::log(10.0);
+    for (int i = 0, offset = 0; i <= kMaxQual; offset += ++i) {
+        for (int j = 0; j <= i; j++) {
+            double sf = approx_log10_sum<float>((float)(-0.1 * i), (float)(-0.1 * j), jf);
+            t->mm_f[offset + j] =
+                (float)std::pow(10.0, std::log1p(-std::min(1.0, std::pow(10.0, sf))) * inv_ln10);
+            double sd = approx_log10_sum<double>(-0.1 * i, -0.1 * j, jd);
+            t->mm_d[offset + j] =
+                std::pow(10.0, std::log1p(-std::min(1.0, std::pow(10.0, sd))) * inv_ln10);
+        }
+    }
+    const float third_f = 1.0f / 3.0f;
+    const double third_d = 1.0 / 3.0;
+    for (int x = 0; x < 128; x++) {
+        t->ph2pr_f[x] = powf(10.f, -((float)x) / 10.f);
+        t->ph2pr_d[x] = std::pow(10.0, -((double)x) / 10.0);
+        DevTables &d = t->dev;
+        d.ph2pr_f[x] = t->ph2pr_f[x];
+        d.omp_f[x] = 1.0f - t->ph2pr_f[x];
+        d.p3_f[x] = t->ph2pr_f[x] / 3.0f;
+        d.pt_f[x] = t->ph2pr_f[x] * third_f;
+        d.ph2pr_d[x] = t->ph2pr_d[x];
+        d.omp_d[x] = 1.0 - t->ph2pr_d[x];
+        d.p3_d[x] = t->ph2pr_d[x] / 3.0;
+        d.pt_d[x] = t->ph2pr_d[x] * third_d;
+    }
+    for (int k = 0; k < kMMQ; k++) {
+        t->dev.mm_f[k] = t->mm_f[k];
+        t->dev.mm_d[k] = t->mm_d[k];
+    }
+    g_tables = t;
+}
+
+const HostTables &tables() {
+    std::call_once(g_tables_once, build_tables);
+    return *g_tables;
+}
+
+// ------------------------------------------------------------------ helpers
+
+bool fail_alloc_hook() {
+    const char *e = std::getenv("RODV_PHMM_FAIL_ALLOC");
+    return e && e[0] == '1';
+}
+
+template <typename T>
+int dev_alloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    if (fail_alloc_hook()) return RODV_PHMM_ERR_OOM;
+    if (hipMalloc(reinterpret_cast<void **>(p), n * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        return RODV_PHMM_ERR_OOM;
+    }
+    return RODV_PHMM_OK;
+}
+
+template <typename T>
+int upload(T **p, const T *src, size_t n) {
+    int rc = dev_alloc(p, n);
+    if (rc) return rc;
+    if (n && hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return RODV_PHMM_ERR_DEVICE;
+    return RODV_PHMM_OK;
+}
+
+inline uint8_t hap_code(uint8_t b, bool *ok) {
+    switch (b) {
+    case 'A': return 1;
+    case 'C': return 2;
+    case 'G': return 4;
+    case 'T': return 8;
+    case 'N': return kHapN;
+    default: *ok = false; return 0;
+    }
+}
+
+inline uint8_t read_code(uint8_t b, bool *ok) {
+    switch (b) {
+    case 'A': return 0;
+    case 'C': return 1;
+    case 'G': return 2;
+    case 'T': return 3;
+    case 'N': return kReadN;
+    default: *ok = false; return 0;
+    }
+}
+
+constexpr int kUnroll = 8;  // must match kU in phmm_kernels.hip
+
+struct Bucket {
+    int L = 16, MR = 1;
+    bool inter = false;
+    void *kern = nullptr;
+    std::vector<int32_t> items;  // G entries per item
+    uint32_t n_items = 0;
+    uint32_t max_h = 0;
+    uint32_t max_strips = 1;
+    uint32_t lds_stride = 0;
+    int32_t *d_items = nullptr;
+    float *d_bnd = nullptr;
+    uint64_t bnd_elems = 0;
+    int grid = 0;
+    uint64_t cells = 0;
+};
+
+// one launch of the plain (per-region) schedule
+struct PlainLaunch {
+    int bucket;
+    uint32_t item_begin, n_items;
+    int grid;
+};
+
+}  // namespace
+
+struct rodv_phmm_ctx {
+    int device = 0;
+    int num_cus = 256;
+    DevTables *d_tab = nullptr;
+};
+
+struct rodv_phmm_plan {
+    rodv_phmm_ctx *ctx = nullptr;
+    rodv_phmm_config cfg{};
+    uint64_t n_pairs = 0;
+    uint64_t cells = 0;
+    // device inputs
+    uint8_t *d_hapc = nullptr;
+    uint64_t *d_hapc_off = nullptr;
+    uint32_t *d_hap_len = nullptr;
+    float *d_initd_f = nullptr;
+    double *d_initd_d = nullptr;
+    uint8_t *d_rcode = nullptr, *d_q = nullptr, *d_ins = nullptr, *d_del = nullptr,
+            *d_gcp = nullptr;
+    uint64_t *d_read_off = nullptr;
+    uint32_t *d_read_len = nullptr;
+    uint32_t *d_pair_hap = nullptr, *d_pair_read = nullptr;
+    uint8_t *d_pair_inter = nullptr;
+    // device outputs / scratch
+    float *d_raw32 = nullptr;
+    double *d_raw64 = nullptr;
+    double *d_out = nullptr;
+    uint8_t *d_flags = nullptr;
+    uint32_t *d_rescue = nullptr;
+    uint32_t *d_counters = nullptr;  // [0] rescue count, [1] rescue work, [2..] per launch
+    uint32_t n_counters = 0;
+    double *d_bnd64 = nullptr;
+    uint64_t bnd64_elems = 0;
+    int rescue_grid = 0;
+    uint32_t rescue_lds = 0;
+    // host copies needed at launch
+    std::vector<Bucket> buckets;
+    std::vector<PlainLaunch> plain;
+    std::vector<int32_t> plain_items_all;  // per bucket, region-ordered items
+    std::vector<int32_t *> d_plain_items;
+    std::vector<uint32_t> pair_R, pair_H;
+    bool any_inter = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t last_stream = nullptr;
+    rodv_phmm_stats stats{};
+};
+
+namespace {
+
+#define HIPCHK(x)                                          \
+    do {                                                   \
+        if ((x) != hipSuccess) {                           \
+            (void)hipGetLastError();                       \
+            return RODV_PHMM_ERR_DEVICE;                   \
+        }                                                  \
+    } while (0)
+
+void free_plan(rodv_phmm_plan *p) {
+    if (!p) return;
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    void *bufs[] = {p->d_hapc, p->d_hapc_off, p->d_hap_len, p->d_initd_f, p->d_initd_d,
+                    p->d_rcode, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
+                    p->d_read_len, p->d_pair_hap, p->d_pair_read, p->d_pair_inter,
+                    p->d_raw32, p->d_raw64, p->d_out, p->d_flags, p->d_rescue,
+                    p->d_counters, p->d_bnd64};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto &b : p->buckets) {
+        if (b.d_items) (void)hipFree(b.d_items);
+        if (b.d_bnd) (void)hipFree(b.d_bnd);
+    }
+    for (auto *d : p->d_plain_items)
+        if (d) (void)hipFree(d);
+    delete p;
+}
+
+// (L, MR) of the FP32 kernel for a read of R rows.  A single strip holds
+// (L-1)*MR read rows; longer reads run strip by strip in the L = 64 kernels.
+void choose_shape(uint32_t R, int short_lanes, int *L, int *MR) {
+    if (short_lanes != 64 && R <= 15u * 16u) {
+        *L = 16;
+        *MR = std::max(1, (int)((R + 14) / 15));
+        return;
+    }
+    *L = 64;
+    int mr = (int)((R + 62) / 63);
+    mr = ((mr + 3) / 4) * 4;
+    *MR = std::min(16, std::max(4, mr));
+}
+
+uint32_t lds_stride_for(uint32_t max_h, int L) {
+    uint32_t s = max_h + 3u * (uint32_t)L + 2u * kUnroll + 32u;
+    return (s + 15u) & ~15u;
+}
+
+int strips_for(uint32_t R, int L, int MR) {
+    const int C = L * MR - 1;
+    int off = 0;
+    for (; off < 2 * MR; off++)
+        if (((int)(R - 1 + off) % C) % MR == MR - 1) break;
+    return (int)(R - 1 + off) / C + 1;
+}
+
+int validate(const rodv_phmm_batch *b) {
+    if (!b) return RODV_PHMM_ERR_INVALID_ARG;
+    if (b->num_haps && (!b->hap_bases || !b->hap_offsets || !b->hap_lengths))
+        return RODV_PHMM_ERR_INVALID_ARG;
+    if (b->num_reads && (!b->read_bases || !b->base_quals || !b->ins_quals || !b->del_quals ||
+                         !b->gcp_quals || !b->read_offsets || !b->read_lengths))
+        return RODV_PHMM_ERR_INVALID_ARG;
+    if (b->num_regions && !b->regions) return RODV_PHMM_ERR_INVALID_ARG;
+    for (uint32_t k = 0; k < b->num_regions; k++) {
+        const rodv_phmm_region &r = b->regions[k];
+        if ((uint64_t)r.hap_begin + r.num_haps > b->num_haps ||
+            (uint64_t)r.read_begin + r.num_reads > b->num_reads)
+            return RODV_PHMM_ERR_INVALID_ARG;
+    }
+    return RODV_PHMM_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+
+extern "C" {
+
+int rodv_phmm_init(void) {
+    (void)tables();
+    return RODV_PHMM_OK;
+}
+
+const char *rodv_phmm_strerror(int status) {
+    switch (status) {
+    case RODV_PHMM_OK: return "ok";
+    case RODV_PHMM_ERR_INVALID_ARG: return "invalid argument";
+    case RODV_PHMM_ERR_OOM: return "out of memory";
+    case RODV_PHMM_ERR_DEVICE: return "HIP device error";
+    case RODV_PHMM_ERR_UNSUPPORTED: return "unsupported request";
+    case RODV_PHMM_ERR_EMPTY: return "empty batch (no haplotypes or no reads)";
+    default: return "unknown status";
+    }
+}
+
+void rodv_phmm_config_default(rodv_phmm_config *cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->struct_size = sizeof(*cfg);
+    cfg->semantics = RODV_PHMM_SEMANTICS_INTRA;
+    cfg->use_double = 0;
+    cfg->max_idle_ratio_float = 0.1;
+    cfg->max_idle_ratio_double = 0.1;
+    cfg->simd_width_float = 16;
+    cfg->simd_width_double = 8;
+    cfg->use_graph = 1;
+    cfg->waves_per_cu = 0;
+    cfg->short_lanes = 16;
+    cfg->extended_range = 0;
+}
+
+int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
+    if (!out) return RODV_PHMM_ERR_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || device < 0 || device >= ndev) {
+        (void)hipGetLastError();
+        return RODV_PHMM_ERR_DEVICE;
+    }
+    HIPCHK(hipSetDevice(device));
+    rodv_phmm_ctx *c = new (std::nothrow) rodv_phmm_ctx();
+    if (!c) return RODV_PHMM_ERR_OOM;
+    c->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cus > 0)
+        c->num_cus = cus;
+    const HostTables &t = tables();
+    int rc = upload(&c->d_tab, &t.dev, 1);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return RODV_PHMM_OK;
+}
+
+void rodv_phmm_ctx_destroy(rodv_phmm_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->d_tab) (void)hipFree(ctx->d_tab);
+    delete ctx;
+}
+
+int rodv_phmm_tables_f32(float *ph2pr, float *mm, int mm_n) {
+    const HostTables &t = tables();
+    if (mm_n < 0 || mm_n > kMMSize) return RODV_PHMM_ERR_INVALID_ARG;
+    if (ph2pr) std::memcpy(ph2pr, t.ph2pr_f, sizeof(t.ph2pr_f));
+    if (mm) std::memcpy(mm, t.mm_f.data(), sizeof(float) * (size_t)mm_n);
+    return RODV_PHMM_OK;
+}
+
+int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n) {
+    const HostTables &t = tables();
+    if (mm_n < 0 || mm_n > kMMSize) return RODV_PHMM_ERR_INVALID_ARG;
+    if (ph2pr) std::memcpy(ph2pr, t.ph2pr_d, sizeof(t.ph2pr_d));
+    if (mm) std::memcpy(mm, t.mm_d.data(), sizeof(double) * (size_t)mm_n);
+    return RODV_PHMM_OK;
+}
+
+int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
+                          const rodv_phmm_config *cfg_in, rodv_phmm_plan **out) {
+    if (!ctx || !out) return RODV_PHMM_ERR_INVALID_ARG;
+    *out = nullptr;
+    int rc = validate(b);
+    if (rc) return rc;
+    rodv_phmm_config cfg;
+    rodv_phmm_config_default(&cfg);
+    if (cfg_in) {
+        if (cfg_in->struct_size != sizeof(rodv_phmm_config)) return RODV_PHMM_ERR_INVALID_ARG;
+        cfg = *cfg_in;
+    }
+    if (cfg.semantics != RODV_PHMM_SEMANTICS_INTRA && cfg.semantics != RODV_PHMM_SEMANTICS_SCHEDULE)
+        return RODV_PHMM_ERR_INVALID_ARG;
+    if (cfg.short_lanes != 16 && cfg.short_lanes != 64) return RODV_PHMM_ERR_INVALID_ARG;
+    if (cfg.semantics == RODV_PHMM_SEMANTICS_SCHEDULE) return RODV_PHMM_ERR_UNSUPPORTED;
+    (void)tables();
+    HIPCHK(hipSetDevice(ctx->device));
+
+    // regions (default: one over everything)
+    std::vector<rodv_phmm_region> regions;
+    if (b->num_regions == 0)
+        regions.push_back(rodv_phmm_region{0, b->num_haps, 0, b->num_reads});
+    else
+        regions.assign(b->regions, b->regions + b->num_regions);
+    uint64_t n_pairs = 0;
+    for (auto &r : regions) {
+        if (r.num_haps == 0 || r.num_reads == 0) return RODV_PHMM_ERR_EMPTY;
+        n_pairs += (uint64_t)r.num_haps * r.num_reads;
+    }
+    if (n_pairs == 0) return RODV_PHMM_ERR_EMPTY;
+    if (n_pairs >= (1ull << 31)) return RODV_PHMM_ERR_UNSUPPORTED;
+
+    // ---- encode and validate sequences
+    const uint32_t M = b->num_haps, N = b->num_reads;
+    std::vector<uint64_t> hapc_off(M);
+    uint64_t hap_bytes = 0;
+    for (uint32_t h = 0; h < M; h++) {
+        if (b->hap_lengths[h] == 0) return RODV_PHMM_ERR_INVALID_ARG;
+        hapc_off[h] = hap_bytes;
+        hap_bytes += ((uint64_t)b->hap_lengths[h] + 15) & ~15ull;
+    }
+    std::vector<uint8_t> hapc(hap_bytes, 0);
+    std::vector<float> initd_f(M);
+    std::vector<double> initd_d(M);
+    bool ok = true;
+    for (uint32_t h = 0; h < M; h++) {
+        const uint8_t *s = b->hap_bases + b->hap_offsets[h];
+        uint8_t *d = hapc.data() + hapc_off[h];
+        for (uint32_t j = 0; j < b->hap_lengths[h]; j++) d[j] = hap_code(s[j], &ok);
+        initd_f[h] = 0x1p120f / (float)b->hap_lengths[h];   // pairhmm_impl.cpp:62
+        initd_d[h] = 0x1p1020 / (double)b->hap_lengths[h];
+    }
+    if (!ok) return RODV_PHMM_ERR_INVALID_ARG;
+    std::vector<uint64_t> read_off(N);
+    uint64_t read_bytes = 0;
+    for (uint32_t r = 0; r < N; r++) {
+        if (b->read_lengths[r] == 0) return RODV_PHMM_ERR_INVALID_ARG;
+        read_off[r] = read_bytes;
+        read_bytes += b->read_lengths[r];
+    }
+    std::vector<uint8_t> rcode(read_bytes), q(read_bytes), qi(read_bytes), qd(read_bytes),
+        qc(read_bytes);
+    for (uint32_t r = 0; r < N; r++) {
+        const uint64_t so = b->read_offsets[r], dst = read_off[r];
+        const uint32_t n = b->read_lengths[r];
+        for (uint32_t k = 0; k < n; k++) rcode[dst + k] = read_code(b->read_bases[so + k], &ok);
+        std::memcpy(&q[dst], b->base_quals + so, n);
+        std::memcpy(&qi[dst], b->ins_quals + so, n);
+        std::memcpy(&qd[dst], b->del_quals + so, n);
+        std::memcpy(&qc[dst], b->gcp_quals + so, n);
+    }
+    if (!ok) return RODV_PHMM_ERR_INVALID_ARG;
+
+    rodv_phmm_plan *p = new (std::nothrow) rodv_phmm_plan();
+    if (!p) return RODV_PHMM_ERR_OOM;
+    p->ctx = ctx;
+    p->cfg = cfg;
+    p->n_pairs = n_pairs;
+
+    // ---- pairs in output order, and their buckets
+    std::vector<uint32_t> ph(n_pairs), pr(n_pairs);
+    std::vector<uint32_t> pair_region(n_pairs);
+    {
+        uint64_t k = 0;
+        for (uint32_t ri = 0; ri < regions.size(); ri++) {
+            const auto &rg = regions[ri];
+            for (uint32_t a = 0; a < rg.num_haps; a++)
+                for (uint32_t c = 0; c < rg.num_reads; c++) {
+                    ph[k] = rg.hap_begin + a;
+                    pr[k] = rg.read_begin + c;
+                    pair_region[k] = ri;
+                    k++;
+                }
+        }
+    }
+    // key = L * 64 + MR
+    std::vector<int> bucket_of_key(64 * 2 + 17, -1);
+    std::vector<std::vector<uint32_t>> members;
+    uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
+    for (uint64_t k = 0; k < n_pairs; k++) {
+        const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
+        p->cells += (uint64_t)R * H;
+        if (R > 252) max_long_h = std::max(max_long_h, H);
+        int L, MR;
+        choose_shape(R, cfg.short_lanes, &L, &MR);
+        // a very long haplotype does not fit four groups' codes in LDS
+        if (L == 16 && 4ull * lds_stride_for(H, 16) > 60 * 1024) {
+            L = 64;
+            MR = 4;
+            while (MR < 16 && 63 * MR < (int)R) MR += 4;
+        }
+        const int key = (L == 16 ? 0 : 64) + MR;
+        if (bucket_of_key[key] < 0) {
+            bucket_of_key[key] = (int)p->buckets.size();
+            Bucket bk;
+            bk.L = L;
+            bk.MR = MR;
+            p->buckets.push_back(bk);
+            members.emplace_back();
+        }
+        members[bucket_of_key[key]].push_back((uint32_t)k);
+    }
+
+    // ---- items: length-sorted packing of G pairs per wave
+    for (size_t bi = 0; bi < p->buckets.size(); bi++) {
+        Bucket &bk = p->buckets[bi];
+        bk.kern = fwd_f32_kernel(bk.L, bk.MR, false);
+        if (!bk.kern) {
+            free_plan(p);
+            return RODV_PHMM_ERR_UNSUPPORTED;
+        }
+        auto &mem = members[bi];
+        std::stable_sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
+            uint32_t hx = b->hap_lengths[ph[x]], hy = b->hap_lengths[ph[y]];
+            if (hx != hy) return hx > hy;
+            return b->read_lengths[pr[x]] > b->read_lengths[pr[y]];
+        });
+        const int G = 64 / bk.L;
+        bk.n_items = (uint32_t)((mem.size() + G - 1) / G);
+        bk.items.assign((size_t)bk.n_items * G, -1);
+        for (size_t k = 0; k < mem.size(); k++) {
+            bk.items[k] = (int32_t)mem[k];
+            const uint32_t R = b->read_lengths[pr[mem[k]]], H = b->hap_lengths[ph[mem[k]]];
+            bk.max_h = std::max(bk.max_h, H);
+            bk.max_strips = std::max(bk.max_strips, (uint32_t)strips_for(R, bk.L, bk.MR));
+            bk.cells += (uint64_t)R * H;
+        }
+        bk.lds_stride = lds_stride_for(bk.max_h, bk.L);
+        const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu : 8;
+        bk.grid = (int)std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus * wpc);
+        if (bk.grid < 1) bk.grid = 1;
+        if (bk.max_strips > 1) {
+            bk.bnd_elems = 3ull * (bk.max_h + 3ull * bk.L + 2);
+        }
+    }
+    // buckets by descending work, so long launches start first
+    std::vector<size_t> order(p->buckets.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+        return p->buckets[x].cells * p->buckets[x].MR > p->buckets[y].cells * p->buckets[y].MR;
+    });
+    {
+        std::vector<Bucket> sorted;
+        std::vector<std::vector<uint32_t>> smem;
+        for (size_t k : order) {
+            sorted.push_back(std::move(p->buckets[k]));
+            smem.push_back(std::move(members[k]));
+        }
+        p->buckets = std::move(sorted);
+        members = std::move(smem);
+    }
+
+    // ---- plain schedule: per region, per bucket, region-ordered items
+    {
+        for (size_t bi = 0; bi < p->buckets.size(); bi++) {
+            Bucket &bk = p->buckets[bi];
+            const int G = 64 / bk.L;
+            auto mem = members[bi];
+            std::stable_sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
+                if (pair_region[x] != pair_region[y]) return pair_region[x] < pair_region[y];
+                uint32_t hx = b->hap_lengths[ph[x]], hy = b->hap_lengths[ph[y]];
+                return hx > hy;
+            });
+            std::vector<int32_t> items;
+            size_t k = 0;
+            while (k < mem.size()) {
+                const uint32_t rg = pair_region[mem[k]];
+                const uint32_t begin = (uint32_t)(items.size() / G);
+                while (k < mem.size() && pair_region[mem[k]] == rg) {
+                    for (int gg = 0; gg < G; gg++) {
+                        if (k < mem.size() && pair_region[mem[k]] == rg)
+                            items.push_back((int32_t)mem[k++]);
+                        else
+                            items.push_back(-1);
+                    }
+                }
+                const uint32_t n = (uint32_t)(items.size() / G) - begin;
+                const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu : 8;
+                PlainLaunch pl;
+                pl.bucket = (int)bi;
+                pl.item_begin = begin;
+                pl.n_items = n;
+                pl.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t)ctx->num_cus * wpc));
+                pl.bucket = (int)bi;
+                p->plain.push_back(pl);
+                (void)rg;
+            }
+            int32_t *d = nullptr;
+            rc = upload(&d, items.data(), items.size());
+            p->d_plain_items.push_back(d);
+            if (rc) {
+                free_plan(p);
+                return rc;
+            }
+        }
+        // regions in order: launches sorted by (region of first item)
+    }
+
+    // ---- device upload
+    rc = upload(&p->d_hapc, hapc.data(), hapc.size());
+    if (!rc) rc = upload(&p->d_hapc_off, hapc_off.data(), hapc_off.size());
+    if (!rc) rc = upload(&p->d_hap_len, b->hap_lengths, M);
+    if (!rc) rc = upload(&p->d_initd_f, initd_f.data(), M);
+    if (!rc) rc = upload(&p->d_initd_d, initd_d.data(), M);
+    if (!rc) rc = upload(&p->d_rcode, rcode.data(), rcode.size());
+    if (!rc) rc = upload(&p->d_q, q.data(), q.size());
+    if (!rc) rc = upload(&p->d_ins, qi.data(), qi.size());
+    if (!rc) rc = upload(&p->d_del, qd.data(), qd.size());
+    if (!rc) rc = upload(&p->d_gcp, qc.data(), qc.size());
+    if (!rc) rc = upload(&p->d_read_off, read_off.data(), N);
+    if (!rc) rc = upload(&p->d_read_len, b->read_lengths, N);
+    if (!rc) rc = upload(&p->d_pair_hap, ph.data(), n_pairs);
+    if (!rc) rc = upload(&p->d_pair_read, pr.data(), n_pairs);
+    if (!rc) rc = dev_alloc(&p->d_raw32, n_pairs);
+    if (!rc) rc = dev_alloc(&p->d_raw64, n_pairs);
+    if (!rc) rc = dev_alloc(&p->d_out, n_pairs);
+    if (!rc) rc = dev_alloc(&p->d_flags, n_pairs);
+    if (!rc) rc = dev_alloc(&p->d_rescue, n_pairs);
+    for (auto &bk : p->buckets) {
+        if (!rc) rc = upload(&bk.d_items, bk.items.data(), bk.items.size());
+        if (!rc && bk.bnd_elems)
+            rc = dev_alloc(&bk.d_bnd, (size_t)bk.grid * 2 * bk.bnd_elems);
+    }
+    p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size();
+    if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
+    // FP64 rescue: one pair per wave; boundary rows only if some read > 252
+    const int wpc64 = 4;
+    p->rescue_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n_pairs, (uint64_t)ctx->num_cus * wpc64));
+    uint32_t max_h_all = 0;
+    for (uint32_t h = 0; h < M; h++) max_h_all = std::max(max_h_all, b->hap_lengths[h]);
+    p->rescue_lds = lds_stride_for(max_h_all, 64);
+    if (max_long_h) {
+        p->bnd64_elems = 3ull * (max_long_h + 3ull * 64 + 2);
+        if (!rc) rc = dev_alloc(&p->d_bnd64, (size_t)p->rescue_grid * 2 * p->bnd64_elems);
+    }
+    if (rc) {
+        free_plan(p);
+        return rc;
+    }
+    p->stats.num_pairs = n_pairs;
+    p->stats.cells = p->cells;
+    p->stats.num_buckets = (uint32_t)p->buckets.size();
+    *out = p;
+    return RODV_PHMM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+FwdArgs base_args(rodv_phmm_plan *p) {
+    FwdArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.tab = p->ctx->d_tab;
+    a.hapc = p->d_hapc;
+    a.hapc_off = p->d_hapc_off;
+    a.hap_len = p->d_hap_len;
+    a.initd_f = p->d_initd_f;
+    a.initd_d = p->d_initd_d;
+    a.rcode = p->d_rcode;
+    a.q = p->d_q;
+    a.ins = p->d_ins;
+    a.del = p->d_del;
+    a.gcp = p->d_gcp;
+    a.read_off = p->d_read_off;
+    a.read_len = p->d_read_len;
+    a.pair_hap = p->d_pair_hap;
+    a.pair_read = p->d_pair_read;
+    a.pair_inter = p->d_pair_inter;
+    a.raw32 = p->d_raw32;
+    a.raw64 = p->d_raw64;
+    a.flags = p->d_flags;
+    a.rescue_list = p->d_rescue;
+    a.rescue_count = p->d_counters;
+    return a;
+}
+
+// Launch sequence.  mode 0: batched (one persistent launch per bucket);
+// mode 1: plain (one launch per region and bucket).  Events, if given,
+// bracket the FP32 part, the FP64 rescue and the finalisation.
+int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
+    uint64_t launches = 0;
+    HIPCHK(launch_reset(p->d_counters, p->n_counters, st));
+    launches++;
+    if (ev) HIPCHK(hipEventRecord(ev[0], st));
+    // schedule_pairhmm's use_double only reroutes pairs from the inter to the
+    // intra kernel, which is FP32-first anyway (pairhmm_schedule.cpp:193,
+    // 259-262, 307-335): under intra semantics it changes nothing.
+    {
+        if (mode == 0) {
+            for (size_t bi = 0; bi < p->buckets.size(); bi++) {
+                Bucket &bk = p->buckets[bi];
+                FwdArgs a = base_args(p);
+                a.items = bk.d_items;
+                a.n_items = bk.n_items;
+                a.counter = p->d_counters + 2 + bi;
+                a.lds_stride = bk.lds_stride;
+                a.bnd = bk.d_bnd;
+                a.bnd_elems = bk.bnd_elems;
+                a.max_strips = bk.max_strips;
+                const size_t lds = (size_t)(64 / bk.L) * bk.lds_stride;
+                HIPCHK(launch_fwd(bk.kern, bk.grid, lds, st, a));
+                launches++;
+            }
+        } else {
+            for (size_t k = 0; k < p->plain.size(); k++) {
+                const PlainLaunch &pl = p->plain[k];
+                Bucket &bk = p->buckets[pl.bucket];
+                const int G = 64 / bk.L;
+                FwdArgs a = base_args(p);
+                a.items = p->d_plain_items[pl.bucket] + (size_t)pl.item_begin * G;
+                a.n_items = pl.n_items;
+                a.counter = p->d_counters + 2 + p->buckets.size() + k;
+                a.lds_stride = bk.lds_stride;
+                a.bnd = bk.d_bnd;
+                a.bnd_elems = bk.bnd_elems;
+                a.max_strips = bk.max_strips;
+                const size_t lds = (size_t)G * bk.lds_stride;
+                HIPCHK(launch_fwd(bk.kern, std::min(pl.grid, bk.grid), lds, st, a));
+                launches++;
+            }
+        }
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    {
+        FwdArgs a = base_args(p);
+        a.counter = p->d_counters + 1;
+        a.n_items_dev = p->d_counters;
+        a.lds_stride = p->rescue_lds;
+        a.bnd = p->d_bnd64;
+        a.bnd_elems = p->bnd64_elems;
+        HIPCHK(launch_fwd(fwd_f64_rescue_kernel(false), p->rescue_grid, p->rescue_lds, st, a));
+        launches++;
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
+    FinalizeArgs f;
+    f.raw32 = p->d_raw32;
+    f.raw64 = p->d_raw64;
+    f.pair_inter = p->d_pair_inter;
+    f.out = p->d_out;
+    f.flags = p->d_flags;
+    f.n = p->n_pairs;
+    HIPCHK(launch_finalize(f, st));
+    launches++;
+    if (ev) HIPCHK(hipEventRecord(ev[3], st));
+    p->stats.launches = launches;
+    p->last_stream = st;
+    return RODV_PHMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rodv_phmm_plan_execute(rodv_phmm_plan *p, void *stream) {
+    if (!p) return RODV_PHMM_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(p->ctx->device));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!p->cfg.use_graph) return enqueue(p, st, 0, nullptr);
+    if (!p->graph_exec) {
+        // capture on a private stream; the graph is replayed on `st`
+        hipStream_t cap;
+        HIPCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        if (hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipStreamDestroy(cap);
+            return RODV_PHMM_ERR_DEVICE;
+        }
+        int rc = enqueue(p, cap, 0, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cap, &g);
+        (void)hipStreamDestroy(cap);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess || !g) return RODV_PHMM_ERR_DEVICE;
+        p->graph = g;
+        if (hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            p->graph_exec = nullptr;
+            return RODV_PHMM_ERR_DEVICE;
+        }
+    }
+    HIPCHK(hipGraphLaunch(p->graph_exec, st));
+    p->last_stream = st;
+    return RODV_PHMM_OK;
+}
+
+int rodv_phmm_plan_execute_plain(rodv_phmm_plan *p, void *stream) {
+    if (!p) return RODV_PHMM_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(p->ctx->device));
+    return enqueue(p, reinterpret_cast<hipStream_t>(stream), 1, nullptr);
+}
+
+int rodv_phmm_plan_profile(rodv_phmm_plan *p, void *stream) {
+    if (!p) return RODV_PHMM_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(p->ctx->device));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t ev[4];
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    int rc = enqueue(p, st, 0, ev);
+    if (!rc && hipEventSynchronize(ev[3]) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc) {
+        float a = 0, b = 0, c = 0;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&c, ev[0], ev[3]);
+        p->stats.fp32_kernel_ms = a;
+        p->stats.fp64_kernel_ms = b;
+        p->stats.total_ms = c;
+        uint32_t cnt = 0;
+        if (hipMemcpy(&cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost) == hipSuccess)
+            p->stats.rescued_pairs = cnt;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int rodv_phmm_plan_read(rodv_phmm_plan *p, double *out_log10, uint8_t *out_flags,
+                        float *out_raw32, double *out_raw64) {
+    if (!p) return RODV_PHMM_ERR_INVALID_ARG;
+    HIPCHK(hipSetDevice(p->ctx->device));
+    HIPCHK(hipStreamSynchronize(p->last_stream));
+    const size_t n = p->n_pairs;
+    if (out_log10) HIPCHK(hipMemcpy(out_log10, p->d_out, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (out_flags) HIPCHK(hipMemcpy(out_flags, p->d_flags, n, hipMemcpyDeviceToHost));
+    if (out_raw32) HIPCHK(hipMemcpy(out_raw32, p->d_raw32, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (out_raw64) HIPCHK(hipMemcpy(out_raw64, p->d_raw64, n * sizeof(double), hipMemcpyDeviceToHost));
+    uint32_t cnt = 0;
+    HIPCHK(hipMemcpy(&cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    p->stats.rescued_pairs = cnt;
+    return RODV_PHMM_OK;
+}
+
+int rodv_phmm_plan_stats(const rodv_phmm_plan *p, rodv_phmm_stats *out) {
+    if (!p || !out) return RODV_PHMM_ERR_INVALID_ARG;
+    *out = p->stats;
+    return RODV_PHMM_OK;
+}
+
+uint64_t rodv_phmm_plan_num_pairs(const rodv_phmm_plan *p) { return p ? p->n_pairs : 0; }
+
+void rodv_phmm_plan_destroy(rodv_phmm_plan *p) {
+    if (!p) return;
+    (void)hipSetDevice(p->ctx->device);
+    if (p->last_stream || true) (void)hipDeviceSynchronize();
+    free_plan(p);
+}
+
+int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                      const rodv_phmm_config *cfg, double *out_log10, uint8_t *out_flags) {
+    if (!ctx || !out_log10) return RODV_PHMM_ERR_INVALID_ARG;
+    rodv_phmm_plan *p = nullptr;
+    rodv_phmm_config c;
+    rodv_phmm_config_default(&c);
+    if (cfg) c = *cfg;
+    c.use_graph = 0;  // one-shot: capture would not pay off
+    int rc = rodv_phmm_plan_create(ctx, batch, &c, &p);
+    if (rc) return rc;
+    rc = rodv_phmm_plan_execute(p, nullptr);
+    if (!rc) rc = rodv_phmm_plan_read(p, out_log10, out_flags, nullptr, nullptr);
+    rodv_phmm_plan_destroy(p);
+    return rc;
+}
+
+int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                      const rodv_phmm_params *params, const double *weights, int ckpt_stride,
+                      double *out_log10, double *grad) {
+    (void)ctx;
+    (void)batch;
+    (void)params;
+    (void)weights;
+    (void)ckpt_stride;
+    (void)out_log10;
+    (void)grad;
+    return RODV_PHMM_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

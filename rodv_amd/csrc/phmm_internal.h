@@ -1,0 +1,79 @@
+// phmm_internal.h -- types shared by the host planner and the gfx950 kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace rodv {
+
+// Qualities are masked with & 127 (pairhmm_impl.cpp:89-92), so the
+// triangular match-to-match table is only read up to index 127*128/2 + 127.
+constexpr int kMMQ = 128 * 129 / 2;
+
+// Prior tables in device memory, built on the host exactly as
+// Context<float>/Context<double> build them (context.h:90-166).
+struct DevTables {
+    float ph2pr_f[128];   // 10^(-q/10), powf
+    float omp_f[128];     // 1 - ph2pr_f (float subtraction)
+    float p3_f[128];      // ph2pr_f / 3 (IEEE division, intra: pairhmm_impl.cpp:154-156)
+    float pt_f[128];      // ph2pr_f * (1/3f) (inter: pairhmm_inter.cpp:81,90)
+    double ph2pr_d[128];
+    double omp_d[128];
+    double p3_d[128];
+    double pt_d[128];
+    float mm_f[kMMQ];
+    double mm_d[kMMQ];
+};
+
+// One-hot haplotype codes: A=1 C=2 G=4 T=8; N matches every read base.
+constexpr uint8_t kHapN = 0x0f;
+// Read codes: bit position of the base in the hap code (A0 C1 G2 T3), 4 = N.
+constexpr uint8_t kReadN = 4;
+
+// Arguments of every forward launch (passed by value).
+struct FwdArgs {
+    const DevTables *tab;
+    // haplotype codes; hap h starts at hapc + hapc_off[h], 16-byte aligned,
+    // zero-padded to a multiple of 16 bytes
+    const uint8_t *hapc;
+    const uint64_t *hapc_off;
+    const uint32_t *hap_len;
+    const float *initd_f;   // 2^120 / (float)H per hap  (pairhmm_impl.cpp:62)
+    const double *initd_d;  // 2^1020 / (double)H per hap
+    // reads: code per base and the four raw quality bytes
+    const uint8_t *rcode;
+    const uint8_t *q, *ins, *del, *gcp;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    // pairs
+    const uint32_t *pair_hap;
+    const uint32_t *pair_read;
+    const uint8_t *pair_inter;  // 1: inter semantics (may be null)
+    // work: items of G pair indices (-1 = empty); or, for the rescue pass,
+    // a list of pair indices whose length is *n_items_dev
+    const int32_t *items;
+    uint32_t n_items;
+    const uint32_t *n_items_dev;
+    uint32_t *counter;  // persistent work counter (zeroed by the reset kernel)
+    // outputs
+    float *raw32;
+    double *raw64;
+    uint8_t *flags;
+    uint32_t *rescue_list;
+    uint32_t *rescue_count;
+    // per-block boundary rows for reads longer than one strip
+    void *bnd;
+    uint64_t bnd_elems;  // elements of T per buffer; 2 buffers per group
+    uint32_t lds_stride; // LDS bytes per group
+    uint32_t max_strips;
+};
+
+struct FinalizeArgs {
+    const float *raw32;
+    const double *raw64;
+    const uint8_t *pair_inter;
+    double *out;
+    uint8_t *flags;  // in: RESCUED set by the FP32 pass; out: all flags
+    uint64_t n;
+};
+
+}  // namespace rodv

@@ -1,0 +1,477 @@
+// phmm_kernels.hip -- gfx950 kernels of the PairHMM forward.
+//
+// The recurrence is roDV's (pairhmm_impl.cpp:266-289) on the DP interior
+// i = 1..R (read rows), j = 1..H (haplotype columns):
+//   M[i][j] = prior(i,j) * (mm_i*M[i-1][j-1] + gapm_i*(I[i-1][j-1] + D[i-1][j-1]))
+//   I[i][j] = mx_i*M[i-1][j] + xx_i*I[i-1][j]
+//   D[i][j] = my_i*M[i][j-1] + yy_i*D[i][j-1]
+// with row 0 = (0, 0, K/H) and column 0 = 0; S = sum_j M[R][j] + sum_j I[R][j].
+// The FP32 pass reproduces the reference's rounding exactly: the FMA split
+// GCC gives the intra kernel (M = fma(D',g, fma(I',g, M'*mm)) * prior,
+// I = fma(I^,xx, M^*mx), D = fma(D<,yy, M<*my)), the inter kernel's own M
+// order when asked for, the same tables and the same left-to-right sums.
+// Compiled with -ffp-contract=off so only the explicit fmas fuse.
+//
+// Mapping (MI355X, wave64).  A pair owns a group of L lanes (L = 16: one DPP
+// row, four pairs per wave; L = 64: one pair per wave).  Lane l holds MR
+// consecutive read rows in registers and sweeps the haplotype one column per
+// step, lagging lane l-1 by one column, so a step is an anti-diagonal of
+// lanes.  Rows inside a lane pass values in registers; the row above a lane's
+// first row is the previous lane's last row, read with one DPP row_ror:1
+// (wave_ror:1 for L = 64).  The rotation makes lane 0 read lane L-1's last
+// slot, which is never a read row: it is an identity row that reproduces the
+// row-0 boundary (M = I = 0, D = K/H) for free, or -- for reads longer than
+// one strip of L*MR-1 rows -- the feeder that injects the previous strip's
+// last row from HBM.  Read rows are shifted down by `off` identity rows so
+// that row R always sits in the last register slot of its lane.
+// Haplotype codes are staged in LDS (one-hot, one byte per column); each
+// step reads one byte per lane and selects the emission prior with
+// v_bfe_i32 + v_bfi_b32.  No MFMA: the recurrence is not a contraction.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "phmm_internal.h"
+
+namespace rodv {
+namespace {
+
+constexpr int kU = 8;  // steps per unrolled block (even: the register rotation)
+
+// ---------------------------------------------------------------- helpers
+
+template <int L>
+__device__ __forceinline__ int ror1_i(int x) {
+    // row_ror:1 (0x121) rotates within a 16-lane row, wave_ror:1 (0x13C)
+    // across the wave: lane l receives lane l-1, lane 0 receives the last.
+    return __builtin_amdgcn_mov_dpp(x, L == 16 ? 0x121 : 0x13C, 0xf, 0xf, true);
+}
+template <int L>
+__device__ __forceinline__ float ror1(float x) {
+    return __int_as_float(ror1_i<L>(__float_as_int(x)));
+}
+template <int L>
+__device__ __forceinline__ double ror1(double x) {
+    return __hiloint2double(ror1_i<L>(__double2hiint(x)), ror1_i<L>(__double2loint(x)));
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Emission prior: X on a match, Y otherwise.  hc is the one-hot hap code of
+// the lane's current column, rpos the bit of the row's read base.
+__device__ __forceinline__ float select_prior(uint32_t hc, uint32_t rpos, float X, float Y) {
+    int m = __builtin_amdgcn_sbfe((int)hc, rpos, 1);  // 0 or -1
+    return __int_as_float((__float_as_int(X) & m) | (__float_as_int(Y) & ~m));
+}
+__device__ __forceinline__ double select_prior(uint32_t hc, uint32_t rpos, double X, double Y) {
+    return ((hc >> rpos) & 1u) ? X : Y;
+}
+
+template <typename T> struct Tab;
+template <> struct Tab<float> {
+    static __device__ __forceinline__ const float *ph2pr(const DevTables *t) { return t->ph2pr_f; }
+    static __device__ __forceinline__ const float *omp(const DevTables *t) { return t->omp_f; }
+    static __device__ __forceinline__ const float *p3(const DevTables *t) { return t->p3_f; }
+    static __device__ __forceinline__ const float *pt(const DevTables *t) { return t->pt_f; }
+    static __device__ __forceinline__ const float *mm(const DevTables *t) { return t->mm_f; }
+    static __device__ __forceinline__ float initd(const FwdArgs &a, uint32_t h) { return a.initd_f[h]; }
+};
+template <> struct Tab<double> {
+    static __device__ __forceinline__ const double *ph2pr(const DevTables *t) { return t->ph2pr_d; }
+    static __device__ __forceinline__ const double *omp(const DevTables *t) { return t->omp_d; }
+    static __device__ __forceinline__ const double *p3(const DevTables *t) { return t->p3_d; }
+    static __device__ __forceinline__ const double *pt(const DevTables *t) { return t->pt_d; }
+    static __device__ __forceinline__ const double *mm(const DevTables *t) { return t->mm_d; }
+    static __device__ __forceinline__ double initd(const FwdArgs &a, uint32_t h) { return a.initd_d[h]; }
+};
+
+// Per-lane register state: MR rows at the lane's last column, plus the last
+// row one column earlier (read by the next lane as its diagonal).
+template <typename T, int MR>
+struct State {
+    T M[MR], I[MR], D[MR];
+    T Mp, Ip, Dp;
+};
+
+// Per-row transition and emission priors (pairhmm_impl.cpp:88-115).
+template <typename T, int MR>
+struct Priors {
+    T mm[MR], g[MR], mx[MR], xx[MR], yy[MR], my[MR], X[MR], Y[MR];
+    uint32_t rpos[MR];
+};
+
+template <typename T, int MR, bool INTER>
+__device__ __forceinline__ void load_row(const FwdArgs &a, Priors<T, MR> &P, int s, bool real,
+                                         uint64_t idx) {
+    // branch-free: the address is always valid (idx points at the read's
+    // first base for identity rows) and identity rows select constants
+    const DevTables *t = a.tab;
+    uint32_t qi = a.ins[idx] & 127u, qd = a.del[idx] & 127u, qc = a.gcp[idx] & 127u;
+    uint32_t qq = a.q[idx] & 127u;
+    uint32_t code = a.rcode[idx];
+    uint32_t lo = qi <= qd ? qi : qd, hi = qi <= qd ? qd : qi;
+    T mm = Tab<T>::mm(t)[((hi * (hi + 1)) >> 1) + lo];
+    T g = Tab<T>::omp(t)[qc];
+    T mx = Tab<T>::ph2pr(t)[qi];
+    T xx = Tab<T>::ph2pr(t)[qc];
+    T my = Tab<T>::ph2pr(t)[qd];
+    T X = Tab<T>::omp(t)[qq];
+    T Y = INTER ? Tab<T>::pt(t)[qq] : Tab<T>::p3(t)[qq];
+    const bool isN = code == kReadN;  // N in the read matches every hap base
+    Y = isN ? X : Y;
+    code = isN ? 0u : code;
+    // identity row: M = 0, I = 0, D carried unchanged
+    P.mm[s] = real ? mm : T(0);
+    P.g[s] = real ? g : T(0);
+    P.mx[s] = real ? mx : T(0);
+    P.xx[s] = real ? xx : T(0);
+    P.yy[s] = real ? xx : T(1);
+    P.my[s] = real ? my : T(0);
+    P.X[s] = real ? X : T(0);
+    P.Y[s] = real ? Y : T(0);
+    P.rpos[s] = real ? code : 0u;
+}
+
+// One anti-diagonal step of one lane: all MR rows at the next column.
+template <typename T, int L, int MR, bool INTER>
+__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, uint32_t hc) {
+    // row above the lane's first row: previous lane's last row, at this column
+    // (up) and the column before (diagonal)
+    T dgM = ror1<L>(S.Mp), dgI = ror1<L>(S.Ip), dgD = ror1<L>(S.Dp);
+    T upM = ror1<L>(S.M[MR - 1]), upI = ror1<L>(S.I[MR - 1]);
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
+        T t;
+        if (INTER) {  // pairhmm_inter.cpp:279-283 as GCC contracts it
+            t = dgI * P.g[s];
+            t = fmaT(dgM, P.mm[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        } else {  // pairhmm_impl.cpp:266-275
+            t = dgM * P.mm[s];
+            t = fmaT(dgI, P.g[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        }
+        T nM = t * pr;
+        T u = upM * P.mx[s];
+        T nI = fmaT(upI, P.xx[s], u);
+        T v = S.M[s] * P.my[s];
+        T nD = fmaT(S.D[s], P.yy[s], v);
+        dgM = S.M[s];
+        dgI = S.I[s];
+        dgD = S.D[s];
+        upM = nM;
+        upI = nI;
+        S.M[s] = nM;
+        S.I[s] = nI;
+        S.D[s] = nD;
+    }
+    S.Mp = dgM;
+    S.Ip = dgI;
+    S.Dp = dgD;
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int wave_min4(int v, int G) {
+    if (G == 1) return __builtin_amdgcn_readfirstlane(v);
+    int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return min(min(a, b), min(c, d));
+}
+__device__ __forceinline__ int wave_max4(int v, int G) {
+    if (G == 1) return __builtin_amdgcn_readfirstlane(v);
+    int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return max(max(a, b), max(c, d));
+}
+
+// ------------------------------------------------------------ the kernel
+
+template <typename T, int L, int MR, bool INTER, bool RESCUE>
+__device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
+    constexpr int G = 64 / L;
+    constexpr int C = L * MR - 1;  // read rows per strip; slot C is the feeder
+    const int lane = threadIdx.x;
+    const int g = lane / L;
+    const int l = lane % L;
+    const bool valid = p >= 0;
+
+    uint32_t h = 0, r = 0;
+    int R = 0, H = 0;
+    uint64_t ro = 0;
+    if (valid) {
+        h = a.pair_hap[p];
+        r = a.pair_read[p];
+        R = (int)a.read_len[r];
+        H = (int)a.hap_len[h];
+        ro = a.read_off[r];
+    }
+    // shift the read down by `off` identity rows so that row R lands in the
+    // last slot of its lane (its strip position is = MR-1 mod MR)
+    int off = 0;
+    if (valid) {
+        for (; off < 2 * MR; off++) {
+            int pos = (R - 1 + off) % C;
+            if (pos % MR == MR - 1) break;
+        }
+    }
+    const int vR = R - 1 + off;
+    const int nstr = valid ? vR / C + 1 : 1;
+    const int lR = valid ? (vR % C) / MR : 0;
+    const T initd = valid ? Tab<T>::initd(a, h) : T(0);
+
+    // --- stage one-hot hap codes: column j of group g at lds[g*stride + L-1+j]
+    __syncthreads();
+    for (uint32_t k = lane * 16; k < (uint32_t)G * a.lds_stride; k += 64 * 16)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    if (valid) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
+        const int nch = (H + 15) >> 4;
+        for (int c = l; c < nch; c += L)
+            *reinterpret_cast<uint4 *>(lds + g * a.lds_stride + L + 16 * c) = src[c];
+    }
+    __syncthreads();
+    const uint8_t *cp = lds + g * a.lds_stride + (L - 1) - l;
+
+    const int key = valid ? H + lR : 0x7fffffff;
+    const int keymax = valid ? H + lR : -1;
+    const int Dmain_last = wave_min4(key, G);
+    const int Dend_last = wave_max4(keymax, G);
+    const int nstr_w = wave_max4(nstr, G);
+
+    T sm = T(0), si = T(0);
+    for (int sidx = 0; sidx < nstr_w; sidx++) {
+        const bool last = sidx == nstr_w - 1;
+        Priors<T, MR> P;
+        State<T, MR> S;
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int pos = l * MR + s;
+            const int i = sidx * C + pos - off + 1;  // 1-based read row
+            const bool real = valid && pos < C && i >= 1 && i <= R;
+            load_row<T, MR, INTER>(a, P, s, real, ro + (uint64_t)(real ? i - 1 : 0));
+            S.M[s] = T(0);
+            S.I[s] = T(0);
+            S.D[s] = real ? T(0) : initd;
+        }
+        S.Mp = T(0);
+        S.Ip = T(0);
+        S.Dp = S.D[MR - 1];
+
+        // boundary rows between strips (L = 64 kernels only)
+        T *bnd_in = nullptr, *bnd_out = nullptr;
+        bool inject = false, store = false;
+        const bool feeder = l == L - 1;
+        if constexpr (L == 64) {
+            if (nstr_w > 1) {
+                T *base = reinterpret_cast<T *>(a.bnd) + (uint64_t)blockIdx.x * 2 * a.bnd_elems;
+                bnd_in = base + (uint64_t)((sidx + 1) & 1) * a.bnd_elems;
+                bnd_out = base + (uint64_t)(sidx & 1) * a.bnd_elems;
+                inject = sidx > 0;
+                store = !last;
+            }
+            if (inject) {
+                __threadfence();
+                // feeder starts at column 1 of the previous strip's last row;
+                // its column-0 values are zero
+                T bm = ld_agent(bnd_in + 3 * (1 + L)), bi = ld_agent(bnd_in + 3 * (1 + L) + 1),
+                  bd = ld_agent(bnd_in + 3 * (1 + L) + 2);
+                if (feeder) {
+                    S.M[MR - 1] = bm;
+                    S.I[MR - 1] = bi;
+                    S.D[MR - 1] = bd;
+                    S.Dp = T(0);
+                }
+            }
+        }
+        // a strip that is not the last one runs until its last lane has passed
+        // column H + L, so the boundary row is written for every column the
+        // next strip's feeder reads; its sums are discarded
+        const int Dend = last ? Dend_last : H + 2 * L - 1;
+        const int Dmain = last ? Dmain_last : Dend;
+        sm = T(0);
+        si = T(0);
+
+        auto after_step = [&](int d) {
+            if constexpr (L == 64) {
+                if (store && feeder) {
+                    // last read row of this strip (slot MR-2 of lane L-1), column d-(L-1)
+                    T *o = bnd_out + 3 * (d + 1);
+                    st_agent(o, S.M[MR - 2]);
+                    st_agent(o + 1, S.I[MR - 2]);
+                    st_agent(o + 2, S.D[MR - 2]);
+                }
+                if (inject) {
+                    const T *in = bnd_in + 3 * (d + 1 + L);
+                    T bm = ld_agent(in), bi = ld_agent(in + 1), bd = ld_agent(in + 2);
+                    if (feeder) {
+                        S.M[MR - 1] = bm;
+                        S.I[MR - 1] = bi;
+                        S.D[MR - 1] = bd;
+                    }
+                }
+            }
+        };
+
+        int d = 1;
+        // main loop: every lane's sum slot is at a column <= H
+        for (; d + kU - 1 <= Dmain; d += kU) {
+            uint32_t hc[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) hc[u] = cp[d + u];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                step<T, L, MR, INTER>(S, P, hc[u]);
+                sm += S.M[MR - 1];
+                si += S.I[MR - 1];
+                after_step(d + u);
+            }
+        }
+        // remaining steps: sums only while the lane's column is <= H
+        for (; d <= Dend; d++) {
+            step<T, L, MR, INTER>(S, P, cp[d]);
+            if (last && d - l <= H) {
+                sm += S.M[MR - 1];
+                si += S.I[MR - 1];
+            }
+            after_step(d);
+        }
+    }
+
+    if (valid && l == lR) {
+        T Ssum = sm + si;  // pairhmm_impl.cpp:434-437
+        if constexpr (!RESCUE) {
+            a.raw32[p] = (float)Ssum;
+            uint8_t fl = 0;
+            if ((float)Ssum < 1e-28f) {  // MIN_ACCEPTED, pairhmm_api.cpp:4,17
+                fl = 0x01;
+                uint32_t k = atomicAdd(a.rescue_count, 1u);
+                a.rescue_list[k] = (uint32_t)p;
+            }
+            a.flags[p] = fl;
+        } else {
+            a.raw64[p] = (double)Ssum;
+        }
+    }
+}
+
+template <typename T, int L, int MR, bool INTER, bool RESCUE>
+__global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int G = 64 / L;
+    const uint32_t n = RESCUE ? *a.n_items_dev : a.n_items;
+    for (;;) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        int p;
+        if constexpr (RESCUE)
+            p = (int)a.rescue_list[it];
+        else
+            p = a.items[(uint64_t)it * G + threadIdx.x / L];
+        process_item<T, L, MR, INTER, RESCUE>(a, lds, p);
+    }
+}
+
+// log10 conversion (pairhmm_api.cpp:17-22 and pairhmm_inter_api.h:11-15).
+__global__ void finalize_kernel(FinalizeArgs f) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= f.n) return;
+    uint8_t fl = f.flags[k];
+    const bool inter = f.pair_inter && f.pair_inter[k];
+    double out;
+    if (fl & 0x01) {
+        double r = f.raw64[k];
+        out = log10(r) - 307.050595577260822;
+        if (!(r > 0.0)) fl |= 0x02;
+    } else {
+        float r = f.raw32[k];
+        if (inter)
+            out = log10((double)r) - (double)36.1236000061f;
+        else
+            out = (double)(log10f(r) - 36.1236000061f);
+    }
+    // within 2^-20 relative of the rescue threshold: the decision is exact
+    // here, but would flip under a 1-ulp perturbation of the sum
+    const float r32 = f.raw32[k];
+    if (fabsf(r32 - 1e-28f) <= 1e-28f * 0x1p-20f) fl |= 0x04;
+    if (inter) fl |= 0x08;
+    f.out[k] = out;
+    f.flags[k] = fl;
+}
+
+__global__ void reset_kernel(uint32_t *counters, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) counters[k] = 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+
+using FwdKernel = void (*)(FwdArgs);
+
+template <typename T, int L, int MR, bool INTER, bool RESCUE>
+static void *kernel_ptr() {
+    return reinterpret_cast<void *>(&fwd_kernel<T, L, MR, INTER, RESCUE>);
+}
+
+// FP32 kernels: L = 16 for MR 1..16, L = 64 for MR in {4, 8, 12, 16}.
+void *fwd_f32_kernel(int L, int MR, bool inter) {
+#define K16(m) \
+    case m: return inter ? kernel_ptr<float, 16, m, true, false>() : kernel_ptr<float, 16, m, false, false>();
+#define K64(m) \
+    case m: return inter ? kernel_ptr<float, 64, m, true, false>() : kernel_ptr<float, 64, m, false, false>();
+    if (L == 16) {
+        switch (MR) {
+            K16(1) K16(2) K16(3) K16(4) K16(5) K16(6) K16(7) K16(8)
+            K16(9) K16(10) K16(11) K16(12) K16(13) K16(14) K16(15) K16(16)
+        default: return nullptr;
+        }
+    }
+    if (L == 64) {
+        switch (MR) {
+            K64(4) K64(8) K64(12) K64(16)
+        default: return nullptr;
+        }
+    }
+#undef K16
+#undef K64
+    return nullptr;
+}
+
+// FP64 rescue kernel: one pair per wave, 4 rows per lane (255-row strips).
+void *fwd_f64_rescue_kernel(bool inter) {
+    return inter ? kernel_ptr<double, 64, 4, true, true>() : kernel_ptr<double, 64, 4, false, true>();
+}
+int fwd_f64_rescue_mr() { return 4; }
+
+hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
+    void *args[] = {const_cast<FwdArgs *>(&a)};
+    return hipLaunchKernel(kern, dim3(grid), dim3(64), args, lds_bytes, st);
+}
+
+hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st) {
+    if (f.n == 0) return hipSuccess;
+    const int block = 256;
+    const uint64_t grid = (f.n + block - 1) / block;
+    hipLaunchKernelGGL(finalize_kernel, dim3((uint32_t)grid), dim3(block), 0, st, f);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(reset_kernel, dim3((n + 255) / 256), dim3(256), 0, st, counters, n);
+    return hipGetLastError();
+}
+
+}  // namespace rodv

@@ -1,0 +1,171 @@
+"""Parity of the HIP forward against the CPU oracle (which is itself pinned
+against the reference's compiled kernels and golden vectors)."""
+import os
+
+import numpy as np
+import pytest
+
+from rodv_amd import synth
+from rodv_amd.batch import Batch, parse_assemble_log, parse_gkl_testdata
+from rodv_amd.engine import (ERR_EMPTY, ERR_INVALID_ARG, FLAG_RESCUED, FLAG_UNDERFLOW,
+                             PhmmError, default_config)
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4  # |d log10 L| bound of this path
+
+
+def run(engine, batch, **cfg):
+    plan = engine.plan(batch, default_config(**cfg))
+    plan.execute()
+    out, fl, r32, r64 = plan.read(raw=True)
+    plan.close()
+    return out, fl, r32, r64
+
+
+def check_against_oracle(oracle_mod, engine, batch, raw_exact=True, **cfg):
+    out, fl, r32, r64 = run(engine, batch, **cfg)
+    o_out, o32, o64, ofl = oracle_mod.run_batch(batch, nthreads=8)
+    # FP32 raw sums bit-exact (same rounding as the reference's intra kernel)
+    if raw_exact:
+        bad = np.nonzero(r32.view(np.uint32) != o32.view(np.uint32))[0]
+        assert len(bad) == 0, f"{len(bad)} raw FP32 mismatches, first {bad[:5]}: {r32[bad[:5]]} vs {o32[bad[:5]]}"
+    # rescue set identical
+    np.testing.assert_array_equal(fl & FLAG_RESCUED, ofl & FLAG_RESCUED)
+    np.testing.assert_array_equal(fl & FLAG_UNDERFLOW, ofl & FLAG_UNDERFLOW)
+    fin = np.isfinite(o_out)
+    np.testing.assert_array_equal(np.isfinite(out), fin)
+    assert np.all(out[~fin] == o_out[~fin])
+    if fin.any():
+        err = np.abs(out[fin] - o_out[fin])
+        assert err.max() <= TOL, f"max |dlog10L| {err.max()}"
+    return out, fl, o_out
+
+
+def test_dpp_lane_layout_and_kats(engine, oracle_mod):
+    # the two known answers of pairhmm_unittest.cpp:103-136
+    q4 = [43] * 4
+    q5 = [43] * 5
+    b = Batch.from_lists(["ACGT", "ACGTA"], ["ACGT", "ACGTA"], [q4, q5], [q4, q5], [q4, q5],
+                         [q4, q5], regions=np.array([[0, 1, 0, 1], [1, 1, 1, 1]]))
+    out, fl, r32, _ = run(engine, b)
+    assert abs(out[0] - (-6.022797e-01)) < 1e-5
+    assert abs(out[1] - (-0.69925308227539062)) < 1e-5
+    assert not (fl & FLAG_RESCUED).any()
+
+
+def test_gkl_golden_vectors(engine, oracle_mod):
+    b, exp = parse_gkl_testdata(os.path.join(GOLDEN, "pairhmm-testdata.txt"))
+    out, fl, _ = check_against_oracle(oracle_mod, engine, b)
+    # pairhmm_unittest.cpp:347-396: relative 1e-5 against the expected column
+    assert np.max(np.abs(out - exp) / np.abs(exp)) < 1e-5
+
+
+def test_assemble_log_regions(engine, oracle_mod):
+    regs = parse_assemble_log(os.path.join(GOLDEN, "pairhmm_debug.txt"))
+    b = Batch.concat(regs)
+    check_against_oracle(oracle_mod, engine, b)
+
+
+@pytest.mark.parametrize("lanes", [16, 64])
+@pytest.mark.parametrize("seed,rlo,rhi", [(1, 1, 20), (2, 10, 120), (3, 100, 240), (4, 200, 400),
+                                          (5, 230, 260)])
+def test_random_lengths(engine, oracle_mod, seed, rlo, rhi, lanes):
+    rng = np.random.default_rng(seed)
+    b = synth.random_batch(rng, 7, 23, hap_len=(1, 420), read_len=(rlo, rhi))
+    check_against_oracle(oracle_mod, engine, b, short_lanes=lanes)
+
+
+def test_every_rows_per_lane_shape(engine, oracle_mod):
+    # one read length per (L=16, MR) shape and both ends of each bucket
+    lens = sorted({max(1, 15 * m - d) for m in range(1, 17) for d in (0, 14)} | {241, 252, 253, 504, 505, 1008})
+    rng = np.random.default_rng(11)
+    haps = [synth.ACGT[rng.integers(0, 4, n)].tobytes() for n in (1, 2, 17, 64, 333)]
+    reads, qs = [], []
+    for R in lens:
+        reads.append(synth.ACGT[rng.integers(0, 4, R)].tobytes())
+        qs.append(rng.integers(2, 45, R))
+    b = Batch.from_lists(haps, reads, qs, [np.full(len(q), 45) for q in qs],
+                         [np.full(len(q), 45) for q in qs], [np.full(len(q), 10) for q in qs])
+    check_against_oracle(oracle_mod, engine, b)
+
+
+def test_n_bases_are_wildcards(engine, oracle_mod):
+    rng = np.random.default_rng(5)
+    b = synth.random_batch(rng, 6, 20, n_frac=0.05)
+    check_against_oracle(oracle_mod, engine, b)
+
+
+def test_rescue_unrelated_reads(engine, oracle_mod):
+    # unrelated reads of length >= ~70 underflow FP32 (log10 L < -64.12)
+    rng = np.random.default_rng(7)
+    b = synth.random_batch(rng, 5, 40, hap_len=(150, 300), read_len=(40, 200), p_sub=0.75)
+    out, fl, _ = check_against_oracle(oracle_mod, engine, b)
+    assert (fl & FLAG_RESCUED).sum() > 10
+
+
+def test_fp64_underflow_long_reads(engine, oracle_mod):
+    b = synth.config(4, scale=4 / 256)
+    out, fl, o = check_against_oracle(oracle_mod, engine, b)
+    assert (fl & FLAG_RESCUED).all()
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_configs(engine, oracle_mod, cfg):
+    check_against_oracle(oracle_mod, engine, synth.config(cfg))
+
+
+def test_config2_subsample(engine, oracle_mod):
+    b = synth.config(2, scale=0.002)
+    check_against_oracle(oracle_mod, engine, b)
+
+
+def test_plain_equals_batched_and_graph_replays(engine):
+    b = synth.config(2, scale=0.003)
+    plan = engine.plan(b, default_config(use_graph=1))
+    plan.execute_plain()
+    ref = plan.read(raw=True)
+    for _ in range(3):
+        plan.execute()
+        got = plan.read(raw=True)
+        np.testing.assert_array_equal(got[2].view(np.uint32), ref[2].view(np.uint32))
+        np.testing.assert_array_equal(got[1], ref[1])
+        np.testing.assert_array_equal(got[0], ref[0])
+    plan.close()
+
+
+def test_schedule_pairhmm_call_shape(engine, oracle_mod):
+    rng = np.random.default_rng(3)
+    b = synth.random_batch(rng, 3, 5)
+    haps = [b.hap(k) for k in range(3)]
+    reads = [b.read(k) for k in range(5)]
+    qs = [b.quals(k) for k in range(5)]
+    res = []
+    ok = engine.schedule_pairhmm(haps, reads, res, [q[0] for q in qs], [q[1] for q in qs],
+                                 [q[2] for q in qs], [q[3] for q in qs])
+    assert ok and len(res) == 3 and all(len(r) == 5 for r in res)
+    assert engine.schedule_pairhmm([], reads, res, [], [], [], []) is False
+    o, _, _, _ = oracle_mod.run_batch(b)
+    assert np.max(np.abs(np.array(res).ravel() - o)) <= TOL
+
+
+def test_invalid_inputs_are_rejected(engine):
+    q = [[30, 30, 30]]
+    bad_base = Batch.from_lists(["ACGT"], ["ACX"], q, q, q, q)
+    with pytest.raises(PhmmError) as e:
+        engine.plan(bad_base)
+    assert e.value.code == ERR_INVALID_ARG
+    lower = Batch.from_lists(["acgt"], ["ACG"], q, q, q, q)
+    with pytest.raises(PhmmError) as e:
+        engine.plan(lower)
+    assert e.value.code == ERR_INVALID_ARG
+    empty_read = Batch.from_lists(["ACGT"], [""], [[]], [[]], [[]], [[]])
+    with pytest.raises(PhmmError) as e:
+        engine.plan(empty_read)
+    assert e.value.code == ERR_INVALID_ARG
+    no_reads = Batch.from_lists(["ACGT"], [], [], [], [], [])
+    with pytest.raises(PhmmError) as e:
+        engine.plan(no_reads)
+    assert e.value.code == ERR_EMPTY
