@@ -28,47 +28,62 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(batch, budget_cells=2.0e9, seed=7):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(batch, seconds=12.0, seed=7):
     """Time the reference's own intra kernel (compiled from its sources into
-    oracle/_ref) on a seeded subsample of regions, one host thread; falls back
-    to the oracle restatement when the reference build is unavailable."""
+    oracle/_ref) on host core 0, one thread, over seeded random regions of
+    the same workload until about `seconds` of CPU work; falls back to the
+    oracle restatement when the reference build is unavailable here."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    rng = np.random.default_rng(seed)
-    order = rng.permutation(len(batch.regions))
-    pairs_h, pairs_r, cells = [], [], 0
-    for k in order:
-        h0, m, r0, n = (int(x) for x in batch.regions[k])
-        c = int(batch.hap_len[h0:h0 + m].sum()) * int(batch.read_len[r0:r0 + n].sum())
-        if cells and cells + c > budget_cells:
-            continue
-        for hh in range(h0, h0 + m):
-            for rr in range(r0, r0 + n):
-                pairs_h.append(hh)
-                pairs_r.append(rr)
-        cells += c
-        if cells >= 0.9 * budget_cells:
-            break
-    ph = np.array(pairs_h, np.uint32)
-    pr = np.array(pairs_r, np.uint32)
-    sample = f"{len(ph)} pairs from {len(set(pairs_h))} haplotypes, {cells:.3e} cells"
+    order = np.random.default_rng(seed).permutation(len(batch.regions))
+
+    def sample_regions():
+        for k in order:
+            h0, m, r0, n = (int(x) for x in batch.regions[k])
+            c = int(batch.hap_len[h0:h0 + m].sum()) * int(batch.read_len[r0:r0 + n].sum())
+            yield h0, m, r0, n, c
+
+    ref = None
     for isa in ("avx512", "avx2"):
         try:
             ref = O.RefLib(isa)
+            break
         except (FileNotFoundError, RuntimeError, OSError):
             continue
-        t0 = time.perf_counter()
-        for h, r in zip(ph, pr):
-            ref.log10(batch, int(h), int(r))
-        dt = time.perf_counter() - t0
-        return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1,
-                "kind": "reference",
-                "sample": f"intra::computeLikelihoods{isa.upper()} per pair, {sample}"}
+    cells = pairs = regions = 0
     t0 = time.perf_counter()
-    O.run_batch(batch, pairs=(ph, pr), nthreads=1)
+    for h0, m, r0, n, c in sample_regions():
+        if ref is not None:
+            for hh in range(h0, h0 + m):
+                for rr in range(r0, r0 + n):
+                    ref.log10(batch, hh, rr)
+        else:
+            hs = np.repeat(np.arange(h0, h0 + m, dtype=np.uint32), n)
+            rs = np.tile(np.arange(r0, r0 + n, dtype=np.uint32), m)
+            O.run_batch(batch, pairs=(hs, rs), nthreads=1)
+        cells += c
+        pairs += m * n
+        regions += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
     dt = time.perf_counter() - t0
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
-            "sample": f"oracle restatement, scalar, {sample}"}
+    what = (f"intra::computeLikelihoods{ref.isa.upper()} per pair (reference sources, "
+            "g++ -O3)" if ref is not None else "oracle restatement, scalar")
+    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1,
+            "kind": "reference" if ref is not None else "port",
+            "sample": f"{what}; {regions} random regions of the workload, {pairs} pairs, "
+                      f"{cells:.3e} cells in {dt:.1f} s; host {_cpu_model()}"}
 
 
 def main():
@@ -80,7 +95,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget-cells", type=float, default=2.0e9)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -172,7 +187,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         t0 = time.time()
-        res["cpu_baseline"] = cpu_baseline(batch, args.cpu_budget_cells)
+        res["cpu_baseline"] = cpu_baseline(batch, args.cpu_seconds)
         log(f"cpu baseline {res['cpu_baseline']} ({time.time() - t0:.1f}s)")
     plan.close()
     print(json.dumps(res), flush=True)

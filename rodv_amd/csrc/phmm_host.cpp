@@ -283,12 +283,11 @@ uint32_t lds_stride_for(uint32_t max_h, int L) {
     return (s + 15u) & ~15u;
 }
 
+// Strips of (L-1)*MR read rows, after `off` < MR leading identity rows.
 int strips_for(uint32_t R, int L, int MR) {
-    const int C = L * MR - 1;
-    int off = 0;
-    for (; off < 2 * MR; off++)
-        if (((int)(R - 1 + off) % C) % MR == MR - 1) break;
-    return (int)(R - 1 + off) / C + 1;
+    const int C = (L - 1) * MR;
+    const int off = (MR - (int)R % MR) % MR;
+    return ((int)R - 1 + off) / C + 1;
 }
 
 int validate(const rodv_phmm_batch *b) {
@@ -498,7 +497,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     for (uint64_t k = 0; k < n_pairs; k++) {
         const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
         p->cells += (uint64_t)R * H;
-        if (R > 252) max_long_h = std::max(max_long_h, H);
+        if (R > 63u * 4u) max_long_h = std::max(max_long_h, H);
         int L, MR;
         choose_shape(R, cfg.short_lanes, &L, &MR);
         // a very long haplotype does not fit four groups' codes in LDS
@@ -636,8 +635,11 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (!rc) rc = dev_alloc(&p->d_rescue, n_pairs);
     for (auto &bk : p->buckets) {
         if (!rc) rc = upload(&bk.d_items, bk.items.data(), bk.items.size());
-        if (!rc && bk.bnd_elems)
+        if (!rc && bk.bnd_elems) {
             rc = dev_alloc(&bk.d_bnd, (size_t)bk.grid * 2 * bk.bnd_elems);
+            if (!rc && hipMemset(bk.d_bnd, 0, (size_t)bk.grid * 2 * bk.bnd_elems * sizeof(float)) != hipSuccess)
+                rc = RODV_PHMM_ERR_DEVICE;
+        }
     }
     p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size();
     if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
@@ -650,6 +652,8 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (max_long_h) {
         p->bnd64_elems = 3ull * (max_long_h + 3ull * 64 + 2);
         if (!rc) rc = dev_alloc(&p->d_bnd64, (size_t)p->rescue_grid * 2 * p->bnd64_elems);
+        if (!rc && hipMemset(p->d_bnd64, 0, (size_t)p->rescue_grid * 2 * p->bnd64_elems * sizeof(double)) != hipSuccess)
+            rc = RODV_PHMM_ERR_DEVICE;
     }
     if (rc) {
         free_plan(p);

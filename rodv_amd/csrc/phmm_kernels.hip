@@ -198,7 +198,9 @@ __device__ __forceinline__ int wave_max4(int v, int G) {
 template <typename T, int L, int MR, bool INTER, bool RESCUE>
 __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
     constexpr int G = 64 / L;
-    constexpr int C = L * MR - 1;  // read rows per strip; slot C is the feeder
+    // read rows per strip: lanes 0..L-2; lane L-1 holds identity rows and,
+    // in its last slot, the row above the strip (row 0, or the feeder)
+    constexpr int C = (L - 1) * MR;
     const int lane = threadIdx.x;
     const int g = lane / L;
     const int l = lane % L;
@@ -214,15 +216,9 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
         H = (int)a.hap_len[h];
         ro = a.read_off[r];
     }
-    // shift the read down by `off` identity rows so that row R lands in the
-    // last slot of its lane (its strip position is = MR-1 mod MR)
-    int off = 0;
-    if (valid) {
-        for (; off < 2 * MR; off++) {
-            int pos = (R - 1 + off) % C;
-            if (pos % MR == MR - 1) break;
-        }
-    }
+    // shift the read down by `off` < MR identity rows (all in lane 0 of the
+    // first strip) so that row R lands in the last slot of its lane
+    const int off = valid ? (MR - R % MR) % MR : 0;
     const int vR = R - 1 + off;
     const int nstr = valid ? vR / C + 1 : 1;
     const int lR = valid ? (vR % C) / MR : 0;
@@ -256,7 +252,7 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
         for (int s = 0; s < MR; s++) {
             const int pos = l * MR + s;
             const int i = sidx * C + pos - off + 1;  // 1-based read row
-            const bool real = valid && pos < C && i >= 1 && i <= R;
+            const bool real = valid && l < L - 1 && i >= 1 && i <= R;
             load_row<T, MR, INTER>(a, P, s, real, ro + (uint64_t)(real ? i - 1 : 0));
             S.M[s] = T(0);
             S.I[s] = T(0);
@@ -292,9 +288,9 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
                 }
             }
         }
-        // a strip that is not the last one runs until its last lane has passed
-        // column H + L, so the boundary row is written for every column the
-        // next strip's feeder reads; its sums are discarded
+        // a strip that is not the last one runs until lane L-2 has passed
+        // column H + L + 1, so the boundary row is written for every column
+        // the next strip's feeder reads up to H; its sums are discarded
         const int Dend = last ? Dend_last : H + 2 * L - 1;
         const int Dmain = last ? Dmain_last : Dend;
         sm = T(0);
@@ -302,12 +298,13 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
 
         auto after_step = [&](int d) {
             if constexpr (L == 64) {
-                if (store && feeder) {
-                    // last read row of this strip (slot MR-2 of lane L-1), column d-(L-1)
-                    T *o = bnd_out + 3 * (d + 1);
-                    st_agent(o, S.M[MR - 2]);
-                    st_agent(o + 1, S.I[MR - 2]);
-                    st_agent(o + 2, S.D[MR - 2]);
+                if (store && l == L - 2) {
+                    // last read row of this strip (last slot of lane L-2), at
+                    // column d-(L-2); column j is stored at index j + L
+                    T *o = bnd_out + 3 * (d + 2);
+                    st_agent(o, S.M[MR - 1]);
+                    st_agent(o + 1, S.I[MR - 1]);
+                    st_agent(o + 2, S.D[MR - 1]);
                 }
                 if (inject) {
                     const T *in = bnd_in + 3 * (d + 1 + L);

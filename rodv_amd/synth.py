@@ -2,11 +2,16 @@
 
 Shapes follow the five benchmark configurations of this path (read length,
 haplotype length, mutation rates, quality distributions); see DESIGN.md for
-the exact interpretation.  Bases are uniform ACGT; a read is a substring of a
-uniformly chosen haplotype of its region with substitutions (to a different
-base) and indel events of length U{1..3} (50/50 insertion/deletion), trimmed
-or padded back to its target length.  Insertion/deletion qualities are 45 and
-gap-continuation quality 10 unless stated.  Seeds are 1000 + config.
+the exact interpretation.  Each region draws one uniform ACGT sequence; its
+haplotypes are prefixes of it (of their own lengths) carrying independent
+0.5% substitutions, as candidate haplotypes of one locus are (the reference's
+own generator builds mutation variants of one base sequence,
+pairhmm_schedule_unittest.cpp:39-200).  A read is a substring of a uniformly
+chosen haplotype of its region, inside the prefix all its haplotypes share,
+with substitutions (to a different base) and indel events of length U{1..3}
+(50/50 insertion/deletion), trimmed or padded back to its target length.
+Insertion/deletion qualities are 45 and gap-continuation quality 10 unless
+stated.  Seeds are 1000 + config.
 """
 from __future__ import annotations
 
@@ -59,19 +64,31 @@ def _mutate_indels(rng, read, hap_tail, p_indel):
 
 def make_workload(hap_len_per_hap, hap_region, read_len_per_read, read_region, region_haps,
                   region_reads, *, p_sub, p_indel, qual_fn, rng, p_unrelated=0.0,
-                  ins_q=45, del_q=45, gcp_q=10):
+                  ins_q=45, del_q=45, gcp_q=10, p_hapvar=0.005):
     """Generic generator.  region_haps/region_reads: (begin, count) per region."""
     H = np.asarray(hap_len_per_hap, np.int64)
-    hap_codes = rng.integers(0, 4, int(H.sum()), dtype=np.uint8)
     hoff = _offsets(H).astype(np.int64)
+    # one base sequence per region; haplotypes are variant prefixes of it
+    n_reg = len(region_haps)
+    reg_max = np.zeros(n_reg, np.int64)
+    reg_min = np.full(n_reg, np.iinfo(np.int64).max)
+    np.maximum.at(reg_max, hap_region, H)
+    np.minimum.at(reg_min, hap_region, H)
+    base_off = _offsets(reg_max).astype(np.int64)
+    base = rng.integers(0, 4, int(reg_max.sum()), dtype=np.uint8)
+    hap_codes = base[_ranges(base_off[hap_region], H)]
+    var = rng.random(len(hap_codes)) < p_hapvar
+    hap_codes[var] = (hap_codes[var] + rng.integers(1, 4, int(var.sum()), dtype=np.uint8)) % 4
     R = np.asarray(read_len_per_read, np.int64)
     nr = len(R)
-    # pick a source hap within the read's region
+    # pick a source hap within the read's region, and a window inside the
+    # prefix every haplotype of the region has
     rh0 = region_haps[read_region, 0]
     rhn = region_haps[read_region, 1]
     src = rh0 + (rng.random(nr) * rhn).astype(np.int64)
     Hs = H[src]
-    start = (rng.random(nr) * np.maximum(Hs - R + 1, 1)).astype(np.int64)
+    span = np.minimum(reg_min[read_region], Hs)
+    start = (rng.random(nr) * np.maximum(span - R + 1, 1)).astype(np.int64)
     take = np.minimum(R, Hs - start)
     idx = _ranges(hoff[src] + start, take)
     roff = _offsets(R).astype(np.int64)
