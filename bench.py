@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -157,6 +158,20 @@ def main():
         same = bool(np.array_equal(out_p, out, equal_nan=True))
         log(f"plain: {plain_ms:.2f} ms/step ({plain_ms / ms:.2f}x batched), identical={same}")
 
+    # adjoint: forward + checkpointed reverse sweep on the forward+adjoint
+    # configuration (4,096 pairs, R = 150, H = 400, checkpoint every 32nd step)
+    adj = None
+    if args.adjoint:
+        b3 = synth.config(3)
+        theta = [45.0, 10.0] + [1.0] * 8
+        eng.adjoint(b3, theta, timing=True)
+        times = [eng.adjoint(b3, theta, timing=True)[2] for _ in range(3)]
+        adj_ms = min(times)
+        adj = {"gcups": round(b3.cells() / (adj_ms / 1e3) / 1e9, 2), "kernel_ms": round(adj_ms, 3),
+               "workload": "config3: 4096 pairs, R=150, H=400, ckpt every 32 steps",
+               "cells": b3.cells()}
+        log(f"adjoint: {adj_ms:.2f} ms, {adj['gcups']} GCUPS")
+
     achieved = FLOP_PER_CELL * cells / (fp32_ms / 1e3) / 1e12
     res = {
         "metric": "GCUPS fwd/adjoint, batched vs plain, 1 MI355X, |dlog10L|<=1e-4",
@@ -177,6 +192,7 @@ def main():
                    "parallelism": "single GPU, persistent length-bucketed hipGraph launch"},
         "plain_gcups": None if plain_ms is None else round(cells / (plain_ms / 1e3) / 1e9, 2),
         "batched_over_plain": None if plain_ms is None else round(plain_ms / ms, 3),
+        "adjoint": adj,
         "rescued_pairs": rescued,
         "fp64_rescue_ms": round(fp64_ms, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),

@@ -163,7 +163,9 @@ int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n);
  * Per-row insertion/deletion/gap-continuation qualities are replaced by the
  * two global parameters.  Computes log10 L per pair (FP32, 2^120 scaling) and
  * grad[10] = sum_k weight[k] * d log10 L_k / d(gap_open, gap_cont, scale[0..7]),
- * checkpointing every `ckpt_stride`-th anti-diagonal and recomputing the rest.
+ * checkpointing every `ckpt_stride`-th anti-diagonal and recomputing the rest
+ * (0 = 32).  kernel_ms (may be NULL) receives the summed device time of the
+ * adjoint launches.  Reads longer than 378 bases return UNSUPPORTED.
  */
 typedef struct rodv_phmm_params {
     double gap_open;
@@ -173,7 +175,7 @@ typedef struct rodv_phmm_params {
 
 int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                       const rodv_phmm_params *params, const double *weights, int ckpt_stride,
-                      double *out_log10, double *grad);
+                      double *out_log10, double *grad, double *kernel_ms);
 
 #ifdef __cplusplus
 }

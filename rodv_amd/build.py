@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librodv_phmm.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("phmm_kernels.hip", "phmm_host.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("phmm_kernels.hip", "phmm_adjoint.hip", "phmm_host.cpp")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: only the explicit fmas of the kernels fuse (bit-exact

@@ -1,0 +1,402 @@
+// phmm_adjoint.hip -- hand-written adjoint (reverse sweep) of the PairHMM
+// forward on gfx950.  The reference has no gradient; this is the model of
+// include/rodv_phmm.h (global gap-open delta and gap-continuation gamma,
+// per-quality-bin scales of the base error probability).
+//
+// One pair per wave (L = 64 lanes, MR read rows per lane), the same lane and
+// row layout as the forward (identity rows, row R in the last slot of its
+// lane).  Three passes per pair:
+//  1. forward over all anti-diagonal steps, saving the whole lane state every
+//     `ckpt` steps to HBM (level-1 checkpoints);
+//  2. for each checkpoint segment, last to first: recompute the segment from
+//     its checkpoint, saving the state every 4 steps in LDS (level 2);
+//  3. for each 4-step window, last to first: recompute the 4 states into
+//     registers and run the adjoint of those steps in reverse, lanes reading
+//     the row below their last row from lane l+1 with DPP wave_rol:1.
+// Adjoint seeds are sigma = 2^-ilogb(S) so that adjoint x forward products
+// stay O(expected counts); the gradient of log10 L follows at the end.
+// Per real cell (i, j) with Tb = p * Mb (p the emission prior):
+//   Mb = seed + mm_{i+1} Tb(i+1,j+1) + mx_{i+1} Ib(i+1,j) + my_i Db(i,j+1)
+//   Ib = seed + gapm_{i+1} Tb(i+1,j+1) + xx_{i+1} Ib(i+1,j)
+//   Db = gapm_{i+1} Tb(i+1,j+1) + yy_i Db(i,j+1)
+//   dS/ddelta += -2 Tb M(i-1,j-1) + Ib M(i-1,j) + Db M(i,j-1)
+//   dS/dgamma += -Tb (I(i-1,j-1) + D(i-1,j-1)) + Ib I(i-1,j) + Db D(i,j-1)
+//   dS/de_i   += Mb t(i,j) (match ? -1 : 1/3)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "phmm_device.h"
+#include "phmm_internal.h"
+
+namespace rodv {
+namespace {
+
+using namespace dev;
+constexpr int kL = 64;
+constexpr int kW = 4;  // steps per register window
+
+template <typename T, int MR>
+struct AdjRows {
+    T real[MR];     // 1 for a read row, 0 for an identity row
+    T wq[MR];       // d e_i / d scale = 10^(-q_i/10)
+    uint32_t bin[MR];
+    uint32_t isN[MR];  // N in the read: always the match prior
+    // transitions of the row below the lane's last row (0 across the wrap)
+    T nb_mm, nb_g, nb_mx, nb_xx;
+};
+
+template <typename T> struct PTab;
+template <> struct PTab<float> {
+    static __device__ __forceinline__ float X(const AdjArgs &a, uint32_t q) { return a.xf[q]; }
+    static __device__ __forceinline__ float Y(const AdjArgs &a, uint32_t q) { return a.yf[q]; }
+    static __device__ __forceinline__ float mm(const AdjArgs &a) { return a.mm_f; }
+    static __device__ __forceinline__ float g(const AdjArgs &a) { return a.g_f; }
+    static __device__ __forceinline__ float dl(const AdjArgs &a) { return a.dl_f; }
+    static __device__ __forceinline__ float gm(const AdjArgs &a) { return a.gm_f; }
+    static __device__ __forceinline__ float initd(const AdjArgs &a, uint32_t h) { return a.initd_f[h]; }
+};
+template <> struct PTab<double> {
+    static __device__ __forceinline__ double X(const AdjArgs &a, uint32_t q) { return a.xd[q]; }
+    static __device__ __forceinline__ double Y(const AdjArgs &a, uint32_t q) { return a.yd[q]; }
+    static __device__ __forceinline__ double mm(const AdjArgs &a) { return a.mm_d; }
+    static __device__ __forceinline__ double g(const AdjArgs &a) { return a.g_d; }
+    static __device__ __forceinline__ double dl(const AdjArgs &a) { return a.delta; }
+    static __device__ __forceinline__ double gm(const AdjArgs &a) { return a.gamma; }
+    static __device__ __forceinline__ double initd(const AdjArgs &a, uint32_t h) { return a.initd_d[h]; }
+};
+
+template <typename T, int MR>
+__device__ __forceinline__ void put_state(T *base, int nvec, int k, const State<T, MR> &S) {
+    // element v of state k of this lane at base[(k*nvec + v)*64 + lane]
+    const int lane = threadIdx.x;
+    T *p = base + (size_t)k * nvec * kL + lane;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        p[(3 * s + 0) * kL] = S.M[s];
+        p[(3 * s + 1) * kL] = S.I[s];
+        p[(3 * s + 2) * kL] = S.D[s];
+    }
+    p[(3 * MR + 0) * kL] = S.Mp;
+    p[(3 * MR + 1) * kL] = S.Ip;
+    p[(3 * MR + 2) * kL] = S.Dp;
+}
+
+template <typename T, int MR>
+__device__ __forceinline__ void get_state(const T *base, int nvec, int k, State<T, MR> &S) {
+    const int lane = threadIdx.x;
+    const T *p = base + (size_t)k * nvec * kL + lane;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        S.M[s] = p[(3 * s + 0) * kL];
+        S.I[s] = p[(3 * s + 1) * kL];
+        S.D[s] = p[(3 * s + 2) * kL];
+    }
+    S.Mp = p[(3 * MR + 0) * kL];
+    S.Ip = p[(3 * MR + 1) * kL];
+    S.Dp = p[(3 * MR + 2) * kL];
+}
+
+// Adjoint of one forward step d for one lane.  Sb = state after step d-1,
+// Sa = state after step d.  Adjoint registers hold the lane's values at its
+// previous reverse step (column j+1); Tb0p holds slot 0's Tb one step older.
+template <typename T, int MR>
+__device__ __forceinline__ void adj_step(const State<T, MR> &Sb, const State<T, MR> &Sa,
+                                         const Priors<T, MR> &P, const AdjRows<T, MR> &A,
+                                         uint32_t hc, T seed, T (&Tb)[MR], T (&Ib)[MR],
+                                         T (&Db)[MR], T &Tb0p, T (&gd)[MR], T (&gg)[MR],
+                                         T (&ge)[MR]) {
+    // forward inputs of slot 0 from lane l-1
+    const T dgM0 = ror1<kL>(Sb.Mp), dgI0 = ror1<kL>(Sb.Ip), dgD0 = ror1<kL>(Sb.Dp);
+    const T upM0 = ror1<kL>(Sb.M[MR - 1]), upI0 = ror1<kL>(Sb.I[MR - 1]);
+    // adjoint inputs of the last slot from lane l+1
+    T bT = rol1<kL>(Tb0p);   // Tb(i+1, j+1)
+    T bI = rol1<kL>(Ib[0]);  // Ib(i+1, j)
+    T b_mm = A.nb_mm, b_g = A.nb_g, b_mx = A.nb_mx, b_xx = A.nb_xx;
+    T oldT0 = Tb[0];
+#pragma unroll
+    for (int s = MR - 1; s >= 0; s--) {
+        const T sd = (s == MR - 1) ? seed : T(0);
+        const T oldT = Tb[s];
+        const T Mb = fmaT(P.my[s], Db[s], fmaT(b_mx, bI, fmaT(b_mm, bT, sd)));
+        const T Ibn = fmaT(b_xx, bI, fmaT(b_g, bT, sd));
+        const T Dbn = fmaT(P.yy[s], Db[s], b_g * bT);
+        // forward values around cell (i, j)
+        const T dgM = s == 0 ? dgM0 : Sb.M[s - 1];
+        const T dgI = s == 0 ? dgI0 : Sb.I[s - 1];
+        const T dgD = s == 0 ? dgD0 : Sb.D[s - 1];
+        const T upM = s == 0 ? upM0 : Sa.M[s - 1];
+        const T upI = s == 0 ? upI0 : Sa.I[s - 1];
+        const T lfM = Sb.M[s], lfD = Sb.D[s];
+        const T t = fmaT(P.g[s], dgD, fmaT(P.g[s], dgI, dgM * P.mm[s]));
+        const T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
+        const bool match = (((hc >> P.rpos[s]) & 1u) | A.isN[s]) != 0u;
+        const T Tbn = pr * Mb;
+        gd[s] = fmaT(Dbn, lfM, fmaT(Ibn, upM, fmaT(T(-2) * Tbn, dgM, gd[s])));
+        gg[s] = fmaT(Dbn, lfD, fmaT(Ibn, upI, fmaT(-Tbn, dgI + dgD, gg[s])));
+        ge[s] = fmaT(Mb * t, match ? T(-1) : T(1) / T(3), ge[s]);
+        Tb[s] = Tbn;
+        Ib[s] = Ibn;
+        Db[s] = Dbn;
+        // the slot above reads this slot as "below"
+        bT = oldT;
+        bI = Ibn;
+        b_mm = P.mm[s];
+        b_g = P.g[s];
+        b_mx = P.mx[s];
+        b_xx = P.xx[s];
+    }
+    Tb0p = oldT0;
+}
+
+// Passes 2 and 3 and the reduction, for a pair whose forward sum is usable.
+template <typename T, int MR, bool RESCUE>
+__device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int p, int H, int lR, int D4,
+                                         int nseg, int K1, T *ck, State<T, MR> S,
+                                         const Priors<T, MR> &P, const AdjRows<T, MR> &A, T Ssum,
+                                         const uint8_t *cp) {
+    const int lane = threadIdx.x;
+    const int l = lane;
+    constexpr int NV = 3 * MR + 3;
+    T *l2 = reinterpret_cast<T *>(lds + a.lds_stride);  // level-2 states
+    const int e = ilogb((double)Ssum);
+    const T sigma = (T)ldexp(1.0, -e);
+
+    // ---- passes 2 and 3: reverse sweep
+    T Tb[MR], Ib[MR], Db[MR], gd[MR], gg[MR], ge[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gg[s] = ge[s] = T(0);
+    T Tb0p = T(0);
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int d0 = k * K1;                       // state after step d0
+        const int nsteps = min(K1, D4 - d0);
+        const int nwin = nsteps / kW;
+        get_state(ck, NV, k, S);
+        __syncthreads();
+        for (int w = 0; w < nwin; w++) {
+            put_state(l2, NV, w, S);
+#pragma unroll
+            for (int u = 0; u < kW; u++) step<T, kL, MR, false>(S, P, cp[d0 + w * kW + u + 1]);
+        }
+        __syncthreads();
+        for (int w = nwin - 1; w >= 0; w--) {
+            const int dw = d0 + w * kW;              // window covers steps dw+1 .. dw+4
+            State<T, MR> W0, W1, W2, W3, W4;
+            get_state(l2, NV, w, W0);
+            W1 = W0;
+            step<T, kL, MR, false>(W1, P, cp[dw + 1]);
+            W2 = W1;
+            step<T, kL, MR, false>(W2, P, cp[dw + 2]);
+            W3 = W2;
+            step<T, kL, MR, false>(W3, P, cp[dw + 3]);
+            W4 = W3;
+            step<T, kL, MR, false>(W4, P, cp[dw + 4]);
+            auto seed_at = [&](int d) {
+                const int j = d - l;
+                return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
+            };
+            adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge);
+            adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge);
+            adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge);
+            adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge);
+        }
+    }
+
+    // ---- reduce over read rows and lanes
+    T vd = T(0), vg = T(0), vb[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) vb[b] = T(0);
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        vd = fmaT(A.real[s], gd[s], vd);
+        vg = fmaT(A.real[s], gg[s], vg);
+        const T c = A.real[s] * ge[s] * A.wq[s];
+#pragma unroll
+        for (int b = 0; b < 8; b++) vb[b] += (A.bin[s] == (uint32_t)b) ? c : T(0);
+    }
+    double acc[10];
+    acc[0] = (double)vd;
+    acc[1] = (double)vg;
+#pragma unroll
+    for (int b = 0; b < 8; b++) acc[2 + b] = (double)vb[b];
+#pragma unroll
+    for (int t = 0; t < 10; t++) {
+        double v = acc[t];
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+        acc[t] = v;
+    }
+    if (lane == 0) {
+        // scaled adjoints: acc = sigma * dS/dtheta;  d log10 L = dS / (S ln 10)
+        const double sS = (double)sigma * (double)Ssum;
+        const double ln10 = 2.302585092994045684;
+        const double K_log10 = RESCUE ? 1020.0 * 0.30102999566398119521 : 120.0 * 0.30102999566398119521;
+        a.out_log10[p] = log10((double)Ssum) - K_log10;
+        double *g = a.out_grad + 10 * (uint64_t)p;
+        g[0] = -a.delta * acc[0] / (10.0 * sS);
+        g[1] = -a.gamma * acc[1] / (10.0 * sS);
+        for (int b = 0; b < 8; b++) g[2 + b] = acc[2 + b] / (sS * ln10);
+        if (!RESCUE) a.flags[p] = 0;
+        else a.flags[p] = 0x01;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int src);  // value of lane `src` (uniform)
+template <>
+__device__ __forceinline__ float lane_value(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+template <>
+__device__ __forceinline__ double lane_value(double v, int src) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), src),
+                            __builtin_amdgcn_readlane(__double2loint(v), src));
+}
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// One pair per call.  Every per-pair scalar is made wave-uniform (SGPR) so
+// that all loops and branches below are scalar and the wave stays converged.
+template <typename T, int MR, bool RESCUE>
+__device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
+    const int lane = threadIdx.x;
+    const int l = lane;
+    constexpr int NV = 3 * MR + 3;
+    const int p = uni(p_in);
+    const uint32_t h = (uint32_t)uni((int)a.pair_hap[p]), r = (uint32_t)uni((int)a.pair_read[p]);
+    const int R = uni((int)a.read_len[r]), H = uni((int)a.hap_len[h]);
+    const uint64_t ro64 = a.read_off[r];
+    const uint64_t ro = ((uint64_t)(uint32_t)uni((int)(ro64 >> 32)) << 32) |
+                        (uint64_t)(uint32_t)uni((int)(uint32_t)ro64);
+    const int off = (MR - R % MR) % MR;
+    const int lR = (R - 1 + off) / MR;
+    const T initd = PTab<T>::initd(a, h);
+
+    // hap codes: column j at lds[L-1+j]
+    __syncthreads();
+    for (uint32_t k = lane * 16; k < a.lds_stride; k += 64 * 16)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
+        const int nch = (H + 15) >> 4;
+        for (int c = l; c < nch; c += kL) *reinterpret_cast<uint4 *>(lds + kL + 16 * c) = src[c];
+    }
+    __syncthreads();
+    const uint8_t *cp = lds + (kL - 1) - l;
+
+    Priors<T, MR> P;
+    AdjRows<T, MR> A;
+    State<T, MR> S0;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int i = l * MR + s - off + 1;
+        const bool real = l < kL - 1 && i >= 1 && i <= R;
+        const uint64_t idx = ro + (uint64_t)(real ? i - 1 : 0);
+        const uint32_t qq = a.q[idx] & 127u;
+        uint32_t code = a.rcode[idx];
+        T X = PTab<T>::X(a, qq), Y = PTab<T>::Y(a, qq);
+        const bool isN = code == kReadN;
+        Y = isN ? X : Y;
+        code = isN ? 0u : code;
+        P.mm[s] = real ? PTab<T>::mm(a) : T(0);
+        P.g[s] = real ? PTab<T>::g(a) : T(0);
+        P.mx[s] = real ? PTab<T>::dl(a) : T(0);
+        P.my[s] = real ? PTab<T>::dl(a) : T(0);
+        P.xx[s] = real ? PTab<T>::gm(a) : T(0);
+        P.yy[s] = real ? PTab<T>::gm(a) : T(1);
+        P.X[s] = real ? X : T(0);
+        P.Y[s] = real ? Y : T(0);
+        P.rpos[s] = real ? code : 0u;
+        A.real[s] = real ? T(1) : T(0);
+        A.wq[s] = (T)a.wq[qq];
+        A.bin[s] = a.qbin[qq];
+        A.isN[s] = isN ? 1u : 0u;
+        S0.M[s] = T(0);
+        S0.I[s] = T(0);
+        S0.D[s] = real ? T(0) : initd;
+    }
+    S0.Mp = T(0);
+    S0.Ip = T(0);
+    S0.Dp = S0.D[MR - 1];
+    {
+        // transitions of the row below this lane's last row: lane l+1's slot 0
+        T m = rol1<kL>(P.mm[0]), g = rol1<kL>(P.g[0]), x = rol1<kL>(P.mx[0]),
+          xx = rol1<kL>(P.xx[0]);
+        const bool wrap = l == kL - 1;
+        A.nb_mm = wrap ? T(0) : m;
+        A.nb_g = wrap ? T(0) : g;
+        A.nb_mx = wrap ? T(0) : x;
+        A.nb_xx = wrap ? T(0) : xx;
+    }
+
+    // ---- pass 1: forward with level-1 checkpoints
+    const int Dend = H + lR;                          // lane lR reaches column H
+    const int D4 = (Dend + kW - 1) / kW * kW;         // whole 4-step windows
+    const int K1 = a.ckpt;
+    const int nseg = (D4 + K1 - 1) / K1;
+    T *ck = reinterpret_cast<T *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
+    State<T, MR> S = S0;
+    T sm = T(0), si = T(0);
+    put_state(ck, NV, 0, S);
+    for (int d = 1; d <= D4; d++) {
+        step<T, kL, MR, false>(S, P, cp[d]);
+        if (d <= Dend && d - l <= H) {
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+        }
+        if (d % K1 == 0 && d < D4) put_state(ck, NV, d / K1, S);
+    }
+    const T Ssum = lane_value(sm + si, lR);
+    const bool underflow = RESCUE ? !(Ssum > T(0)) : !((float)Ssum >= 1e-28f);
+    if (underflow) {
+        if (lane == 0) {
+            if (!RESCUE) {  // FP32 underflow: the FP64 instance redoes this pair
+                a.flags[p] = 0x01;
+                const uint32_t k = atomicAdd(a.rescue_count, 1u);
+                a.rescue_list[k] = (uint32_t)p;
+            } else {        // FP64 underflow too: no gradient
+                a.out_log10[p] = -INFINITY;
+                for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)p + t] = 0.0;
+                a.flags[p] = 0x03;
+            }
+        }
+    } else {
+        adj_reverse<T, MR, RESCUE>(a, lds, p, H, lR, D4, nseg, K1, ck, S, P, A, Ssum, cp);
+    }
+}
+
+template <typename T, int MR, bool RESCUE>
+__global__ __launch_bounds__(64) void adj_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t n = RESCUE ? *a.n_items_dev : a.n_items;
+    // a wave takes at most n items: the bound makes every wave exit
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        const int p = RESCUE ? (int)a.rescue_list[it] : a.items[it];
+        adj_item<T, MR, RESCUE>(a, lds, p);
+    }
+}
+
+}  // namespace
+
+int adj_max_mr() { return 6; }
+
+void *adj_kernel_ptr(int MR, bool rescue) {
+#define KA(m)                                                                        \
+    case m:                                                                          \
+        return rescue ? reinterpret_cast<void *>(&adj_kernel<double, m, true>)       \
+                      : reinterpret_cast<void *>(&adj_kernel<float, m, false>);
+    switch (MR) {
+        KA(1) KA(2) KA(3) KA(4) KA(5) KA(6)
+    default: return nullptr;
+    }
+#undef KA
+}
+
+hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a) {
+    void *args[] = {const_cast<AdjArgs *>(&a)};
+    return hipLaunchKernel(kern, dim3(grid), dim3(64), args, lds_bytes, st);
+}
+
+}  // namespace rodv

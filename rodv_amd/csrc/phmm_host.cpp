@@ -23,6 +23,9 @@ int fwd_f64_rescue_mr();
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
+int adj_max_mr();
+void *adj_kernel_ptr(int MR, bool rescue);
+hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
 }  // namespace rodv
 
 using namespace rodv;
@@ -887,15 +890,181 @@ int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
 
 int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                       const rodv_phmm_params *params, const double *weights, int ckpt_stride,
-                      double *out_log10, double *grad) {
-    (void)ctx;
-    (void)batch;
-    (void)params;
-    (void)weights;
-    (void)ckpt_stride;
-    (void)out_log10;
-    (void)grad;
-    return RODV_PHMM_ERR_UNSUPPORTED;
+                      double *out_log10, double *grad, double *kernel_ms) {
+    if (!ctx || !batch || !params || !grad) return RODV_PHMM_ERR_INVALID_ARG;
+    if (ckpt_stride == 0) ckpt_stride = 32;
+    if (ckpt_stride < 4 || ckpt_stride > 256 || ckpt_stride % 4 != 0) return RODV_PHMM_ERR_INVALID_ARG;
+    if (!(params->gap_open > 0.0) || !(params->gap_cont > 0.0)) return RODV_PHMM_ERR_INVALID_ARG;
+    for (int b = 0; b < 8; b++)
+        if (!(params->scale[b] >= 0.0)) return RODV_PHMM_ERR_INVALID_ARG;
+    // upload and validate through the forward planner (same encodings)
+    rodv_phmm_config cfg;
+    rodv_phmm_config_default(&cfg);
+    cfg.use_graph = 0;
+    rodv_phmm_plan *p = nullptr;
+    int rc = rodv_phmm_plan_create(ctx, batch, &cfg, &p);
+    if (rc) return rc;
+    const uint64_t n = p->n_pairs;
+    // per-pair lengths, bucketed by rows per lane (one pair per wave)
+    std::vector<uint32_t> ph(n), pr(n);
+    if (hipMemcpy(ph.data(), p->d_pair_hap, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(pr.data(), p->d_pair_read, n * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+        rodv_phmm_plan_destroy(p);
+        return RODV_PHMM_ERR_DEVICE;
+    }
+    const int maxmr = adj_max_mr();
+    std::vector<std::vector<int32_t>> bucket(maxmr + 1);
+    uint32_t max_h = 0;
+    for (uint64_t k = 0; k < n; k++) {
+        const uint32_t R = batch->read_lengths[pr[k]], H = batch->hap_lengths[ph[k]];
+        const int mr = (int)((R + 62) / 63);
+        if (mr > maxmr) {
+            rodv_phmm_plan_destroy(p);
+            return RODV_PHMM_ERR_UNSUPPORTED;  // reads longer than 63*maxmr rows
+        }
+        bucket[mr].push_back((int32_t)k);
+        max_h = std::max(max_h, H);
+    }
+    // per-quality tables of the parametrised model (double, then rounded)
+    const HostTables &t = tables();
+    const double delta = std::pow(10.0, -params->gap_open / 10.0);
+    const double gamma = std::pow(10.0, -params->gap_cont / 10.0);
+    std::vector<float> xf(128), yf(128);
+    std::vector<double> xd(128), yd(128), wq(128);
+    std::vector<uint8_t> qbin(128);
+    for (int q = 0; q < 128; q++) {
+        const int b = std::min(q, 47) / 6;
+        const double e = params->scale[b] * t.ph2pr_d[q];
+        xd[q] = 1.0 - e;
+        yd[q] = e / 3.0;
+        xf[q] = (float)xd[q];
+        yf[q] = (float)yd[q];
+        wq[q] = t.ph2pr_d[q];
+        qbin[q] = (uint8_t)b;
+    }
+    float *d_xf = nullptr, *d_yf = nullptr;
+    double *d_xd = nullptr, *d_yd = nullptr, *d_wq = nullptr, *d_out = nullptr, *d_grad = nullptr;
+    uint8_t *d_qbin = nullptr, *d_flags = nullptr;
+    int32_t *d_items = nullptr;
+    uint32_t *d_cnt = nullptr, *d_resc = nullptr;
+    void *d_ck = nullptr;
+    std::vector<int32_t> all_items;
+    std::vector<size_t> item_begin(maxmr + 2, 0);
+    for (int mr = 1; mr <= maxmr; mr++) {
+        item_begin[mr] = all_items.size();
+        all_items.insert(all_items.end(), bucket[mr].begin(), bucket[mr].end());
+    }
+    item_begin[maxmr + 1] = all_items.size();
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t)ctx->num_cus * 4));
+    const uint32_t stride = lds_stride_for(max_h, 64);
+    const uint64_t d4max = ((uint64_t)max_h + 63 + 3) / 4 * 4;
+    const uint64_t nseg_max = d4max / 4 + 2;  // enough for any stride >= 4
+    const uint64_t ck_elems = nseg_max * (3ull * maxmr + 3) * 64;
+    int maxlds = 65536;
+    (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+    rc = upload(&d_xf, xf.data(), 128);
+    if (!rc) rc = upload(&d_yf, yf.data(), 128);
+    if (!rc) rc = upload(&d_xd, xd.data(), 128);
+    if (!rc) rc = upload(&d_yd, yd.data(), 128);
+    if (!rc) rc = upload(&d_wq, wq.data(), 128);
+    if (!rc) rc = upload(&d_qbin, qbin.data(), 128);
+    if (!rc) rc = upload(&d_items, all_items.data(), all_items.size());
+    if (!rc) rc = dev_alloc(&d_out, n);
+    if (!rc) rc = dev_alloc(&d_grad, 10 * n);
+    if (!rc) rc = dev_alloc(&d_flags, n);
+    if (!rc) rc = dev_alloc(&d_resc, n);
+    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)(maxmr + 1));
+    if (!rc) rc = dev_alloc(reinterpret_cast<double **>(&d_ck), (size_t)grid * ck_elems);
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (!rc && (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess))
+        rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && launch_reset(d_cnt, 3 * (maxmr + 1), st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && hipEventRecord(ev0, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    for (int mr = 1; mr <= maxmr && !rc; mr++) {
+        const uint32_t cnt = (uint32_t)(item_begin[mr + 1] - item_begin[mr]);
+        if (!cnt) continue;
+        AdjArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.hapc = p->d_hapc;
+        a.hapc_off = p->d_hapc_off;
+        a.hap_len = p->d_hap_len;
+        a.initd_f = p->d_initd_f;
+        a.initd_d = p->d_initd_d;
+        a.rcode = p->d_rcode;
+        a.q = p->d_q;
+        a.read_off = p->d_read_off;
+        a.read_len = p->d_read_len;
+        a.pair_hap = p->d_pair_hap;
+        a.pair_read = p->d_pair_read;
+        a.items = d_items + item_begin[mr];
+        a.n_items = cnt;
+        a.xf = d_xf;
+        a.yf = d_yf;
+        a.xd = d_xd;
+        a.yd = d_yd;
+        a.wq = d_wq;
+        a.qbin = d_qbin;
+        a.delta = delta;
+        a.gamma = gamma;
+        a.mm_d = 1.0 - 2.0 * delta;
+        a.g_d = 1.0 - gamma;
+        a.mm_f = (float)a.mm_d;
+        a.g_f = (float)a.g_d;
+        a.dl_f = (float)delta;
+        a.gm_f = (float)gamma;
+        a.out_log10 = d_out;
+        a.out_grad = d_grad;
+        a.flags = d_flags;
+        a.rescue_list = d_resc + item_begin[mr];
+        a.rescue_count = d_cnt + 3 * mr;
+        a.ck = d_ck;
+        a.ck_elems = ck_elems;
+        a.lds_stride = stride;
+        const uint64_t nv = 3ull * mr + 3;
+        for (int pass = 0; pass < 2 && !rc; pass++) {
+            const size_t esz = pass == 0 ? sizeof(float) : sizeof(double);
+            // level-2 windows live in LDS: shrink the checkpoint stride if needed
+            int K1 = ckpt_stride;
+            while (K1 > 4 && stride + (size_t)(K1 / 4) * nv * 64 * esz > (size_t)maxlds) K1 /= 2;
+            K1 = K1 / 4 * 4;
+            a.ckpt = K1;
+            a.counter = d_cnt + 3 * mr + 1 + pass;
+            a.n_items_dev = pass == 0 ? nullptr : a.rescue_count;
+            const size_t lds = stride + (size_t)(K1 / 4) * nv * 64 * esz;
+            if (launch_adj(adj_kernel_ptr(mr, pass == 1), grid, lds, st, a) != hipSuccess)
+                rc = RODV_PHMM_ERR_DEVICE;
+        }
+    }
+    std::vector<double> out(n), g(10 * n);
+    if (!rc && hipEventRecord(ev1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && hipDeviceSynchronize() != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && kernel_ms) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        *kernel_ms = ms;
+    }
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (!rc && hipMemcpy(out.data(), d_out, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && hipMemcpy(g.data(), d_grad, 10 * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc) {
+        for (int k = 0; k < 10; k++) grad[k] = 0.0;
+        for (uint64_t k = 0; k < n; k++) {
+            const double w = weights ? weights[k] : 1.0;
+            if (w == 0.0) continue;
+            for (int c = 0; c < 10; c++) grad[c] += w * g[10 * k + c];
+        }
+        if (out_log10) std::memcpy(out_log10, out.data(), n * sizeof(double));
+    }
+    void *bufs[] = {d_xf, d_yf, d_xd, d_yd, d_wq, d_qbin, d_items, d_out, d_grad, d_flags,
+                    d_resc, d_cnt, d_ck};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    rodv_phmm_plan_destroy(p);
+    return rc;
 }
 
 }  // extern "C"

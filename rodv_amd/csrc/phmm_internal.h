@@ -76,4 +76,38 @@ struct FinalizeArgs {
     uint64_t n;
 };
 
+// Arguments of the adjoint launches (parametrised model, include/rodv_phmm.h).
+struct AdjArgs {
+    const uint8_t *hapc;
+    const uint64_t *hapc_off;
+    const uint32_t *hap_len;
+    const float *initd_f;
+    const double *initd_d;
+    const uint8_t *rcode;
+    const uint8_t *q;
+    const uint64_t *read_off;
+    const uint32_t *read_len;
+    const uint32_t *pair_hap;
+    const uint32_t *pair_read;
+    const int32_t *items;  // one pair per item
+    uint32_t n_items;
+    const uint32_t *n_items_dev;
+    uint32_t *counter;
+    // per base quality: emission priors 1 - e, e / 3, d e / d scale, bin
+    const float *xf, *yf;
+    const double *xd, *yd, *wq;
+    const uint8_t *qbin;
+    double delta, gamma, mm_d, g_d;
+    float mm_f, g_f, dl_f, gm_f;
+    double *out_log10;
+    double *out_grad;  // 10 per pair
+    uint8_t *flags;
+    uint32_t *rescue_list;
+    uint32_t *rescue_count;
+    void *ck;          // level-1 checkpoints, ck_elems elements of T per block
+    uint64_t ck_elems;
+    uint32_t lds_stride;  // bytes of hap codes before the level-2 states
+    int ckpt;             // level-1 checkpoint stride in steps (multiple of 4)
+};
+
 }  // namespace rodv

@@ -30,75 +30,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "phmm_device.h"
 #include "phmm_internal.h"
 
 namespace rodv {
 namespace {
 
+using namespace dev;
 constexpr int kU = 8;  // steps per unrolled block (even: the register rotation)
 
-// ---------------------------------------------------------------- helpers
-
-template <int L>
-__device__ __forceinline__ int ror1_i(int x) {
-    // row_ror:1 (0x121) rotates within a 16-lane row, wave_ror:1 (0x13C)
-    // across the wave: lane l receives lane l-1, lane 0 receives the last.
-    return __builtin_amdgcn_mov_dpp(x, L == 16 ? 0x121 : 0x13C, 0xf, 0xf, true);
-}
-template <int L>
-__device__ __forceinline__ float ror1(float x) {
-    return __int_as_float(ror1_i<L>(__float_as_int(x)));
-}
-template <int L>
-__device__ __forceinline__ double ror1(double x) {
-    return __hiloint2double(ror1_i<L>(__double2hiint(x)), ror1_i<L>(__double2loint(x)));
-}
-
-__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
-
-// Emission prior: X on a match, Y otherwise.  hc is the one-hot hap code of
-// the lane's current column, rpos the bit of the row's read base.
-__device__ __forceinline__ float select_prior(uint32_t hc, uint32_t rpos, float X, float Y) {
-    int m = __builtin_amdgcn_sbfe((int)hc, rpos, 1);  // 0 or -1
-    return __int_as_float((__float_as_int(X) & m) | (__float_as_int(Y) & ~m));
-}
-__device__ __forceinline__ double select_prior(uint32_t hc, uint32_t rpos, double X, double Y) {
-    return ((hc >> rpos) & 1u) ? X : Y;
-}
-
-template <typename T> struct Tab;
-template <> struct Tab<float> {
-    static __device__ __forceinline__ const float *ph2pr(const DevTables *t) { return t->ph2pr_f; }
-    static __device__ __forceinline__ const float *omp(const DevTables *t) { return t->omp_f; }
-    static __device__ __forceinline__ const float *p3(const DevTables *t) { return t->p3_f; }
-    static __device__ __forceinline__ const float *pt(const DevTables *t) { return t->pt_f; }
-    static __device__ __forceinline__ const float *mm(const DevTables *t) { return t->mm_f; }
-    static __device__ __forceinline__ float initd(const FwdArgs &a, uint32_t h) { return a.initd_f[h]; }
-};
-template <> struct Tab<double> {
-    static __device__ __forceinline__ const double *ph2pr(const DevTables *t) { return t->ph2pr_d; }
-    static __device__ __forceinline__ const double *omp(const DevTables *t) { return t->omp_d; }
-    static __device__ __forceinline__ const double *p3(const DevTables *t) { return t->p3_d; }
-    static __device__ __forceinline__ const double *pt(const DevTables *t) { return t->pt_d; }
-    static __device__ __forceinline__ const double *mm(const DevTables *t) { return t->mm_d; }
-    static __device__ __forceinline__ double initd(const FwdArgs &a, uint32_t h) { return a.initd_d[h]; }
-};
-
-// Per-lane register state: MR rows at the lane's last column, plus the last
-// row one column earlier (read by the next lane as its diagonal).
-template <typename T, int MR>
-struct State {
-    T M[MR], I[MR], D[MR];
-    T Mp, Ip, Dp;
-};
-
-// Per-row transition and emission priors (pairhmm_impl.cpp:88-115).
-template <typename T, int MR>
-struct Priors {
-    T mm[MR], g[MR], mx[MR], xx[MR], yy[MR], my[MR], X[MR], Y[MR];
-    uint32_t rpos[MR];
-};
 
 template <typename T, int MR, bool INTER>
 __device__ __forceinline__ void load_row(const FwdArgs &a, Priors<T, MR> &P, int s, bool real,
@@ -130,45 +70,6 @@ __device__ __forceinline__ void load_row(const FwdArgs &a, Priors<T, MR> &P, int
     P.X[s] = real ? X : T(0);
     P.Y[s] = real ? Y : T(0);
     P.rpos[s] = real ? code : 0u;
-}
-
-// One anti-diagonal step of one lane: all MR rows at the next column.
-template <typename T, int L, int MR, bool INTER>
-__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, uint32_t hc) {
-    // row above the lane's first row: previous lane's last row, at this column
-    // (up) and the column before (diagonal)
-    T dgM = ror1<L>(S.Mp), dgI = ror1<L>(S.Ip), dgD = ror1<L>(S.Dp);
-    T upM = ror1<L>(S.M[MR - 1]), upI = ror1<L>(S.I[MR - 1]);
-#pragma unroll
-    for (int s = 0; s < MR; s++) {
-        T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
-        T t;
-        if (INTER) {  // pairhmm_inter.cpp:279-283 as GCC contracts it
-            t = dgI * P.g[s];
-            t = fmaT(dgM, P.mm[s], t);
-            t = fmaT(dgD, P.g[s], t);
-        } else {  // pairhmm_impl.cpp:266-275
-            t = dgM * P.mm[s];
-            t = fmaT(dgI, P.g[s], t);
-            t = fmaT(dgD, P.g[s], t);
-        }
-        T nM = t * pr;
-        T u = upM * P.mx[s];
-        T nI = fmaT(upI, P.xx[s], u);
-        T v = S.M[s] * P.my[s];
-        T nD = fmaT(S.D[s], P.yy[s], v);
-        dgM = S.M[s];
-        dgI = S.I[s];
-        dgD = S.D[s];
-        upM = nM;
-        upI = nI;
-        S.M[s] = nM;
-        S.I[s] = nI;
-        S.D[s] = nD;
-    }
-    S.Mp = dgM;
-    S.Ip = dgI;
-    S.Dp = dgD;
 }
 
 template <typename T>
@@ -365,7 +266,8 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int G = 64 / L;
     const uint32_t n = RESCUE ? *a.n_items_dev : a.n_items;
-    for (;;) {
+    // a wave takes at most n items: the bound makes every wave exit
+    for (uint32_t iter = 0; iter <= n; iter++) {
         uint32_t it = 0;
         if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
         it = __builtin_amdgcn_readfirstlane(it);

@@ -108,7 +108,7 @@ def load_library(path: str = LIB_PATH):
     L.rodv_phmm_plan_destroy.argtypes = [vp]
     L.rodv_phmm_tables_f32.argtypes = [vp, vp, i32]
     L.rodv_phmm_tables_f64.argtypes = [vp, vp, i32]
-    L.rodv_phmm_adjoint.argtypes = [vp, C.POINTER(CBatch), C.POINTER(Params), vp, i32, vp, vp]
+    L.rodv_phmm_adjoint.argtypes = [vp, C.POINTER(CBatch), C.POINTER(Params), vp, i32, vp, vp, vp]
     for name in EXPORTS:
         getattr(L, name)  # every declared entry point must be exported
     _lib = L
@@ -223,9 +223,10 @@ class Engine:
         result[:] = [list(row) for row in out.reshape(M, N)]
         return True
 
-    def adjoint(self, batch: Batch, theta, weights=None, ckpt_stride=32):
+    def adjoint(self, batch: Batch, theta, weights=None, ckpt_stride=32, timing=False):
         """log10 L per pair and grad[10] = sum_k w_k d log10 L_k / d theta for
-        the parametrised model (include/rodv_phmm.h)."""
+        the parametrised model (include/rodv_phmm.h).  theta = (gap_open,
+        gap_cont, scale[0..7]).  With timing=True also the kernel ms."""
         hold = _CBatchHolder(batch)
         n = batch.num_pairs()
         p = Params()
@@ -233,11 +234,15 @@ class Engine:
         for k in range(8):
             p.scale[k] = float(theta[2 + k])
         w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        if w is not None and len(w) != n:
+            raise ValueError("one weight per pair")
         out = np.zeros(n, np.float64)
         grad = np.zeros(10, np.float64)
+        ms = C.c_double(0.0)
         _check(self._lib.rodv_phmm_adjoint(self._h, C.byref(hold.c), C.byref(p), _ptr(w),
-                                           int(ckpt_stride), _ptr(out), _ptr(grad)), "adjoint")
-        return out, grad
+                                           int(ckpt_stride), _ptr(out), _ptr(grad),
+                                           C.byref(ms)), "adjoint")
+        return (out, grad, ms.value) if timing else (out, grad)
 
 
 class Plan:

@@ -1,0 +1,89 @@
+"""The HIP adjoint against the FP64 oracle gradient, central finite
+differences of the FP64 forward (h = 1e-3), checkpoint-stride invariance and
+the torch.autograd wrapper."""
+import numpy as np
+import pytest
+
+from rodv_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+THETA = np.array([45.0, 10.0, 1.0, 0.9, 1.1, 1.0, 0.95, 1.05, 1.0, 1.2])
+REL = 1e-3
+
+
+def assert_grad_close(g, ref, rel=REL):
+    scale = np.max(np.abs(ref))
+    for k in range(10):
+        assert abs(g[k] - ref[k]) <= rel * max(abs(ref[k]), 1e-3 * scale), (k, g[k], ref[k])
+
+
+def small_batch(seed, **kw):
+    rng = np.random.default_rng(seed)
+    return synth.random_batch(rng, 3, 5, **kw)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_adjoint_matches_fp64_oracle(engine, oracle_mod, seed):
+    b = small_batch(seed, hap_len=(20, 200), read_len=(5, 150), qual=(5, 45))
+    w = np.random.default_rng(seed).normal(size=b.num_pairs())
+    out, g = engine.adjoint(b, THETA, weights=w)
+    o_out, o_g = oracle_mod.param_batch(b, THETA, weights=w, nthreads=8)
+    assert np.max(np.abs(out - o_out)) <= 1e-4
+    assert_grad_close(g, o_g)
+
+
+def test_adjoint_config3_sample(engine, oracle_mod):
+    b = synth.config(3, scale=1 / 32)  # 1 hap x 128 reads, R = 150, H = 400
+    out, g = engine.adjoint(b, THETA)
+    o_out, o_g = oracle_mod.param_batch(b, THETA, nthreads=8)
+    assert np.max(np.abs(out - o_out)) <= 1e-4
+    assert_grad_close(g, o_g)
+
+
+def test_adjoint_vs_central_differences(engine, oracle_mod):
+    b = small_batch(7, hap_len=(30, 120), read_len=(20, 100), qual=(5, 45))
+    _, g = engine.adjoint(b, THETA)
+    h = 1e-3
+    fd = np.zeros(10)
+    for k in range(10):
+        tp, tm = THETA.copy(), THETA.copy()
+        tp[k] += h
+        tm[k] -= h
+        fp, _ = oracle_mod.param_batch(b, tp, nthreads=8)
+        fm, _ = oracle_mod.param_batch(b, tm, nthreads=8)
+        fd[k] = np.sum(fp - fm) / (2 * h)
+    assert_grad_close(g, fd)
+
+
+def test_checkpoint_stride_invariance(engine):
+    b = small_batch(3, hap_len=(100, 300), read_len=(60, 200))
+    ref = None
+    for stride in (4, 8, 16, 32, 64):
+        out, g = engine.adjoint(b, THETA, ckpt_stride=stride)
+        if ref is None:
+            ref = (out, g)
+        else:
+            np.testing.assert_array_equal(out, ref[0])
+            np.testing.assert_allclose(g, ref[1], rtol=1e-9)
+
+
+def test_rescued_pairs_use_fp64(engine, oracle_mod):
+    b = small_batch(4, hap_len=(150, 250), read_len=(120, 200), p_sub=0.75)
+    out, g = engine.adjoint(b, THETA)
+    o_out, o_g = oracle_mod.param_batch(b, THETA, nthreads=8)
+    assert np.max(out) < -40  # these pairs underflow FP32 (log10 L < -64 mostly)
+    assert np.max(np.abs(out - o_out)) <= 1e-4
+    assert_grad_close(g, o_g)
+
+
+def test_torch_autograd_function(engine, oracle_mod):
+    import torch
+    from rodv_amd.autograd import PairHMMLogLik
+    b = small_batch(5)
+    theta = torch.tensor(THETA, dtype=torch.float64, requires_grad=True)
+    ll = PairHMMLogLik.apply(theta, engine, b)
+    w = torch.linspace(-1, 1, ll.numel(), dtype=torch.float64)
+    (ll * w).sum().backward()
+    _, o_g = oracle_mod.param_batch(b, THETA, weights=w.numpy(), nthreads=8)
+    assert_grad_close(theta.grad.numpy(), o_g)
