@@ -96,6 +96,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint", type=int, default=1)
+    ap.add_argument("--prior-table", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -114,7 +115,7 @@ def main():
         f"({time.time() - t0:.1f}s to generate)")
     eng = Engine(0)
     t0 = time.time()
-    plan = eng.plan(batch, default_config(use_graph=1))
+    plan = eng.plan(batch, default_config(use_graph=1, prior_table=args.prior_table))
     log(f"plan + upload {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream

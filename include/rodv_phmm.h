@@ -93,6 +93,8 @@ typedef struct rodv_phmm_config {
     int32_t waves_per_cu;         /* persistent grid size; 0 = autotuned/default */
     int32_t short_lanes;          /* lanes per pair for reads <= 240: 16 or 64  */
     int32_t extended_range;       /* write finite log10 where FP64 underflows   */
+    int32_t prior_table;          /* 1: per-pair LDS table of emission priors for
+                                     N-free haplotypes (default); 0: select per cell */
 } rodv_phmm_config;
 
 /* Counters of the last execution of a plan. */

@@ -19,6 +19,8 @@ namespace rodv {
 // defined in phmm_kernels.hip
 void *fwd_f32_kernel(int L, int MR, bool inter);
 void *fwd_f64_rescue_kernel(bool inter);
+void *fwd_tab_kernel_ptr(int L, int MR, bool inter);
+uint32_t fwd_tab_bytes(int L, int MR);
 int fwd_f64_rescue_mr();
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
@@ -167,6 +169,8 @@ constexpr int kUnroll = 8;  // must match kU in phmm_kernels.hip
 struct Bucket {
     int L = 16, MR = 1;
     bool inter = false;
+    bool tab = false;        // table-prior kernel (N-free haplotypes, one strip)
+    uint32_t tab_bytes = 0;  // LDS prior table per group
     void *kern = nullptr;
     std::vector<int32_t> items;  // G entries per item
     uint32_t n_items = 0;
@@ -347,6 +351,7 @@ void rodv_phmm_config_default(rodv_phmm_config *cfg) {
     cfg->waves_per_cu = 0;
     cfg->short_lanes = 16;
     cfg->extended_range = 0;
+    cfg->prior_table = 1;
 }
 
 int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
@@ -440,13 +445,17 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         hap_bytes += ((uint64_t)b->hap_lengths[h] + 15) & ~15ull;
     }
     std::vector<uint8_t> hapc(hap_bytes, 0);
+    std::vector<uint8_t> hap_has_n(M, 0);
     std::vector<float> initd_f(M);
     std::vector<double> initd_d(M);
     bool ok = true;
     for (uint32_t h = 0; h < M; h++) {
         const uint8_t *s = b->hap_bases + b->hap_offsets[h];
         uint8_t *d = hapc.data() + hapc_off[h];
-        for (uint32_t j = 0; j < b->hap_lengths[h]; j++) d[j] = hap_code(s[j], &ok);
+        for (uint32_t j = 0; j < b->hap_lengths[h]; j++) {
+            d[j] = hap_code(s[j], &ok);
+            hap_has_n[h] |= d[j] == kHapN;
+        }
         initd_f[h] = 0x1p120f / (float)b->hap_lengths[h];   // pairhmm_impl.cpp:62
         initd_d[h] = 0x1p1020 / (double)b->hap_lengths[h];
     }
@@ -494,7 +503,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         }
     }
     // key = L * 64 + MR
-    std::vector<int> bucket_of_key(64 * 2 + 17, -1);
+    std::vector<int> bucket_of_key(2 * 81, -1);
     std::vector<std::vector<uint32_t>> members;
     uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
     for (uint64_t k = 0; k < n_pairs; k++) {
@@ -504,17 +513,19 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         int L, MR;
         choose_shape(R, cfg.short_lanes, &L, &MR);
         // a very long haplotype does not fit four groups' codes in LDS
-        if (L == 16 && 4ull * lds_stride_for(H, 16) > 60 * 1024) {
+        if (L == 16 && 4ull * (2ull * lds_stride_for(H, 16) + fwd_tab_bytes(16, MR)) > 60 * 1024) {
             L = 64;
             MR = 4;
             while (MR < 16 && 63 * MR < (int)R) MR += 4;
         }
-        const int key = (L == 16 ? 0 : 64) + MR;
+        const bool tab = cfg.prior_table && !hap_has_n[ph[k]] && strips_for(R, L, MR) == 1;
+        const int key = (tab ? 81 : 0) + (L == 16 ? 0 : 64) + MR;
         if (bucket_of_key[key] < 0) {
             bucket_of_key[key] = (int)p->buckets.size();
             Bucket bk;
             bk.L = L;
             bk.MR = MR;
+            bk.tab = tab;
             p->buckets.push_back(bk);
             members.emplace_back();
         }
@@ -524,7 +535,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     // ---- items: length-sorted packing of G pairs per wave
     for (size_t bi = 0; bi < p->buckets.size(); bi++) {
         Bucket &bk = p->buckets[bi];
-        bk.kern = fwd_f32_kernel(bk.L, bk.MR, false);
+        bk.kern = bk.tab ? fwd_tab_kernel_ptr(bk.L, bk.MR, false) : fwd_f32_kernel(bk.L, bk.MR, false);
         if (!bk.kern) {
             free_plan(p);
             return RODV_PHMM_ERR_UNSUPPORTED;
@@ -545,7 +556,13 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             bk.max_strips = std::max(bk.max_strips, (uint32_t)strips_for(R, bk.L, bk.MR));
             bk.cells += (uint64_t)R * H;
         }
-        bk.lds_stride = lds_stride_for(bk.max_h, bk.L);
+        if (bk.tab) {
+            // u16 code per column, then the prior table
+            bk.lds_stride = (2u * lds_stride_for(bk.max_h, bk.L) + 15u) & ~15u;
+            bk.tab_bytes = fwd_tab_bytes(bk.L, bk.MR);
+        } else {
+            bk.lds_stride = lds_stride_for(bk.max_h, bk.L);
+        }
         const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu : 8;
         bk.grid = (int)std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus * wpc);
         if (bk.grid < 1) bk.grid = 1;
@@ -723,7 +740,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 a.bnd = bk.d_bnd;
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
-                const size_t lds = (size_t)(64 / bk.L) * bk.lds_stride;
+                const size_t lds = (size_t)(64 / bk.L) * (bk.lds_stride + bk.tab_bytes);
                 HIPCHK(launch_fwd(bk.kern, bk.grid, lds, st, a));
                 launches++;
             }
@@ -740,7 +757,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 a.bnd = bk.d_bnd;
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
-                const size_t lds = (size_t)G * bk.lds_stride;
+                const size_t lds = (size_t)G * (bk.lds_stride + bk.tab_bytes);
                 HIPCHK(launch_fwd(bk.kern, std::min(pl.grid, bk.grid), lds, st, a));
                 launches++;
             }

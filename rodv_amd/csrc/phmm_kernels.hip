@@ -261,6 +261,221 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
     }
 }
 
+// ------------------------------------------------- table-prior variant
+// Same lane/row mapping as process_item, FP32, one strip, haplotypes without
+// N.  The emission prior of every (hap base, lane, row slot) is written once
+// per item into an LDS table; a step reads the lane's MR priors for its
+// column with ds_read_b128 instead of selecting them per cell, which takes
+// the select (2 VALU per cell) out of the hot loop.  The haplotype is staged
+// as u16 byte offsets of its base's table block.
+template <int MR>
+struct TabLayout {
+    static constexpr int MRT = (MR + 3) / 4 * 4;  // slots padded to 16 bytes
+};
+
+template <int L, int MR, bool INTER>
+__device__ __forceinline__ void step_tab(State<float, MR> &S, const Priors<float, MR> &P,
+                                         const float (&pr)[MR]) {
+    float dgM = ror1<L>(S.Mp), dgI = ror1<L>(S.Ip), dgD = ror1<L>(S.Dp);
+    float upM = ror1<L>(S.M[MR - 1]), upI = ror1<L>(S.I[MR - 1]);
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        float t;
+        if (INTER) {
+            t = dgI * P.g[s];
+            t = fmaT(dgM, P.mm[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        } else {
+            t = dgM * P.mm[s];
+            t = fmaT(dgI, P.g[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        }
+        const float nM = t * pr[s];
+        const float u = upM * P.mx[s];
+        const float nI = fmaT(upI, P.xx[s], u);
+        const float v = S.M[s] * P.my[s];
+        const float nD = fmaT(S.D[s], P.yy[s], v);
+        dgM = S.M[s];
+        dgI = S.I[s];
+        dgD = S.D[s];
+        upM = nM;
+        upI = nI;
+        S.M[s] = nM;
+        S.I[s] = nI;
+        S.D[s] = nD;
+    }
+    S.Mp = dgM;
+    S.Ip = dgI;
+    S.Dp = dgD;
+}
+
+template <int L, int MR, bool INTER>
+__device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
+    constexpr int G = 64 / L;
+    constexpr int MRT = TabLayout<MR>::MRT;
+    constexpr uint32_t kBlk = (uint32_t)L * MRT * 4;  // bytes per hap base
+    const int lane = threadIdx.x;
+    const int g = lane / L;
+    const int l = lane % L;
+    const bool valid = p >= 0;
+
+    uint32_t h = 0, r = 0;
+    int R = 0, H = 0;
+    uint64_t ro = 0;
+    if (valid) {
+        h = a.pair_hap[p];
+        r = a.pair_read[p];
+        R = (int)a.read_len[r];
+        H = (int)a.hap_len[h];
+        ro = a.read_off[r];
+    }
+    const int off = valid ? (MR - R % MR) % MR : 0;
+    const int lR = valid ? (R - 1 + off) / MR : 0;
+    const float initd = valid ? a.initd_f[h] : 0.0f;
+
+    // LDS per group: codes (u16, lds_stride bytes) then the prior table
+    const uint32_t gstride = a.lds_stride + 4 * kBlk;
+    uint8_t *gb = lds + g * gstride;
+    uint16_t *codes = reinterpret_cast<uint16_t *>(gb);
+    uint8_t *tab = gb + a.lds_stride;
+
+    __syncthreads();
+    for (uint32_t k = lane * 16; k < (uint32_t)G * gstride; k += 64 * 16)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    if (valid) {
+        // column j (1-based) at codes[L-1+j]: one-hot 1,2,4,8 -> block offset;
+        // 16 columns per lane and iteration from the 16-byte aligned codes
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
+        const int nch = (H + 15) >> 4;
+        for (int c = l; c < nch; c += L) {
+            const uint4 v = src[c];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            uint32_t o[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint32_t b0 = (w[k >> 1] >> (16 * (k & 1))) & 0xffu;
+                const uint32_t b1 = (w[k >> 1] >> (16 * (k & 1) + 8)) & 0xffu;
+                const uint32_t i0 = (b0 >> 1) - (b0 >> 3);  // 1->0 2->1 4->2 8->3
+                const uint32_t i1 = (b1 >> 1) - (b1 >> 3);
+                o[k] = (i0 * kBlk) | ((i1 * kBlk) << 16);
+            }
+            uint4 *dst = reinterpret_cast<uint4 *>(codes + L + 16 * c);
+            dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+            dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+        }
+    }
+
+    Priors<float, MR> P;
+    float X[MR], Y[MR];
+    uint32_t rc[MR];
+    bool realv[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int pos = l * MR + s;
+        const int i = pos - off + 1;
+        const bool real = valid && l < L - 1 && i >= 1 && i <= R;
+        load_row<float, MR, INTER>(a, P, s, real, ro + (uint64_t)(real ? i - 1 : 0));
+        X[s] = P.X[s];
+        Y[s] = P.Y[s];
+        rc[s] = P.rpos[s];
+        realv[s] = real;
+    }
+    // table: block c, lane l, slot s -> prior of row s when the hap base is c
+    float *tl = reinterpret_cast<float *>(tab + (uint32_t)l * MRT * 4);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int s = 0; s < MR; s++)
+            tl[c * (kBlk / 4) + s] = (rc[s] == (uint32_t)c) ? X[s] : Y[s];
+    }
+    __syncthreads();
+
+    State<float, MR> S;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        S.M[s] = 0.0f;
+        S.I[s] = 0.0f;
+        S.D[s] = realv[s] ? 0.0f : initd;  // identity rows carry K/H
+    }
+    S.Mp = 0.0f;
+    S.Ip = 0.0f;
+    S.Dp = S.D[MR - 1];
+
+    const int key = valid ? H + lR : 0x7fffffff;
+    const int keymax = valid ? H + lR : -1;
+    const int Dmain = wave_min4(key, G);
+    const int Dend = wave_max4(keymax, G);
+    const uint16_t *cl = codes + (L - 1) - l;
+    const uint8_t *tlane = tab + (uint32_t)l * MRT * 4;
+    float sm = 0.0f, si = 0.0f;
+    int d = 1;
+    for (; d + kU - 1 <= Dmain; d += kU) {
+        uint32_t co[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) co[u] = cl[d + u];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const float4 *tp = reinterpret_cast<const float4 *>(tlane + co[u]);
+            float pr[MR];
+#pragma unroll
+            for (int q = 0; q < MRT / 4; q++) {
+                const float4 v = tp[q];
+                if (4 * q + 0 < MR) pr[4 * q + 0] = v.x;
+                if (4 * q + 1 < MR) pr[4 * q + 1] = v.y;
+                if (4 * q + 2 < MR) pr[4 * q + 2] = v.z;
+                if (4 * q + 3 < MR) pr[4 * q + 3] = v.w;
+            }
+            step_tab<L, MR, INTER>(S, P, pr);
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+        }
+    }
+    for (; d <= Dend; d++) {
+        const float4 *tp = reinterpret_cast<const float4 *>(tlane + cl[d]);
+        float pr[MR];
+#pragma unroll
+        for (int q = 0; q < MRT / 4; q++) {
+            const float4 v = tp[q];
+            if (4 * q + 0 < MR) pr[4 * q + 0] = v.x;
+            if (4 * q + 1 < MR) pr[4 * q + 1] = v.y;
+            if (4 * q + 2 < MR) pr[4 * q + 2] = v.z;
+            if (4 * q + 3 < MR) pr[4 * q + 3] = v.w;
+        }
+        step_tab<L, MR, INTER>(S, P, pr);
+        if (d - l <= H) {
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+        }
+    }
+    if (valid && l == lR) {
+        const float Ssum = sm + si;  // pairhmm_impl.cpp:434-437
+        a.raw32[p] = Ssum;
+        uint8_t fl = 0;
+        if (Ssum < 1e-28f) {  // MIN_ACCEPTED, pairhmm_api.cpp:4,17
+            fl = 0x01;
+            const uint32_t k = atomicAdd(a.rescue_count, 1u);
+            a.rescue_list[k] = (uint32_t)p;
+        }
+        a.flags[p] = fl;
+    }
+}
+
+template <int L, int MR, bool INTER>
+__global__ __launch_bounds__(64) void fwd_tab_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int G = 64 / L;
+    const uint32_t n = a.n_items;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        const int p = a.items[(uint64_t)it * G + threadIdx.x / L];
+        process_item_tab<L, MR, INTER>(a, lds, p);
+    }
+}
+
 template <typename T, int L, int MR, bool INTER, bool RESCUE>
 __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
@@ -347,6 +562,31 @@ void *fwd_f32_kernel(int L, int MR, bool inter) {
 #undef K64
     return nullptr;
 }
+
+// Table-prior FP32 kernels (haplotypes without N, one strip).
+void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
+#define T16(m) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<16, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<16, m, false>);
+#define T64(m) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<64, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<64, m, false>);
+    if (L == 16) {
+        switch (MR) {
+            T16(1) T16(2) T16(3) T16(4) T16(5) T16(6) T16(7) T16(8)
+            T16(9) T16(10) T16(11) T16(12) T16(13) T16(14) T16(15) T16(16)
+        default: return nullptr;
+        }
+    }
+    if (L == 64) {
+        switch (MR) {
+            T64(4) T64(8) T64(12) T64(16)
+        default: return nullptr;
+        }
+    }
+#undef T16
+#undef T64
+    return nullptr;
+}
+uint32_t fwd_tab_bytes(int L, int MR) { return 4u * (uint32_t)L * (uint32_t)((MR + 3) / 4 * 4) * 4u; }
 
 // FP64 rescue kernel: one pair per wave, 4 rows per lane (255-row strips).
 void *fwd_f64_rescue_kernel(bool inter) {

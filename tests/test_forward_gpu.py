@@ -69,13 +69,14 @@ def test_assemble_log_regions(engine, oracle_mod):
     check_against_oracle(oracle_mod, engine, b)
 
 
+@pytest.mark.parametrize("table", [1, 0])
 @pytest.mark.parametrize("lanes", [16, 64])
 @pytest.mark.parametrize("seed,rlo,rhi", [(1, 1, 20), (2, 10, 120), (3, 100, 240), (4, 200, 400),
                                           (5, 230, 260)])
-def test_random_lengths(engine, oracle_mod, seed, rlo, rhi, lanes):
+def test_random_lengths(engine, oracle_mod, seed, rlo, rhi, lanes, table):
     rng = np.random.default_rng(seed)
     b = synth.random_batch(rng, 7, 23, hap_len=(1, 420), read_len=(rlo, rhi))
-    check_against_oracle(oracle_mod, engine, b, short_lanes=lanes)
+    check_against_oracle(oracle_mod, engine, b, short_lanes=lanes, prior_table=table)
 
 
 def test_every_rows_per_lane_shape(engine, oracle_mod):
@@ -115,6 +116,7 @@ def test_fp64_underflow_long_reads(engine, oracle_mod):
 @pytest.mark.parametrize("cfg", [0, 1])
 def test_configs(engine, oracle_mod, cfg):
     check_against_oracle(oracle_mod, engine, synth.config(cfg))
+    check_against_oracle(oracle_mod, engine, synth.config(cfg), prior_table=0)
 
 
 def test_config2_subsample(engine, oracle_mod):
