@@ -22,6 +22,13 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
 FLOP_PER_CELL = 12        # 8 mul + 4 add per forward cell (pairhmm_impl.cpp:266-289)
+FP_INSTR_PER_CELL = 8     # non-packed FP32 instructions of the bit-exact FMA split
+
+# HBM bytes per FP32 forward pass (all length buckets), from two rocprofv3
+# --pmc runs of this script on the same workload: FETCH_SIZE x 2.000 +
+# WRITE_SIZE x 1.000, the factors a 1 GiB device copy in the same run gives
+# (--copy-calibration-mib 1024).  Keyed by (config, scale).
+MEASURED_TRAFFIC = {(2, 1.0): 2.5104e10}
 
 
 def log(msg):
@@ -86,17 +93,48 @@ def cpu_baseline(batch, seconds=12.0, seed=7):
                       f"{cells:.3e} cells in {dt:.1f} s; host {_cpu_model()}"}
 
 
+def valu_ceiling():
+    """FP32 issue ceiling of this device (rodv_amd/libvalu_ceiling.so): lane
+    operations per second of independent v_fma_f32, v_mul_f32 and of the
+    forward cell's own 8-instruction mix, at 8 waves per SIMD; the last one
+    divided by 8 is the cells/s a bit-exact non-packed forward can reach."""
+    import ctypes
+    path = os.path.join(ROOT, "rodv_amd", "libvalu_ceiling.so")
+    if not os.path.exists(path):
+        log("libvalu_ceiling.so not built; no issue ceiling")
+        return None
+    lib = ctypes.CDLL(path)
+    lib.valu_ceiling.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for kind, name in ((0, "fma_lane_ops_per_s"), (1, "mul_lane_ops_per_s"),
+                       (2, "cell_mix_lane_ops_per_s")):
+        v = ctypes.c_double(0.0)
+        rc = lib.valu_ceiling(0, kind, 8, 3, ctypes.byref(v))
+        if rc:
+            log(f"valu_ceiling kind {kind} failed ({rc})")
+            return None
+        out[name] = round(v.value, 1)
+    out["cells_per_s"] = round(out["cell_mix_lane_ops_per_s"] / FP_INSTR_PER_CELL, 1)
+    out["tflops_at_12_flop_per_cell"] = round(out["cells_per_s"] * FLOP_PER_CELL / 1e12, 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint", type=int, default=1)
     ap.add_argument("--prior-table", type=int, default=1)
+    ap.add_argument("--waves-per-cu", type=int, default=0,
+                    help="persistent grid size per CU (0 = engine default)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device copy of this size (reference for PMC byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -115,25 +153,40 @@ def main():
         f"({time.time() - t0:.1f}s to generate)")
     eng = Engine(0)
     t0 = time.time()
-    plan = eng.plan(batch, default_config(use_graph=1, prior_table=args.prior_table))
+    plan = eng.plan(batch, default_config(use_graph=1, prior_table=args.prior_table,
+                                         waves_per_cu=args.waves_per_cu))
     log(f"plan + upload {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
+    if args.copy_calibration_mib:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel
+        # calibrate the counters of the PairHMM kernels in the same run
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {n * 4} bytes read, {n * 4} bytes written")
+        del src, dst
 
     for _ in range(args.warmup):
         plan.execute(sh)
     torch.cuda.synchronize()
     log("warmup done")
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     ev0.record(stream)
-    for _ in range(args.steps):
+    evs[0].record(stream)
+    for k in range(args.steps):
         plan.execute(sh)
+        evs[k + 1].record(stream)
     ev1.record(stream)
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1) / args.steps
+    median_ms = float(np.median([evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]))
     gcups = cells / (ms / 1e3) / 1e9
-    log(f"batched: {ms:.2f} ms/step, {gcups:.1f} GCUPS")
+    log(f"batched: {ms:.2f} ms/step (median replay {median_ms:.2f} ms), {gcups:.1f} GCUPS")
 
     # per-kernel times with hipEvents on the launch stream
     profs = [plan.profile(sh) for _ in range(2)]
@@ -174,6 +227,14 @@ def main():
         log(f"adjoint: {adj_ms:.2f} ms, {adj['gcups']} GCUPS")
 
     achieved = FLOP_PER_CELL * cells / (fp32_ms / 1e3) / 1e12
+    ceiling = valu_ceiling()
+    if ceiling:
+        cps = cells / (fp32_ms / 1e3)
+        ceiling["forward_cells_per_s"] = round(cps, 1)
+        ceiling["frac_of_cell_mix_ceiling"] = round(cps / ceiling["cells_per_s"], 4)
+        log(f"VALU ceiling: {ceiling}")
+    traffic = MEASURED_TRAFFIC.get((args.config, args.scale)) \
+        if (args.waves_per_cu == 0 and args.prior_table) else None
     res = {
         "metric": "GCUPS fwd/adjoint, batched vs plain, 1 MI355X, |dlog10L|<=1e-4",
         "value": round(gcups, 2),
@@ -196,11 +257,14 @@ def main():
         "adjoint": adj,
         "rescued_pairs": rescued,
         "fp64_rescue_ms": round(fp64_ms, 3),
+        "median_replay_ms": round(median_ms, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "fwd_kernel<float> (all length buckets)",
-                     "kernel_ms": round(fp32_ms, 3)},
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "kernel": "fwd_tab_kernel<float> (all length buckets, one FP32 pass)"
+                     if args.prior_table else "fwd_kernel<float> (all length buckets)",
+                     "kernel_ms": round(fp32_ms, 3),
+                     "valu_issue_ceiling": ceiling},
     }
     if not args.no_cpu_baseline:
         t0 = time.time()

@@ -1,6 +1,7 @@
 """Builds the engine's shared library (HIP for gfx950) in-tree.
 
     python -m rodv_amd.build        -> rodv_amd/librodv_phmm.so
+                                       rodv_amd/libvalu_ceiling.so (benchmark only)
 """
 from __future__ import annotations
 
@@ -12,6 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librodv_phmm.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("phmm_kernels.hip", "phmm_adjoint.hip", "phmm_host.cpp")]
+# FP32 VALU issue-ceiling probe used by bench.py's roofline (not part of the engine)
+CEILING_LIB = os.path.join(HERE, "libvalu_ceiling.so")
+CEILING_SOURCES = [os.path.join(HERE, "csrc", "valu_ceiling.hip")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: only the explicit fmas of the kernels fuse (bit-exact
@@ -24,16 +28,23 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
 
 
 def build(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *SOURCES, "-o", tmp]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    errs = "\n".join(l for l in (res.stderr or "").splitlines()
-                     if "packed-fp32-ops" not in l)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{errs}")
-    if verbose and errs.strip():
-        print(errs, file=sys.stderr)
-    os.replace(tmp, LIB)
+    jobs = []
+    for out, srcs in ((LIB, SOURCES), (CEILING_LIB, CEILING_SOURCES)):
+        cmd = [HIPCC, *FLAGS, *srcs, "-o", out + ".tmp"]
+        jobs.append((out, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                           text=True)))
+    failed = []
+    for out, proc in jobs:
+        _, stderr = proc.communicate()
+        errs = "\n".join(l for l in (stderr or "").splitlines() if "packed-fp32-ops" not in l)
+        if proc.returncode != 0:
+            failed.append(f"hipcc -> {os.path.basename(out)} failed ({proc.returncode}):\n{errs}")
+            continue
+        if verbose and errs.strip():
+            print(errs, file=sys.stderr)
+        os.replace(out + ".tmp", out)
+    if failed:
+        raise RuntimeError("\n".join(failed))
     return LIB
 
 
