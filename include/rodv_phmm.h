@@ -75,10 +75,16 @@ typedef struct rodv_phmm_batch {
 
 /* Which reference kernel's FP32 arithmetic to reproduce bit for bit. */
 enum {
-    RODV_PHMM_SEMANTICS_INTRA = 0,   /* intra::computeLikelihoods for every pair */
-    RODV_PHMM_SEMANTICS_SCHEDULE = 1 /* schedule_pairhmm routing: pairs its greedy
-                                        grouping sends to the inter kernel use the
-                                        inter priors and FMA order */
+    RODV_PHMM_SEMANTICS_INTRA = 0,   /* intra::computeLikelihoods for every pair
+                                        (pairhmm_api.cpp:14-35) */
+    RODV_PHMM_SEMANTICS_SCHEDULE = 1 /* schedule_pairhmm routing (default): per
+                                        region, the greedy grouping of
+                                        pairhmm_schedule.cpp:46-105 sends windows
+                                        of simd_width_float pairs to the inter
+                                        arithmetic (pairhmm_inter.cpp), regroups
+                                        their FP32 rejections in windows of
+                                        simd_width_double for inter FP64, and
+                                        scores every other pair as intra */
 };
 
 typedef struct rodv_phmm_config {
@@ -87,10 +93,12 @@ typedef struct rodv_phmm_config {
     int32_t use_double;           /* schedule_pairhmm's use_double flag         */
     double max_idle_ratio_float;  /* schedule_pairhmm's idle bounds (0.1)       */
     double max_idle_ratio_double;
-    uint32_t simd_width_float;    /* reference host's group widths: 16 (AVX-512) */
-    uint32_t simd_width_double;   /* or 8/4 (AVX2); used by SEMANTICS_SCHEDULE  */
+    uint32_t simd_width_float;    /* group widths of the reference's host: the  */
+    uint32_t simd_width_double;   /* default is this host's (16/8 AVX-512, 8/4
+                                     AVX2, 0/0 neither: cpu_features.h:53-154);
+                                     at most 64 */
     int32_t use_graph;            /* capture the batched launch in a hipGraph   */
-    int32_t waves_per_cu;         /* persistent grid size; 0 = autotuned/default */
+    int32_t waves_per_cu;         /* persistent grid size; 0 = engine default   */
     int32_t short_lanes;          /* lanes per pair for reads <= 240: 16 or 64  */
     int32_t extended_range;       /* write finite log10 where FP64 underflows   */
     int32_t prior_table;          /* 1: per-pair LDS table of emission priors for

@@ -6,8 +6,10 @@
 // Contract kept from pairhmm_schedule.cpp:116-340:
 //  - returns false when there are no haplotypes or no reads;
 //  - result is resized to M rows (haplotypes) of N log10 likelihoods;
-//  - use_double / idle ratios keep their meaning (they only steer which
-//    reference kernel's rounding is reproduced, see rodv_phmm_config).
+//  - use_double / idle ratios keep their meaning: the engine routes pairs
+//    between the inter and intra arithmetic exactly as the reference's
+//    greedy grouping does, with the group widths of this host's CPU
+//    (RODV_PHMM_SEMANTICS_SCHEDULE, the default of rodv_phmm_config_default).
 // Differences: invalid input (bases outside ACGTN, zero lengths, quality
 // strings of the wrong length) returns false instead of undefined behaviour,
 // and nothing is printed unless `verbose`.
@@ -104,9 +106,13 @@ inline bool schedule_pairhmm(const std::vector<std::vector<uint8_t>> &haplotypes
     result.resize(M);
     for (size_t h = 0; h < M; h++) result[h].assign(out.begin() + h * N, out.begin() + (h + 1) * N);
     if (verbose) {
-        size_t rescued = 0;
-        for (uint8_t f : flags) rescued += (f & RODV_PHMM_FLAG_RESCUED) != 0;
-        std::fprintf(stderr, "Total num pairs: %zu, rescued in FP64: %zu\n", M * N, rescued);
+        size_t rescued = 0, inter = 0;
+        for (uint8_t f : flags) {
+            rescued += (f & RODV_PHMM_FLAG_RESCUED) != 0;
+            inter += (f & RODV_PHMM_FLAG_INTER) != 0;
+        }
+        std::fprintf(stderr, "Total num pairs: %zu, inter: %zu, rescued in FP64: %zu\n", M * N,
+                     inter, rescued);
     }
     return true;
 }

@@ -55,7 +55,7 @@ def lib():
         L.phmm_oracle_schedule.argtypes = [
             C.c_uint32, C.POINTER(_u8p), _u32p, C.c_uint32, C.POINTER(_u8p),
             C.POINTER(_u8p), C.POINTER(_u8p), C.POINTER(_u8p), C.POINTER(_u8p), _u32p,
-            C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, _f64p, _u8p]
+            C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, _f64p, _u8p, _f32p, _u8p]
         L.phmm_oracle_param_batch.restype = C.c_int
         L.phmm_oracle_param_batch.argtypes = [
             C.c_uint32, _u8p, _u64p, _u32p, C.c_uint32, _u8p, _u8p, _u64p, _u32p,
@@ -108,10 +108,14 @@ def run_batch(batch, pairs=None, mode=MODE_INTRA_API, semantics=INTRA, nthreads=
     return out, r32, r64, fl
 
 
+ROUTE_INTRA, ROUTE_INTER32, ROUTE_INTER64, ROUTE_INTRA64 = 0, 1, 2, 3
+
+
 def schedule(haps, reads, quals, ins, dels, gcps, use_double=False, idle_f=0.1, idle_d=0.1,
-             wf=16, wd=8):
+             wf=16, wd=8, details=False):
     """schedule_pairhmm restatement over lists of uint8 arrays; returns the
-    M x N result matrix and the flags."""
+    M x N result matrix and the flags (and, with details, the deciding FP32
+    raw sums and the ROUTE_* of every pair)."""
     M, N = len(haps), len(reads)
     keep = []
 
@@ -127,10 +131,40 @@ def schedule(haps, reads, quals, ins, dels, gcps, use_double=False, idle_f=0.1, 
     rl = np.array([len(r) for r in reads], np.uint32)
     res = np.zeros(max(M * N, 1), np.float64)
     fl = np.zeros(max(M * N, 1), np.uint8)
+    r32 = np.zeros(max(M * N, 1), np.float32)
+    route = np.zeros(max(M * N, 1), np.uint8)
     ok = lib().phmm_oracle_schedule(
         M, arr(haps), _p(hl, _u32p), N, arr(reads), arr(quals), arr(ins), arr(dels), arr(gcps),
-        _p(rl, _u32p), int(use_double), idle_f, idle_d, wf, wd, _p(res, _f64p), _p(fl, _u8p))
-    return bool(ok), res[:M * N].reshape(M, N), fl[:M * N].reshape(M, N)
+        _p(rl, _u32p), int(use_double), idle_f, idle_d, wf, wd, _p(res, _f64p), _p(fl, _u8p),
+        _p(r32, _f32p), _p(route, _u8p))
+    out = (bool(ok), res[:M * N].reshape(M, N), fl[:M * N].reshape(M, N))
+    if details:
+        out += (r32[:M * N].reshape(M, N), route[:M * N].reshape(M, N))
+    return out
+
+
+def schedule_batch(batch, use_double=False, idle_f=0.1, idle_d=0.1, wf=16, wd=8, nthreads=8):
+    """One schedule_pairhmm restatement call per region of `batch`; flat
+    arrays in the engine's output order (region by region, hap-major):
+    (log10, flags, raw32, route)."""
+    from concurrent.futures import ThreadPoolExecutor
+    regs = batch.regions if len(batch.regions) else \
+        np.array([[0, len(batch.hap_len), 0, len(batch.read_len)]], np.int64)
+
+    def one(reg):
+        h0, m, r0, n = (int(x) for x in reg)
+        haps = [np.frombuffer(batch.hap(k), np.uint8) for k in range(h0, h0 + m)]
+        reads = [np.frombuffer(batch.read(k), np.uint8) for k in range(r0, r0 + n)]
+        qs = [batch.quals(k) for k in range(r0, r0 + n)]
+        _, res, fl, r32, route = schedule(haps, reads, [q[0] for q in qs], [q[1] for q in qs],
+                                          [q[2] for q in qs], [q[3] for q in qs],
+                                          use_double=use_double, idle_f=idle_f, idle_d=idle_d,
+                                          wf=wf, wd=wd, details=True)
+        return res.ravel(), fl.ravel(), r32.ravel(), route.ravel()
+
+    with ThreadPoolExecutor(max(1, nthreads)) as ex:
+        parts = list(ex.map(one, regs))
+    return tuple(np.concatenate([p[k] for p in parts]) for k in range(4))
 
 
 class Params(C.Structure):

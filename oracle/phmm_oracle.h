@@ -77,10 +77,19 @@ int phmm_oracle_batch(uint32_t num_haps, const uint8_t *hap_bases, const uint64_
                       int mode, int semantics, int nthreads, double *out_log10,
                       float *out_raw32, double *out_raw64, uint8_t *out_flags);
 
+/* Route a pair takes through schedule_pairhmm (pairhmm_schedule.cpp:185-335). */
+#define PHMM_ORACLE_ROUTE_INTRA 0u   /* intra kernel, FP32 first               */
+#define PHMM_ORACLE_ROUTE_INTER32 1u /* inter FP32 group, accepted             */
+#define PHMM_ORACLE_ROUTE_INTER64 2u /* inter FP32 rejected, inter FP64 group  */
+#define PHMM_ORACLE_ROUTE_INTRA64 3u /* inter FP32 rejected, intra FP64 only   */
+
 /*
  * schedule_pairhmm restatement for one region (M haps x N reads), result
  * hap-major like pairhmm_schedule.cpp:135-138.  simd_width_f / _d are the
  * group widths the host CPU would use (16/8 with AVX-512, 8/4 with AVX2).
+ * raw32 (may be NULL) receives the FP32 sum that decided the pair's rescue
+ * (inter arithmetic for routes 1-3, intra for route 0), route (may be NULL)
+ * the PHMM_ORACLE_ROUTE_* of each pair.
  * Returns 0 for M == 0 or N == 0 (the reference's `false`), 1 otherwise.
  */
 int phmm_oracle_schedule(uint32_t M, const uint8_t *const *haps, const uint32_t *hlen,
@@ -88,7 +97,8 @@ int phmm_oracle_schedule(uint32_t M, const uint8_t *const *haps, const uint32_t 
                          const uint8_t *const *ins, const uint8_t *const *del,
                          const uint8_t *const *gcp, const uint32_t *rlen, int use_double,
                          double idle_f, double idle_d, uint32_t simd_width_f,
-                         uint32_t simd_width_d, double *result, uint8_t *flags);
+                         uint32_t simd_width_d, double *result, uint8_t *flags, float *raw32,
+                         uint8_t *route);
 
 /*
  * Parametrised model used by the adjoint (the reference has no gradient):

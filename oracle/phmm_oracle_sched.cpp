@@ -74,7 +74,7 @@ extern "C" int phmm_oracle_schedule(uint32_t M, const uint8_t *const *haps, cons
                                     const uint8_t *const *del, const uint8_t *const *gcp,
                                     const uint32_t *rlen, int use_double, double idle_f,
                                     double idle_d, uint32_t Wf, uint32_t Wd, double *result,
-                                    uint8_t *flags) {
+                                    uint8_t *flags, float *raw32, uint8_t *route) {
     if (M == 0 || N == 0) return 0;
     auto mkpair = [&](size_t h, size_t r) {
         phmm_oracle_pair p;
@@ -91,6 +91,7 @@ extern "C" int phmm_oracle_schedule(uint32_t M, const uint8_t *const *haps, cons
     for (size_t k = 0; k < (size_t)M * N; k++) {
         result[k] = 0.0;
         if (flags) flags[k] = 0;
+        if (route) route[k] = PHMM_ORACLE_ROUTE_INTRA;
     }
     std::vector<Pair> pairs;
     pairs.reserve((size_t)M * N);
@@ -112,12 +113,15 @@ extern "C" int phmm_oracle_schedule(uint32_t M, const uint8_t *const *haps, cons
                 phmm_oracle_pair p = mkpair(pr.hap_idx, pr.read_idx);
                 float raw = phmm_oracle_raw_f32(&p, PHMM_ORACLE_INTER);
                 size_t o = pr.hap_idx * N + pr.read_idx;
+                if (raw32) raw32[o] = raw;
                 if (raw < kMinAccepted) {
                     pr.used = false;
+                    if (route) route[o] = PHMM_ORACLE_ROUTE_INTRA64;
                 } else {
                     pr.used = true;
                     result[o] = std::log10((double)raw) - (double)kLog10KF;
                     if (flags) flags[o] |= PHMM_ORACLE_FLAG_INTER;
+                    if (route) route[o] = PHMM_ORACLE_ROUTE_INTER32;
                     done_d[o] = true;
                 }
                 done_f[o] = true;
@@ -134,6 +138,7 @@ extern "C" int phmm_oracle_schedule(uint32_t M, const uint8_t *const *haps, cons
                 double raw = phmm_oracle_raw_f64(&p, PHMM_ORACLE_INTER);
                 size_t o = pr.hap_idx * N + pr.read_idx;
                 result[o] = std::log10(raw) - kLog10KD;
+                if (route) route[o] = PHMM_ORACLE_ROUTE_INTER64;
                 if (flags) {
                     flags[o] |= PHMM_ORACLE_FLAG_INTER | PHMM_ORACLE_FLAG_RESCUED;
                     if (!(raw > 0.0)) flags[o] |= PHMM_ORACLE_FLAG_UNDERFLOW;
@@ -148,7 +153,9 @@ extern "C" int phmm_oracle_schedule(uint32_t M, const uint8_t *const *haps, cons
             if (done_d[o]) continue;
             phmm_oracle_pair p = mkpair(h, r);
             uint8_t fl = 0;
-            result[o] = phmm_oracle_intra_log10(&p, done_f[o] ? 1 : 0, nullptr, nullptr, &fl);
+            float r32 = 0.0f;
+            result[o] = phmm_oracle_intra_log10(&p, done_f[o] ? 1 : 0, &r32, nullptr, &fl);
+            if (raw32 && !done_f[o]) raw32[o] = r32;
             if (flags) flags[o] |= fl;
         }
     }

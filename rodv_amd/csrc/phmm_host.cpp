@@ -1,6 +1,7 @@
 // phmm_host.cpp -- host side of the MI355X PairHMM engine: prior tables,
 // validation, length bucketing, device upload, launch sequencing, hipGraph
 // capture and the C ABI declared in include/rodv_phmm.h.
+#include <cpuid.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -10,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/rodv_phmm.h"
@@ -25,6 +27,7 @@ int fwd_f64_rescue_mr();
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
+hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st);
 int adj_max_mr();
 void *adj_kernel_ptr(int MR, bool rescue);
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
@@ -220,6 +223,10 @@ struct rodv_phmm_plan {
     uint32_t *d_read_len = nullptr;
     uint32_t *d_pair_hap = nullptr, *d_pair_read = nullptr;
     uint8_t *d_pair_inter = nullptr;
+    // schedule_pairhmm routing: per region, its FP32 inter pairs in sorted order
+    uint32_t *d_sched_list = nullptr;
+    uint64_t *d_sched_off = nullptr;
+    uint32_t n_regions = 0;
     // device outputs / scratch
     float *d_raw32 = nullptr;
     double *d_raw64 = nullptr;
@@ -263,7 +270,7 @@ void free_plan(rodv_phmm_plan *p) {
                     p->d_rcode, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
                     p->d_read_len, p->d_pair_hap, p->d_pair_read, p->d_pair_inter,
                     p->d_raw32, p->d_raw64, p->d_out, p->d_flags, p->d_rescue,
-                    p->d_counters, p->d_bnd64};
+                    p->d_counters, p->d_bnd64, p->d_sched_list, p->d_sched_off};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &b : p->buckets) {
@@ -299,6 +306,89 @@ int strips_for(uint32_t R, int L, int MR) {
     const int C = (L - 1) * MR;
     const int off = (MR - (int)R % MR) % MR;
     return ((int)R - 1 + off) / C + 1;
+}
+
+// Group widths schedule_pairhmm would use on this host (pairhmm_schedule.cpp:
+// 176-186 with CpuFeatures, cpu_features.h:53-154): 16/8 with AVX-512
+// F+DQ+BW+VL and ZMM state enabled by the OS, 8/4 with AVX2 and YMM state,
+// else 0/0.
+void host_simd_widths(uint32_t *wf, uint32_t *wd) {
+    *wf = *wd = 0;
+    unsigned a, b, c, d;
+    if (!__get_cpuid_count(1, 0, &a, &b, &c, &d) || !(c & (1u << 27))) return;  // OSXSAVE
+    uint32_t xcr0, xcr0_hi;
+    __asm__("xgetbv" : "=a"(xcr0), "=d"(xcr0_hi) : "c"(0));
+    (void)xcr0_hi;
+    if (!__get_cpuid_count(7, 0, &a, &b, &c, &d)) return;
+    const uint32_t skx = (1u << 16) | (1u << 17) | (1u << 30) | (1u << 31);
+    const uint32_t zmm = (7u << 5) | (1u << 2) | (1u << 1);
+    if ((xcr0 & zmm) == zmm && (b & skx) == skx) {
+        *wf = 16;
+        *wd = 8;
+    } else if ((xcr0 & 6u) == 6u && (b & (1u << 5))) {
+        *wf = 8;
+        *wd = 4;
+    }
+}
+
+// ---------------------------------------------------------------- routing
+// schedule_pairhmm's FP32 stage (pairhmm_schedule.cpp:141-196): the region's
+// pairs in hap-major order, sorted by (hap length desc, read length desc)
+// with std::sort -- the algorithm the reference calls, so ties keep its
+// order -- then greedy windows of `wf` unused N-free pairs whose idle ratio
+// is <= idle_f go to the inter kernel (greedy_grouping, :46-105).
+struct RoutePair {
+    uint32_t local;  // h * num_reads + r inside the region
+    uint32_t hap_len, read_len;
+    bool contains_n, used;
+};
+
+void route_region(const rodv_phmm_batch *b, const rodv_phmm_region &rg,
+                  const std::vector<uint8_t> &hap_n, const std::vector<uint8_t> &read_n,
+                  const rodv_phmm_config &cfg, uint8_t *inter32, std::vector<uint32_t> &sorted_inter) {
+    sorted_inter.clear();
+    const uint32_t m = rg.num_haps, n = rg.num_reads, W = cfg.simd_width_float;
+    if (cfg.use_double || W == 0) return;
+    std::vector<RoutePair> pairs;
+    pairs.reserve((size_t)m * n);
+    for (uint32_t h = 0; h < m; h++)
+        for (uint32_t r = 0; r < n; r++)
+            pairs.push_back(RoutePair{h * n + r, b->hap_lengths[rg.hap_begin + h],
+                                      b->read_lengths[rg.read_begin + r],
+                                      hap_n[rg.hap_begin + h] || read_n[rg.read_begin + r], false});
+    std::sort(pairs.begin(), pairs.end(), [](const RoutePair &x, const RoutePair &y) {
+        if (x.hap_len == y.hap_len) return x.read_len > y.read_len;
+        return x.hap_len > y.hap_len;
+    });
+    const size_t total = pairs.size();
+    std::vector<size_t> win;
+    win.reserve(W);
+    for (size_t i = 0; i < total; i++) {
+        if (pairs[i].used || pairs[i].contains_n) continue;
+        win.clear();
+        for (size_t idx = i; idx < total && win.size() < W; idx++)
+            if (!pairs[idx].used && !pairs[idx].contains_n) win.push_back(idx);
+        if (win.size() < W) break;
+        uint32_t mh = 0, mr = 0;
+        uint64_t sum = 0;
+        for (size_t k : win) {
+            mh = std::max(mh, pairs[k].hap_len);
+            mr = std::max(mr, pairs[k].read_len);
+            sum += (uint64_t)pairs[k].hap_len * pairs[k].read_len;
+        }
+        const uint64_t cap = (uint64_t)mh * mr * W;
+        const uint64_t idle = cap > sum ? cap - sum : 0;
+        const double ratio = cap == 0 ? 0.0 : (double)idle / (double)cap;
+        if (ratio <= cfg.max_idle_ratio_float)
+            for (size_t k : win) pairs[k].used = true;
+    }
+    // the FP64 stage scans the same sorted order, over the pairs the FP32
+    // stage grouped (pairhmm_schedule.cpp:258-263)
+    for (const RoutePair &pp : pairs)
+        if (pp.used) {
+            inter32[pp.local] = 1;
+            sorted_inter.push_back(pp.local);
+        }
 }
 
 int validate(const rodv_phmm_batch *b) {
@@ -345,12 +435,11 @@ void rodv_phmm_config_default(rodv_phmm_config *cfg) {
     if (!cfg) return;
     std::memset(cfg, 0, sizeof(*cfg));
     cfg->struct_size = sizeof(*cfg);
-    cfg->semantics = RODV_PHMM_SEMANTICS_INTRA;
+    cfg->semantics = RODV_PHMM_SEMANTICS_SCHEDULE;
     cfg->use_double = 0;
     cfg->max_idle_ratio_float = 0.1;
     cfg->max_idle_ratio_double = 0.1;
-    cfg->simd_width_float = 16;
-    cfg->simd_width_double = 8;
+    host_simd_widths(&cfg->simd_width_float, &cfg->simd_width_double);
     cfg->use_graph = 1;
     cfg->waves_per_cu = 0;
     cfg->short_lanes = 16;
@@ -422,7 +511,10 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.short_lanes != 16 && cfg.short_lanes != 64) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.waves_per_cu < 0 || cfg.waves_per_cu > 32) return RODV_PHMM_ERR_INVALID_ARG;
-    if (cfg.semantics == RODV_PHMM_SEMANTICS_SCHEDULE) return RODV_PHMM_ERR_UNSUPPORTED;
+    if (cfg.simd_width_float > 64 || cfg.simd_width_double > 64) return RODV_PHMM_ERR_INVALID_ARG;
+    if (!(cfg.max_idle_ratio_float == cfg.max_idle_ratio_float) ||
+        !(cfg.max_idle_ratio_double == cfg.max_idle_ratio_double))
+        return RODV_PHMM_ERR_INVALID_ARG;
     (void)tables();
     HIPCHK(hipSetDevice(ctx->device));
 
@@ -484,6 +576,10 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         std::memcpy(&qc[dst], b->gcp_quals + so, n);
     }
     if (!ok) return RODV_PHMM_ERR_INVALID_ARG;
+    std::vector<uint8_t> read_has_n(N, 0);
+    for (uint32_t r = 0; r < N; r++)
+        for (uint32_t k = 0; k < b->read_lengths[r]; k++)
+            read_has_n[r] |= rcode[read_off[r] + k] == kReadN;
 
     rodv_phmm_plan *p = new (std::nothrow) rodv_phmm_plan();
     if (!p) return RODV_PHMM_ERR_OOM;
@@ -507,8 +603,39 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                 }
         }
     }
-    // key = L * 64 + MR
-    std::vector<int> bucket_of_key(2 * 81, -1);
+    // ---- schedule_pairhmm routing: which pairs its FP32 stage sends to the
+    // inter kernel; regions are independent calls, routed in parallel
+    std::vector<uint8_t> inter32;
+    std::vector<uint32_t> sched_list;
+    std::vector<uint64_t> sched_off;
+    if (cfg.semantics == RODV_PHMM_SEMANTICS_SCHEDULE) {
+        inter32.assign(n_pairs, 0);
+        const size_t nr = regions.size();
+        std::vector<uint64_t> base(nr + 1, 0);
+        for (size_t ri = 0; ri < nr; ri++)
+            base[ri + 1] = base[ri] + (uint64_t)regions[ri].num_haps * regions[ri].num_reads;
+        std::vector<std::vector<uint32_t>> lists(nr);
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t ri = next++; ri < nr; ri = next++)
+                route_region(b, regions[ri], hap_has_n, read_has_n, cfg, inter32.data() + base[ri],
+                             lists[ri]);
+        };
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nt && t < nr; t++) pool.emplace_back(work);
+        work();
+        for (auto &t : pool) t.join();
+        sched_off.assign(nr + 1, 0);
+        for (size_t ri = 0; ri < nr; ri++) {
+            sched_off[ri + 1] = sched_off[ri] + lists[ri].size();
+            for (uint32_t local : lists[ri]) sched_list.push_back((uint32_t)(base[ri] + local));
+        }
+        p->any_inter = !sched_list.empty();
+        p->n_regions = (uint32_t)nr;
+    }
+    // key = (tab ? 81 : 0) + (L == 16 ? 0 : 64) + MR, + 162 for inter pairs
+    std::vector<int> bucket_of_key(4 * 81, -1);
     std::vector<std::vector<uint32_t>> members;
     uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
     for (uint64_t k = 0; k < n_pairs; k++) {
@@ -524,13 +651,15 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             while (MR < 16 && 63 * MR < (int)R) MR += 4;
         }
         const bool tab = cfg.prior_table && !hap_has_n[ph[k]] && strips_for(R, L, MR) == 1;
-        const int key = (tab ? 81 : 0) + (L == 16 ? 0 : 64) + MR;
+        const bool inter = !inter32.empty() && inter32[k];
+        const int key = (inter ? 162 : 0) + (tab ? 81 : 0) + (L == 16 ? 0 : 64) + MR;
         if (bucket_of_key[key] < 0) {
             bucket_of_key[key] = (int)p->buckets.size();
             Bucket bk;
             bk.L = L;
             bk.MR = MR;
             bk.tab = tab;
+            bk.inter = inter;
             p->buckets.push_back(bk);
             members.emplace_back();
         }
@@ -540,7 +669,8 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     // ---- items: length-sorted packing of G pairs per wave
     for (size_t bi = 0; bi < p->buckets.size(); bi++) {
         Bucket &bk = p->buckets[bi];
-        bk.kern = bk.tab ? fwd_tab_kernel_ptr(bk.L, bk.MR, false) : fwd_f32_kernel(bk.L, bk.MR, false);
+        bk.kern = bk.tab ? fwd_tab_kernel_ptr(bk.L, bk.MR, bk.inter)
+                         : fwd_f32_kernel(bk.L, bk.MR, bk.inter);
         if (!bk.kern) {
             free_plan(p);
             return RODV_PHMM_ERR_UNSUPPORTED;
@@ -653,6 +783,11 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (!rc) rc = upload(&p->d_read_len, b->read_lengths, N);
     if (!rc) rc = upload(&p->d_pair_hap, ph.data(), n_pairs);
     if (!rc) rc = upload(&p->d_pair_read, pr.data(), n_pairs);
+    if (!rc && p->any_inter) {
+        rc = upload(&p->d_pair_inter, inter32.data(), n_pairs);
+        if (!rc) rc = upload(&p->d_sched_list, sched_list.data(), sched_list.size());
+        if (!rc) rc = upload(&p->d_sched_off, sched_off.data(), sched_off.size());
+    }
     if (!rc) rc = dev_alloc(&p->d_raw32, n_pairs);
     if (!rc) rc = dev_alloc(&p->d_raw64, n_pairs);
     if (!rc) rc = dev_alloc(&p->d_out, n_pairs);
@@ -769,6 +904,22 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
         }
     }
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
+    if (p->any_inter && p->cfg.simd_width_double > 0) {
+        // schedule_pairhmm's FP64 stage groups the rejected inter pairs
+        RegroupArgs r;
+        r.list = p->d_sched_list;
+        r.off = p->d_sched_off;
+        r.n_regions = p->n_regions;
+        r.wd = p->cfg.simd_width_double;
+        r.idle_d = p->cfg.max_idle_ratio_double;
+        r.pair_hap = p->d_pair_hap;
+        r.pair_read = p->d_pair_read;
+        r.hap_len = p->d_hap_len;
+        r.read_len = p->d_read_len;
+        r.flags = p->d_flags;
+        HIPCHK(launch_regroup(r, st));
+        launches++;
+    }
     {
         FwdArgs a = base_args(p);
         a.counter = p->d_counters + 1;
@@ -776,7 +927,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
         a.lds_stride = p->rescue_lds;
         a.bnd = p->d_bnd64;
         a.bnd_elems = p->bnd64_elems;
-        HIPCHK(launch_fwd(fwd_f64_rescue_kernel(false), p->rescue_grid, p->rescue_lds, st, a));
+        HIPCHK(launch_fwd(fwd_f64_rescue_kernel(p->any_inter), p->rescue_grid, p->rescue_lds, st, a));
         launches++;
     }
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
@@ -923,6 +1074,7 @@ int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     rodv_phmm_config cfg;
     rodv_phmm_config_default(&cfg);
     cfg.use_graph = 0;
+    cfg.semantics = RODV_PHMM_SEMANTICS_INTRA;  // no routing: only the uploads are used
     rodv_phmm_plan *p = nullptr;
     int rc = rodv_phmm_plan_create(ctx, batch, &cfg, &p);
     if (rc) return rc;

@@ -67,6 +67,27 @@ struct FwdArgs {
     uint32_t max_strips;
 };
 
+// Flag bit used between the passes of a plan: the FP64 regroup of
+// schedule_pairhmm put this FP32-rejected inter pair in an inter FP64 group.
+// The finalisation turns it into RODV_PHMM_FLAG_INTER and clears it.
+constexpr uint8_t kFlagGrouped64 = 0x10;
+
+// schedule_pairhmm's FP64 stage (pairhmm_schedule.cpp:251-304): per region,
+// greedy windows of `wd` FP32-rejected inter pairs in the region's sorted
+// order, accepted when their idle ratio is <= idle_d.
+struct RegroupArgs {
+    const uint32_t *list;     // per region, its FP32 inter pairs in sorted order
+    const uint64_t *off;      // n_regions + 1 offsets into list
+    uint32_t n_regions;
+    uint32_t wd;
+    double idle_d;
+    const uint32_t *pair_hap;
+    const uint32_t *pair_read;
+    const uint32_t *hap_len;
+    const uint32_t *read_len;
+    uint8_t *flags;           // RESCUED from the FP32 pass; kFlagGrouped64 set here
+};
+
 struct FinalizeArgs {
     const float *raw32;
     const double *raw64;

@@ -273,40 +273,82 @@ struct TabLayout {
     static constexpr int MRT = (MR + 3) / 4 * 4;  // slots padded to 16 bytes
 };
 
+// the MR priors of one lane for one column: MRT/4 ds_read_b128
+template <int MR>
+__device__ __forceinline__ void load_priors(float (&pr)[MR], const uint8_t *p) {
+    constexpr int MRT = TabLayout<MR>::MRT;
+    const float4 *tp = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+    for (int q = 0; q < MRT / 4; q++) {
+        const float4 v = tp[q];
+        if (4 * q + 0 < MR) pr[4 * q + 0] = v.x;
+        if (4 * q + 1 < MR) pr[4 * q + 1] = v.y;
+        if (4 * q + 2 < MR) pr[4 * q + 2] = v.z;
+        if (4 * q + 3 < MR) pr[4 * q + 3] = v.w;
+    }
+}
+
+// The row above a lane's first row comes from the previous lane's last slot:
+// `up` (this step's column, rotated from the slot's value after the previous
+// step) and `dg` (the column before).  dg of step t is up of step t-1 for M
+// and I, so each step rotates three values: the new M and I of the last slot
+// (up of the next step) and its old D (dg of D for the next step).
+struct Carry {
+    float dM, dI, dD;  // dg of this step
+    float uM, uI;      // up of this step
+};
+
+// One step of one lane (all MR rows at the next column), ordered for the LDS
+// unit: the old-D rotation and the next column's priors are issued first,
+// then M and D of every slot (which need only the previous column), then the
+// I chain down the slots (which needs `up`, rotated at the end of the
+// previous step), then the rotations for the next step.  LDS results return
+// in issue order, so each wait covers only what the next phase needs.
 template <int L, int MR, bool INTER>
-__device__ __forceinline__ void step_tab(State<float, MR> &S, const Priors<float, MR> &P,
-                                         const float (&pr)[MR]) {
-    float dgM = ror1<L>(S.Mp), dgI = ror1<L>(S.Ip), dgD = ror1<L>(S.Dp);
-    float upM = ror1<L>(S.M[MR - 1]), upI = ror1<L>(S.I[MR - 1]);
+__device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Priors<float, MR> &P,
+                                         const float (&pr)[MR], const LaneRot<L> &rot,
+                                         float (&pb)[MR], const uint8_t *pnext) {
+    const float nextD = rot(S.D[MR - 1]);
+    load_priors<MR>(pb, pnext);
+    __builtin_amdgcn_sched_barrier(0);
+    float nM[MR], nD[MR];
+    float dgM = C.dM, dgI = C.dI, dgD = C.dD;
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         float t;
-        if (INTER) {
+        if (INTER) {  // pairhmm_inter.cpp:279-283 as GCC contracts it
             t = dgI * P.g[s];
             t = fmaT(dgM, P.mm[s], t);
             t = fmaT(dgD, P.g[s], t);
-        } else {
+        } else {  // pairhmm_impl.cpp:266-275
             t = dgM * P.mm[s];
             t = fmaT(dgI, P.g[s], t);
             t = fmaT(dgD, P.g[s], t);
         }
-        const float nM = t * pr[s];
-        const float u = upM * P.mx[s];
-        const float nI = fmaT(upI, P.xx[s], u);
+        nM[s] = t * pr[s];
         const float v = S.M[s] * P.my[s];
-        const float nD = fmaT(S.D[s], P.yy[s], v);
+        nD[s] = fmaT(S.D[s], P.yy[s], v);
         dgM = S.M[s];
         dgI = S.I[s];
         dgD = S.D[s];
-        upM = nM;
-        upI = nI;
-        S.M[s] = nM;
-        S.I[s] = nI;
-        S.D[s] = nD;
     }
-    S.Mp = dgM;
-    S.Ip = dgI;
-    S.Dp = dgD;
+    float upM = C.uM, upI = C.uI;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const float u = upM * P.mx[s];
+        const float nI = fmaT(upI, P.xx[s], u);
+        upM = nM[s];
+        upI = nI;
+        S.I[s] = nI;
+        S.M[s] = nM[s];
+        S.D[s] = nD[s];
+    }
+    C.dM = C.uM;
+    C.dI = C.uI;
+    C.dD = nextD;
+    __builtin_amdgcn_sched_barrier(0);
+    C.uM = rot(S.M[MR - 1]);
+    C.uI = rot(S.I[MR - 1]);
 }
 
 template <int L, int MR, bool INTER>
@@ -398,9 +440,13 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
         S.I[s] = 0.0f;
         S.D[s] = realv[s] ? 0.0f : initd;  // identity rows carry K/H
     }
-    S.Mp = 0.0f;
-    S.Ip = 0.0f;
-    S.Dp = S.D[MR - 1];
+    const LaneRot<L> rot;
+    Carry C;
+    C.dM = 0.0f;
+    C.dI = 0.0f;
+    C.dD = rot(S.D[MR - 1]);
+    C.uM = 0.0f;  // rotations of the all-zero M and I of the first column
+    C.uI = 0.0f;
 
     const int key = valid ? H + lR : 0x7fffffff;
     const int keymax = valid ? H + lR : -1;
@@ -409,44 +455,44 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
     const uint16_t *cl = codes + (L - 1) - l;
     const uint8_t *tlane = tab + (uint32_t)l * MRT * 4;
     float sm = 0.0f, si = 0.0f;
+    // Software pipeline: the priors of column d+1 are read while step d
+    // computes, the codes of the next block while this block runs.  Codes past
+    // the haplotype are zero (LDS cleared per item), so every prefetch address
+    // stays inside the group's table.
     int d = 1;
-    for (; d + kU - 1 <= Dmain; d += kU) {
+    float pa[MR];
+    load_priors<MR>(pa, tlane + cl[d]);
+    if (d + kU - 1 <= Dmain) {
         uint32_t co[kU];
 #pragma unroll
         for (int u = 0; u < kU; u++) co[u] = cl[d + u];
+        for (; d + kU - 1 <= Dmain; d += kU) {
+            uint32_t cn[kU];
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const float4 *tp = reinterpret_cast<const float4 *>(tlane + co[u]);
-            float pr[MR];
+            for (int u = 0; u < kU; u++) cn[u] = cl[d + kU + u];
 #pragma unroll
-            for (int q = 0; q < MRT / 4; q++) {
-                const float4 v = tp[q];
-                if (4 * q + 0 < MR) pr[4 * q + 0] = v.x;
-                if (4 * q + 1 < MR) pr[4 * q + 1] = v.y;
-                if (4 * q + 2 < MR) pr[4 * q + 2] = v.z;
-                if (4 * q + 3 < MR) pr[4 * q + 3] = v.w;
+            for (int u = 0; u < kU; u++) {
+                float pb[MR];
+                step_tab<L, MR, INTER>(S, C, P, pa, rot, pb,
+                                       tlane + (u + 1 < kU ? co[u + 1] : cn[0]));
+                sm += S.M[MR - 1];
+                si += S.I[MR - 1];
+#pragma unroll
+                for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
             }
-            step_tab<L, MR, INTER>(S, P, pr);
-            sm += S.M[MR - 1];
-            si += S.I[MR - 1];
+#pragma unroll
+            for (int u = 0; u < kU; u++) co[u] = cn[u];
         }
     }
     for (; d <= Dend; d++) {
-        const float4 *tp = reinterpret_cast<const float4 *>(tlane + cl[d]);
-        float pr[MR];
-#pragma unroll
-        for (int q = 0; q < MRT / 4; q++) {
-            const float4 v = tp[q];
-            if (4 * q + 0 < MR) pr[4 * q + 0] = v.x;
-            if (4 * q + 1 < MR) pr[4 * q + 1] = v.y;
-            if (4 * q + 2 < MR) pr[4 * q + 2] = v.z;
-            if (4 * q + 3 < MR) pr[4 * q + 3] = v.w;
-        }
-        step_tab<L, MR, INTER>(S, P, pr);
+        float pb[MR];
+        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + cl[d + 1]);
         if (d - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
         }
+#pragma unroll
+        for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
     }
     if (valid && l == lR) {
         const float Ssum = sm + si;  // pairhmm_impl.cpp:434-437
@@ -496,23 +542,91 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
     }
 }
 
+// FP64 rescue under schedule_pairhmm routing: pairs the FP64 regroup put in
+// an inter group run the inter arithmetic, every other rescued pair the
+// intra one (pairhmm_schedule.cpp:273-304, 307-335).  One pair per wave, so
+// the choice is uniform.
+__global__ __launch_bounds__(64) void fwd_rescue_route_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t n = *a.n_items_dev;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        const int p = (int)a.rescue_list[it];
+        const uint32_t inter =
+            __builtin_amdgcn_readfirstlane((a.flags[p] & kFlagGrouped64) ? 1u : 0u);
+        if (inter)
+            process_item<double, 64, 4, true, true>(a, lds, p);
+        else
+            process_item<double, 64, 4, false, true>(a, lds, p);
+    }
+}
+
+// greedy_grouping (pairhmm_schedule.cpp:46-105) over one region's FP32-rejected
+// inter pairs, one thread per region.  A window is the next `wd` candidates
+// that no earlier group took; idle = 1 - sum(R*H) / (wd * max R * max H) in
+// the reference's integer and double arithmetic.
+__global__ void regroup_kernel(RegroupArgs a) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n_regions || a.wd == 0) return;
+    const uint32_t *list = a.list + a.off[g];
+    const uint32_t n = (uint32_t)(a.off[g + 1] - a.off[g]);
+    auto candidate = [&](uint32_t p) {
+        const uint8_t f = a.flags[p];
+        return (f & 0x01) && !(f & kFlagGrouped64);
+    };
+    for (uint32_t i = 0; i < n; i++) {
+        if (!candidate(list[i])) continue;
+        uint32_t idx = i, cnt = 0, mh = 0, mr = 0;
+        uint64_t sum = 0;
+        while (idx < n && cnt < a.wd) {
+            const uint32_t p = list[idx];
+            if (candidate(p)) {
+                const uint32_t H = a.hap_len[a.pair_hap[p]], R = a.read_len[a.pair_read[p]];
+                mh = H > mh ? H : mh;
+                mr = R > mr ? R : mr;
+                sum += (uint64_t)H * R;
+                cnt++;
+            }
+            idx++;
+        }
+        if (cnt < a.wd) break;
+        const uint64_t cap = (uint64_t)mh * mr * a.wd;
+        const uint64_t idle = cap > sum ? cap - sum : 0;
+        const double ratio = cap == 0 ? 0.0 : (double)idle / (double)cap;
+        if (ratio <= a.idle_d) {
+            for (uint32_t j = i; j < idx; j++)
+                if (candidate(list[j])) a.flags[list[j]] |= kFlagGrouped64;
+        }
+    }
+}
+
 // log10 conversion (pairhmm_api.cpp:17-22 and pairhmm_inter_api.h:11-15).
 __global__ void finalize_kernel(FinalizeArgs f) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= f.n) return;
     uint8_t fl = f.flags[k];
-    const bool inter = f.pair_inter && f.pair_inter[k];
+    const bool inter32 = f.pair_inter && f.pair_inter[k];
+    const bool inter64 = (fl & kFlagGrouped64) != 0;
+    fl &= (uint8_t)~kFlagGrouped64;
     double out;
+    bool inter = false;
     if (fl & 0x01) {
+        // FP64 result: intra rescue, inter FP64 group, or the intra FP64
+        // fallback of a rejected inter pair -- the same conversion
         double r = f.raw64[k];
         out = log10(r) - 307.050595577260822;
         if (!(r > 0.0)) fl |= 0x02;
+        inter = inter64;
     } else {
         float r = f.raw32[k];
-        if (inter)
+        if (inter32)
             out = log10((double)r) - (double)36.1236000061f;
         else
             out = (double)(log10f(r) - 36.1236000061f);
+        inter = inter32;
     }
     // within 2^-20 relative of the rescue threshold: the decision is exact
     // here, but would flip under a 1-ulp perturbation of the sum
@@ -589,8 +703,17 @@ void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
 uint32_t fwd_tab_bytes(int L, int MR) { return 4u * (uint32_t)L * (uint32_t)((MR + 3) / 4 * 4) * 4u; }
 
 // FP64 rescue kernel: one pair per wave, 4 rows per lane (255-row strips).
-void *fwd_f64_rescue_kernel(bool inter) {
-    return inter ? kernel_ptr<double, 64, 4, true, true>() : kernel_ptr<double, 64, 4, false, true>();
+// route: per pair intra or inter arithmetic as the FP64 regroup decided.
+void *fwd_f64_rescue_kernel(bool route) {
+    return route ? reinterpret_cast<void *>(&fwd_rescue_route_kernel)
+                 : kernel_ptr<double, 64, 4, false, true>();
+}
+
+hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st) {
+    const uint32_t blocks = (r.n_regions + 255) / 256;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(regroup_kernel, dim3(blocks), dim3(256), 0, st, r);
+    return hipGetLastError();
 }
 int fwd_f64_rescue_mr() { return 4; }
 

@@ -209,3 +209,35 @@ def random_batch(rng, n_haps, n_reads, hap_len=(20, 200), read_len=(10, 150), p_
         b.hap_bases = b.hap_bases.copy()
         b.hap_bases[rng.random(len(b.hap_bases)) < n_frac] = ord("N")
     return b
+
+
+def region(rng, hap_lens, read_lens, unrelated=(), n_read=(), n_hap=(), qual=30):
+    """One region (len(hap_lens) haplotypes x len(read_lens) reads) with the
+    exact lengths given.  Haplotypes are prefixes of one random sequence;
+    read k is a substring of the shortest of them (padded with random bases
+    where it is longer), or random bases when k is in `unrelated`; reads in
+    `n_read` and haplotypes in `n_hap` get one N.  Qualities: base `qual`,
+    insertion/deletion 45, gap continuation 10."""
+    base = np.frombuffer(b"ACGT", np.uint8)
+    seq = base[rng.integers(0, 4, int(max(hap_lens)))]
+    haps = [seq[:int(h)].copy() for h in hap_lens]
+    hmin = int(min(hap_lens))
+    reads = []
+    for k, r in enumerate(read_lens):
+        r = int(r)
+        if k in unrelated:
+            rd = base[rng.integers(0, 4, r)]
+        else:
+            s = int(rng.integers(0, max(1, hmin - r + 1)))
+            rd = seq[s:min(s + r, hmin)]
+            if len(rd) < r:
+                rd = np.concatenate([rd, base[rng.integers(0, 4, r - len(rd))]])
+        reads.append(rd.copy())
+    for k in n_read:
+        reads[k][len(reads[k]) // 2] = ord("N")
+    for k in n_hap:
+        haps[k][len(haps[k]) // 2] = ord("N")
+    q = [np.full(len(r), qual, np.uint8) for r in reads]
+    i = [np.full(len(r), 45, np.uint8) for r in reads]
+    c = [np.full(len(r), 10, np.uint8) for r in reads]
+    return Batch.from_lists([h.tobytes() for h in haps], [r.tobytes() for r in reads], q, i, i, c)

@@ -18,6 +18,9 @@ TOL = 1e-4  # |d log10 L| bound of this path
 
 
 def run(engine, batch, **cfg):
+    # per-pair intra semantics: the oracle's run_batch restates the intra API;
+    # schedule_pairhmm routing has its own tests (test_schedule_gpu.py)
+    cfg.setdefault("semantics", 0)
     plan = engine.plan(batch, default_config(**cfg))
     plan.execute()
     out, fl, r32, r64 = plan.read(raw=True)
@@ -126,7 +129,7 @@ def test_config2_subsample(engine, oracle_mod):
 
 def test_plain_equals_batched_and_graph_replays(engine):
     b = synth.config(2, scale=0.003)
-    plan = engine.plan(b, default_config(use_graph=1))
+    plan = engine.plan(b, default_config(use_graph=1, semantics=0))
     plan.execute_plain()
     ref = plan.read(raw=True)
     for _ in range(3):

@@ -49,7 +49,18 @@ def test_strerror_and_config_defaults():
     assert lib.rodv_phmm_strerror(99) == b"unknown status"
     c = E.default_config()
     assert c.struct_size == E.C.sizeof(E.Config)
-    assert c.max_idle_ratio_float == 0.1 and c.simd_width_float == 16
+    assert c.semantics == E.SEMANTICS_SCHEDULE and c.max_idle_ratio_float == 0.1
+    # group widths follow this host's CPU like the reference's CpuFeatures
+    flags = set()
+    with open("/proc/cpuinfo") as f:
+        for line in f:
+            if line.startswith("flags"):
+                flags = set(line.split(":", 1)[1].split())
+                break
+    if {"avx512f", "avx512dq", "avx512bw", "avx512vl"} <= flags:
+        assert (c.simd_width_float, c.simd_width_double) == (16, 8)
+    elif "avx2" in flags:
+        assert (c.simd_width_float, c.simd_width_double) == (8, 4)
     assert c.short_lanes == 16 and c.use_graph == 1
     with pytest.raises(E.PhmmError):
         E.tables_f32(40000)

@@ -1,0 +1,116 @@
+"""schedule_pairhmm routing on the GPU (RODV_PHMM_SEMANTICS_SCHEDULE) against
+the oracle's restatement of it (oracle/phmm_oracle_sched.cpp, pinned by
+test_schedule_oracle.py): the same pairs take the inter arithmetic, the FP32
+sums that decide the rescue are bit-identical, the same pairs are regrouped
+for inter FP64, and every log10 L agrees within 1e-4."""
+import numpy as np
+import pytest
+
+from rodv_amd import synth
+from rodv_amd.batch import Batch
+from rodv_amd.engine import FLAG_INTER, FLAG_RESCUED, FLAG_UNDERFLOW, default_config
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+ROUTE_INTER32, ROUTE_INTER64 = 1, 2
+
+
+def check_schedule(engine, oracle_mod, batch, wf=16, wd=8, idle_f=0.1, idle_d=0.1,
+                   use_double=False, mode="batched"):
+    cfg = default_config(semantics=1, simd_width_float=wf, simd_width_double=wd,
+                         max_idle_ratio_float=idle_f, max_idle_ratio_double=idle_d,
+                         use_double=int(use_double), use_graph=1)
+    plan = engine.plan(batch, cfg)
+    if mode == "plain":
+        plan.execute_plain()
+    else:
+        plan.execute()
+    out, fl, r32, _ = plan.read(raw=True)
+    plan.close()
+    o_out, o_fl, o_r32, route = oracle_mod.schedule_batch(
+        batch, use_double=use_double, idle_f=idle_f, idle_d=idle_d, wf=wf, wd=wd)
+    # the deciding FP32 sums, bit for bit (inter arithmetic where routed)
+    bad = np.nonzero(r32.view(np.uint32) != o_r32.view(np.uint32))[0]
+    assert len(bad) == 0, f"{len(bad)} raw FP32 mismatches, first {bad[:5]}, routes {route[bad[:5]]}"
+    mask = FLAG_RESCUED | FLAG_UNDERFLOW | FLAG_INTER
+    np.testing.assert_array_equal(fl & mask, o_fl & mask)
+    fin = np.isfinite(o_out)
+    np.testing.assert_array_equal(np.isfinite(out), fin)
+    if fin.any():
+        err = np.abs(out[fin] - o_out[fin])
+        assert err.max() <= TOL, f"max |dlog10L| {err.max()}"
+    return fl, route
+
+
+def test_hand_cases_route_like_the_oracle(engine, oracle_mod):
+    rng = np.random.default_rng(1)
+    regions = [
+        synth.region(rng, [300], [100, 100, 100, 60, 60, 60, 60, 60]),
+        synth.region(rng, [300], [53, 100, 51, 55, 52, 54]),
+        synth.region(rng, [300, 280], [100] * 5, n_read=[2]),
+        synth.region(rng, [200, 300], [90, 100]),
+        synth.region(rng, [250], [100, 101, 102, 103, 104, 105, 106, 107], unrelated=range(8)),
+    ]
+    b = Batch.concat(regions)
+    for idle in (0.1, 0.09, 1.0):
+        fl, route = check_schedule(engine, oracle_mod, b, wf=4, wd=3, idle_f=idle, idle_d=1.0)
+        assert (route == ROUTE_INTER32).any()
+    fl, route = check_schedule(engine, oracle_mod, b, wf=4, wd=3, idle_f=1.0, idle_d=1.0)
+    assert (route == ROUTE_INTER64).any()
+
+
+@pytest.mark.parametrize("wf,wd", [(16, 8), (8, 4), (4, 2)])
+def test_random_regions(engine, oracle_mod, wf, wd):
+    rng = np.random.default_rng(10 + wf)
+    regs = []
+    for k in range(12):
+        nh, nr = int(rng.integers(2, 9)), int(rng.integers(4, 40))
+        hl = rng.integers(60, 300, nh)
+        rl = rng.integers(30, 200, nr)
+        unrelated = [i for i in range(nr) if rng.random() < 0.3]
+        regs.append(synth.region(rng, hl, rl, unrelated=unrelated,
+                                 n_read=[0] if k % 5 == 0 else []))
+    b = Batch.concat(regs)
+    fl, route = check_schedule(engine, oracle_mod, b, wf=wf, wd=wd, idle_f=0.5, idle_d=0.5)
+    assert (fl & FLAG_INTER).any() and (fl & FLAG_RESCUED).any()
+
+
+def test_use_double_is_all_intra(engine, oracle_mod):
+    rng = np.random.default_rng(3)
+    b = synth.region(rng, [300, 250], [100] * 8, unrelated=[0, 1, 2])
+    fl, route = check_schedule(engine, oracle_mod, b, wf=4, wd=2, idle_f=1.0, use_double=True)
+    assert not (fl & FLAG_INTER).any()
+
+
+def test_no_simd_widths_is_all_intra(engine, oracle_mod):
+    rng = np.random.default_rng(4)
+    b = synth.region(rng, [300], [100] * 8)
+    fl, _ = check_schedule(engine, oracle_mod, b, wf=0, wd=0, idle_f=1.0)
+    assert not (fl & FLAG_INTER).any()
+
+
+def test_config2_sample_schedule(engine, oracle_mod):
+    b = synth.config(2, scale=0.002)
+    fl, route = check_schedule(engine, oracle_mod, b)
+    assert (route == ROUTE_INTER32).sum() > 0.5 * len(route)
+
+
+def test_plain_and_graph_agree_under_routing(engine, oracle_mod):
+    rng = np.random.default_rng(5)
+    regs = [synth.region(rng, rng.integers(100, 300, 4), rng.integers(60, 160, 24),
+                         unrelated=[1, 2, 3, 7, 9, 11]) for _ in range(6)]
+    b = Batch.concat(regs)
+    check_schedule(engine, oracle_mod, b, wf=8, wd=4, idle_f=0.5, idle_d=0.5, mode="plain")
+    cfg = default_config(semantics=1, simd_width_float=8, simd_width_double=4,
+                         max_idle_ratio_float=0.5, max_idle_ratio_double=0.5, use_graph=1)
+    plan = engine.plan(b, cfg)
+    plan.execute_plain()
+    ref = plan.read(raw=True)
+    for _ in range(3):
+        plan.execute()
+        got = plan.read(raw=True)
+        np.testing.assert_array_equal(got[1], ref[1])
+        np.testing.assert_array_equal(got[2].view(np.uint32), ref[2].view(np.uint32))
+        np.testing.assert_array_equal(got[0], ref[0])
+    plan.close()
