@@ -226,6 +226,7 @@ struct rodv_phmm_plan {
     // schedule_pairhmm routing: per region, its FP32 inter pairs in sorted order
     uint32_t *d_sched_list = nullptr;
     uint64_t *d_sched_off = nullptr;
+    uint32_t *d_sched_scratch = nullptr;
     uint32_t n_regions = 0;
     // device outputs / scratch
     float *d_raw32 = nullptr;
@@ -270,7 +271,7 @@ void free_plan(rodv_phmm_plan *p) {
                     p->d_rcode, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
                     p->d_read_len, p->d_pair_hap, p->d_pair_read, p->d_pair_inter,
                     p->d_raw32, p->d_raw64, p->d_out, p->d_flags, p->d_rescue,
-                    p->d_counters, p->d_bnd64, p->d_sched_list, p->d_sched_off};
+                    p->d_counters, p->d_bnd64, p->d_sched_list, p->d_sched_off, p->d_sched_scratch};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &b : p->buckets) {
@@ -692,8 +693,10 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             bk.cells += (uint64_t)R * H;
         }
         if (bk.tab) {
-            // u16 code per column, then the prior table
-            bk.lds_stride = (2u * lds_stride_for(bk.max_h, bk.L) + 15u) & ~15u;
+            // u16 code per column, then the prior table.  A multiple of 256
+            // bytes: every group's table then starts on LDS bank 0, which
+            // keeps the four groups' 16-byte prior reads free of bank conflicts
+            bk.lds_stride = (2u * lds_stride_for(bk.max_h, bk.L) + 255u) & ~255u;
             bk.tab_bytes = fwd_tab_bytes(bk.L, bk.MR);
         } else {
             bk.lds_stride = lds_stride_for(bk.max_h, bk.L);
@@ -787,6 +790,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         rc = upload(&p->d_pair_inter, inter32.data(), n_pairs);
         if (!rc) rc = upload(&p->d_sched_list, sched_list.data(), sched_list.size());
         if (!rc) rc = upload(&p->d_sched_off, sched_off.data(), sched_off.size());
+        if (!rc) rc = dev_alloc(&p->d_sched_scratch, sched_list.size());
     }
     if (!rc) rc = dev_alloc(&p->d_raw32, n_pairs);
     if (!rc) rc = dev_alloc(&p->d_raw64, n_pairs);
@@ -917,6 +921,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
         r.hap_len = p->d_hap_len;
         r.read_len = p->d_read_len;
         r.flags = p->d_flags;
+        r.scratch = p->d_sched_scratch;
         HIPCHK(launch_regroup(r, st));
         launches++;
     }

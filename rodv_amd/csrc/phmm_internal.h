@@ -86,6 +86,7 @@ struct RegroupArgs {
     const uint32_t *hap_len;
     const uint32_t *read_len;
     uint8_t *flags;           // RESCUED from the FP32 pass; kFlagGrouped64 set here
+    uint32_t *scratch;        // same layout as list: the region's rejected pairs
 };
 
 struct FinalizeArgs {

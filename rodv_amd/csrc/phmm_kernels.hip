@@ -273,6 +273,13 @@ struct TabLayout {
     static constexpr int MRT = (MR + 3) / 4 * 4;  // slots padded to 16 bytes
 };
 
+// One haplotype code (a u16 byte offset into the prior table).  A relaxed
+// atomic load is a plain ds_read_u16 that the compiler does not merge with
+// its neighbours into a wider, unaligned read.
+__device__ __forceinline__ uint32_t ld_code(const uint16_t *p) {
+    return __atomic_load_n(p, __ATOMIC_RELAXED);
+}
+
 // the MR priors of one lane for one column: MRT/4 ds_read_b128
 template <int MR>
 __device__ __forceinline__ void load_priors(float (&pr)[MR], const uint8_t *p) {
@@ -456,43 +463,39 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
     const uint8_t *tlane = tab + (uint32_t)l * MRT * 4;
     float sm = 0.0f, si = 0.0f;
     // Software pipeline: the priors of column d+1 are read while step d
-    // computes, the codes of the next block while this block runs.  Codes past
-    // the haplotype are zero (LDS cleared per item), so every prefetch address
+    // computes, the code of column d+2 one step earlier still.  Codes are read
+    // one per step with 2-byte loads: lanes start at different columns, so a
+    // wider load would be unaligned for most of them.  Codes past the
+    // haplotype are zero (LDS cleared per item), so every prefetch address
     // stays inside the group's table.
     int d = 1;
     float pa[MR];
-    load_priors<MR>(pa, tlane + cl[d]);
-    if (d + kU - 1 <= Dmain) {
-        uint32_t co[kU];
+    load_priors<MR>(pa, tlane + ld_code(cl + d));
+    uint32_t c1 = ld_code(cl + d + 1);
+    for (; d + kU - 1 <= Dmain; d += kU) {
 #pragma unroll
-        for (int u = 0; u < kU; u++) co[u] = cl[d + u];
-        for (; d + kU - 1 <= Dmain; d += kU) {
-            uint32_t cn[kU];
+        for (int u = 0; u < kU; u++) {
+            const uint32_t c2 = ld_code(cl + d + u + 2);
+            float pb[MR];
+            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1);
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
 #pragma unroll
-            for (int u = 0; u < kU; u++) cn[u] = cl[d + kU + u];
-#pragma unroll
-            for (int u = 0; u < kU; u++) {
-                float pb[MR];
-                step_tab<L, MR, INTER>(S, C, P, pa, rot, pb,
-                                       tlane + (u + 1 < kU ? co[u + 1] : cn[0]));
-                sm += S.M[MR - 1];
-                si += S.I[MR - 1];
-#pragma unroll
-                for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
-            }
-#pragma unroll
-            for (int u = 0; u < kU; u++) co[u] = cn[u];
+            for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+            c1 = c2;
         }
     }
     for (; d <= Dend; d++) {
+        const uint32_t c2 = ld_code(cl + d + 2);
         float pb[MR];
-        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + cl[d + 1]);
+        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1);
         if (d - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
         }
 #pragma unroll
         for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+        c1 = c2;
     }
     if (valid && l == lR) {
         const float Ssum = sm + si;  // pairhmm_impl.cpp:434-437
@@ -565,40 +568,56 @@ __global__ __launch_bounds__(64) void fwd_rescue_route_kernel(FwdArgs a) {
 }
 
 // greedy_grouping (pairhmm_schedule.cpp:46-105) over one region's FP32-rejected
-// inter pairs, one thread per region.  A window is the next `wd` candidates
-// that no earlier group took; idle = 1 - sum(R*H) / (wd * max R * max H) in
-// the reference's integer and double arithmetic.
-__global__ void regroup_kernel(RegroupArgs a) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+// inter pairs, one wave per region.  The wave first compacts the region's
+// rejected pairs, in sorted order, into the scratch list (ballot + prefix
+// count).  On that list the reference's scan reduces to: at position i, stop
+// if fewer than wd remain; else the window is the next wd entries (every
+// accepted window is a contiguous block the scan then skips), accepted when
+// idle = 1 - sum(R*H) / (wd * max R * max H) <= idle_d, in the reference's
+// integer and double arithmetic.
+__global__ __launch_bounds__(64) void regroup_kernel(RegroupArgs a) {
+    const uint32_t g = blockIdx.x;
     if (g >= a.n_regions || a.wd == 0) return;
+    const uint32_t lane = threadIdx.x;
     const uint32_t *list = a.list + a.off[g];
+    uint32_t *cand = a.scratch + a.off[g];
     const uint32_t n = (uint32_t)(a.off[g + 1] - a.off[g]);
-    auto candidate = [&](uint32_t p) {
-        const uint8_t f = a.flags[p];
-        return (f & 0x01) && !(f & kFlagGrouped64);
-    };
-    for (uint32_t i = 0; i < n; i++) {
-        if (!candidate(list[i])) continue;
-        uint32_t idx = i, cnt = 0, mh = 0, mr = 0;
-        uint64_t sum = 0;
-        while (idx < n && cnt < a.wd) {
-            const uint32_t p = list[idx];
-            if (candidate(p)) {
-                const uint32_t H = a.hap_len[a.pair_hap[p]], R = a.read_len[a.pair_read[p]];
-                mh = H > mh ? H : mh;
-                mr = R > mr ? R : mr;
-                sum += (uint64_t)H * R;
-                cnt++;
-            }
-            idx++;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t cnt = 0;
+    for (uint32_t base = 0; base < n; base += 64) {
+        const uint32_t i = base + lane;
+        uint32_t p = 0;
+        bool c = false;
+        if (i < n) {
+            p = list[i];
+            c = (a.flags[p] & 0x01) != 0;
         }
-        if (cnt < a.wd) break;
-        const uint64_t cap = (uint64_t)mh * mr * a.wd;
+        const uint64_t m = __ballot(c);
+        if (c) cand[cnt + (uint32_t)__popcll(m & below)] = p;
+        cnt += (uint32_t)__popcll(m);
+    }
+    __syncthreads();  // the compacted list, written by all lanes, is read by lane 0
+    if (lane != 0) return;
+    const uint32_t W = a.wd;
+    uint32_t i = 0;
+    while (i < cnt && cnt - i >= W) {
+        uint32_t mh = 0, mr = 0;
+        uint64_t sum = 0;
+        for (uint32_t k = i; k < i + W; k++) {
+            const uint32_t p = cand[k];
+            const uint32_t H = a.hap_len[a.pair_hap[p]], R = a.read_len[a.pair_read[p]];
+            mh = H > mh ? H : mh;
+            mr = R > mr ? R : mr;
+            sum += (uint64_t)H * R;
+        }
+        const uint64_t cap = (uint64_t)mh * mr * W;
         const uint64_t idle = cap > sum ? cap - sum : 0;
         const double ratio = cap == 0 ? 0.0 : (double)idle / (double)cap;
         if (ratio <= a.idle_d) {
-            for (uint32_t j = i; j < idx; j++)
-                if (candidate(list[j])) a.flags[list[j]] |= kFlagGrouped64;
+            for (uint32_t k = i; k < i + W; k++) a.flags[cand[k]] |= kFlagGrouped64;
+            i += W;
+        } else {
+            i += 1;
         }
     }
 }
@@ -710,9 +729,8 @@ void *fwd_f64_rescue_kernel(bool route) {
 }
 
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st) {
-    const uint32_t blocks = (r.n_regions + 255) / 256;
-    if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(regroup_kernel, dim3(blocks), dim3(256), 0, st, r);
+    if (r.n_regions == 0) return hipSuccess;
+    hipLaunchKernelGGL(regroup_kernel, dim3(r.n_regions), dim3(64), 0, st, r);
     return hipGetLastError();
 }
 int fwd_f64_rescue_mr() { return 4; }
