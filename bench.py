@@ -131,6 +131,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint", type=int, default=1)
     ap.add_argument("--prior-table", type=int, default=1)
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="tune pairs per wavefront / strip width / waves per CU on a 4%% sample "
+                         "of the workload's distribution before timing")
+    ap.add_argument("--semantics", choices=("schedule", "intra"), default="schedule")
     ap.add_argument("--waves-per-cu", type=int, default=0,
                     help="persistent grid size per CU (0 = engine default)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -152,9 +156,28 @@ def main():
     log(f"workload config {args.config}: {batch.num_pairs()} pairs, {cells:.4e} cells "
         f"({time.time() - t0:.1f}s to generate)")
     eng = Engine(0)
+    sem = 1 if args.semantics == "schedule" else 0
+    tuning = None
+    if args.autotune:
+        t0 = time.time()
+        sample = synth.config(args.config, scale=args.scale * 0.04)
+        tn = eng.autotune(sample, default_config(prior_table=args.prior_table))
+        tuning = {"sample_pairs": sample.num_pairs(), "seconds": round(time.time() - t0, 1),
+                  "classes": [{"max_read_len": e["max_read_len"],
+                               "pairs_per_wave": e["pairs_per_wave"],
+                               "max_rows_per_lane": e["max_rows_per_lane"],
+                               "waves_per_cu": e["waves_per_cu"],
+                               "gcups": round(e["gcups"], 1),
+                               "default_gcups": round(e["default_gcups"], 1)}
+                              for e in tn.entries() if e["pairs_per_wave"]]}
+        log(f"autotune on {sample.num_pairs()} sample pairs: {tuning['seconds']} s")
+        for c in tuning["classes"]:
+            log(f"  R <= {c['max_read_len']}: {c['pairs_per_wave']} pairs/wave, "
+                f"rows/lane cap {c['max_rows_per_lane']}, waves/CU {c['waves_per_cu']}: "
+                f"{c['gcups']} GCUPS (built-in {c['default_gcups']})")
     t0 = time.time()
     plan = eng.plan(batch, default_config(use_graph=1, prior_table=args.prior_table,
-                                         waves_per_cu=args.waves_per_cu))
+                                         waves_per_cu=args.waves_per_cu, semantics=sem))
     log(f"plan + upload {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -234,7 +257,7 @@ def main():
         ceiling["frac_of_cell_mix_ceiling"] = round(cps / ceiling["cells_per_s"], 4)
         log(f"VALU ceiling: {ceiling}")
     traffic = MEASURED_TRAFFIC.get((args.config, args.scale)) \
-        if (args.waves_per_cu == 0 and args.prior_table) else None
+        if (args.waves_per_cu == 0 and args.prior_table and not args.autotune) else None
     res = {
         "metric": "GCUPS fwd/adjoint, batched vs plain, 1 MI355X, |dlog10L|<=1e-4",
         "value": round(gcups, 2),
@@ -255,6 +278,8 @@ def main():
         "plain_gcups": None if plain_ms is None else round(cells / (plain_ms / 1e3) / 1e9, 2),
         "batched_over_plain": None if plain_ms is None else round(plain_ms / ms, 3),
         "adjoint": adj,
+        "semantics": args.semantics,
+        "autotune": tuning,
         "rescued_pairs": rescued,
         "fp64_rescue_ms": round(fp64_ms, 3),
         "median_replay_ms": round(median_ms, 3),

@@ -103,6 +103,8 @@ typedef struct rodv_phmm_config {
     int32_t extended_range;       /* write finite log10 where FP64 underflows   */
     int32_t prior_table;          /* 1: per-pair LDS table of emission priors for
                                      N-free haplotypes (default); 0: select per cell */
+    int32_t use_tuning;           /* 1: shapes from rodv_phmm_autotune when the
+                                     context has been tuned (default)          */
 } rodv_phmm_config;
 
 /* Counters of the last execution of a plan. */
@@ -163,6 +165,44 @@ void rodv_phmm_plan_destroy(rodv_phmm_plan *plan);
  * ph2pr[128] and the triangular match-to-match table, first `mm_n` entries. */
 int rodv_phmm_tables_f32(float *ph2pr, float *mm, int mm_n);
 int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n);
+
+/*
+ * Autotuner (no counterpart in the reference: its SIMD widths are fixed by
+ * the host CPU).  Reads fall in RODV_PHMM_TUNE_CLASSES length classes
+ * (1..15, 16..30, ..., 226..240 rows, then 241..310, then longer).  For each
+ * class present in `sample`, rodv_phmm_autotune times the FP32 forward of
+ * (up to 131,072 of) the sample's pairs of that class under every candidate
+ * shape -- pairs per wavefront 4, 2 or 1 (16, 32 or 64 lanes per pair, rows
+ * per lane as few as the read needs), for the longest class the 64-lane
+ * strip width (rows per lane 4..16, shorter strips for longer reads) -- then
+ * the persistent waves per CU of the best shape, and keeps the fastest
+ * in the context.  Plans created afterwards use it (config use_tuning).  All
+ * shapes compute bit-identical results; only speed changes.  The tuning is
+ * per context, i.e. per device; set_tuning installs a saved one (NULL
+ * restores the built-in shapes) and get_tuning reads the current one.
+ */
+#define RODV_PHMM_TUNE_CLASSES 18
+
+typedef struct rodv_phmm_tune_entry {
+    uint32_t max_read_len;      /* the class's longest read (UINT32_MAX: open)  */
+    uint32_t pairs_per_wave;    /* 4, 2 or 1; 0: class not tuned (defaults)     */
+    uint32_t max_rows_per_lane; /* strip width cap of 64-lane shapes; 0 = 16    */
+    uint32_t waves_per_cu;      /* persistent grid per CU; 0 = engine default   */
+    uint32_t sample_pairs;      /* pairs of the sample timed for this class     */
+    double gcups;               /* measured with the chosen shape               */
+    double default_gcups;       /* measured with the built-in shape             */
+} rodv_phmm_tune_entry;
+
+typedef struct rodv_phmm_tuning {
+    uint32_t struct_size;       /* sizeof(rodv_phmm_tuning)                    */
+    uint32_t num_classes;       /* RODV_PHMM_TUNE_CLASSES                       */
+    rodv_phmm_tune_entry cls[RODV_PHMM_TUNE_CLASSES];
+} rodv_phmm_tuning;
+
+int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
+                       const rodv_phmm_config *cfg, rodv_phmm_tuning *out);
+int rodv_phmm_set_tuning(rodv_phmm_ctx *ctx, const rodv_phmm_tuning *tuning);
+int rodv_phmm_get_tuning(const rodv_phmm_ctx *ctx, rodv_phmm_tuning *out);
 
 /*
  * Adjoint (no counterpart in the reference).  Parametrised model:

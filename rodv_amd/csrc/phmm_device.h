@@ -33,10 +33,11 @@ __device__ __forceinline__ double ror1(double x) {
 // ds_bpermute, which runs in the LDS unit, so their arithmetic co-issues.
 template <int L>
 struct LaneRot {
+    static_assert(L == 16 || L == 32 || L == 64, "lanes per pair");
     int addr;
     __device__ __forceinline__ LaneRot() {
         const int lane = threadIdx.x;
-        const int src = L == 16 ? ((lane & ~15) | ((lane + 15) & 15)) : ((lane + 63) & 63);
+        const int src = (lane & ~(L - 1)) | ((lane + L - 1) & (L - 1));
         addr = src << 2;
     }
     __device__ __forceinline__ float operator()(float x) const {

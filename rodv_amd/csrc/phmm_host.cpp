@@ -173,8 +173,28 @@ constexpr int kUnroll = 8;  // must match kU in phmm_kernels.hip
 // shapes near 7, so 16 adds nothing)
 constexpr int kDefaultWavesPerCU = 12;
 
+// Read-length classes of the autotuner: class k (0-based) holds reads of
+// 15k+1 .. 15(k+1) rows for k < 16 (the 16-lane shapes' rows-per-lane
+// steps), then 241..310 and longer.
+constexpr uint32_t kClassMax[RODV_PHMM_TUNE_CLASSES] = {
+    15, 30, 45, 60, 75, 90, 105, 120, 135, 150, 165, 180, 195, 210, 225, 240, 310, 0xffffffffu};
+
+int tune_class(uint32_t R) {
+    int c = 0;
+    while (c < RODV_PHMM_TUNE_CLASSES - 1 && R > kClassMax[c]) c++;
+    return c;
+}
+
+// A class's shape: lanes per pair (16, 32, 64; 0 = engine default), the cap
+// on rows per lane of the 64-lane shapes (0 = 16), persistent waves per CU
+// (0 = engine default).
+struct TuneEntry {
+    int L = 0, mr_cap = 0, wpc = 0;
+};
+
 struct Bucket {
     int L = 16, MR = 1;
+    int wpc = 0;
     bool inter = false;
     bool tab = false;        // table-prior kernel (N-free haplotypes, one strip)
     uint32_t tab_bytes = 0;  // LDS prior table per group
@@ -204,6 +224,12 @@ struct rodv_phmm_ctx {
     int device = 0;
     int num_cus = 256;
     DevTables *d_tab = nullptr;
+    // autotuned shapes per read-length class (rodv_phmm_autotune)
+    bool tuned = false;
+    TuneEntry tune[RODV_PHMM_TUNE_CLASSES];
+    // forced entry while the tuner measures a candidate
+    int force_class = -1;
+    TuneEntry force{};
 };
 
 struct rodv_phmm_plan {
@@ -295,6 +321,33 @@ void choose_shape(uint32_t R, int short_lanes, int *L, int *MR) {
     int mr = (int)((R + 62) / 63);
     mr = ((mr + 3) / 4) * 4;
     *MR = std::min(16, std::max(4, mr));
+}
+
+// Shape of one pair: the class's tuned lanes per pair when the context has
+// one (rows per lane then as few as the read needs), else choose_shape.
+// 32-lane shapes exist only as table kernels (prior_tab), so pairs the table
+// kernel cannot take fall back to the default shape.
+void shape_for(uint32_t R, const rodv_phmm_config &cfg, const TuneEntry *te, bool tab_ok,
+               int *L, int *MR) {
+    if (te && te->L == 16 && R <= 240u) {
+        *L = 16;
+        *MR = std::max(1, (int)((R + 14) / 15));
+        return;
+    }
+    if (te && te->L == 32 && R <= 310u && tab_ok) {
+        *L = 32;
+        *MR = std::max(2, (int)((R + 30) / 31));
+        return;
+    }
+    if (te && te->L == 64) {
+        const int cap = te->mr_cap > 0 ? te->mr_cap : 16;
+        int mr = (int)((R + 62) / 63);
+        mr = ((mr + 3) / 4) * 4;
+        *L = 64;
+        *MR = std::min(cap, std::max(4, mr));
+        return;
+    }
+    choose_shape(R, cfg.short_lanes, L, MR);
 }
 
 uint32_t lds_stride_for(uint32_t max_h, int L) {
@@ -446,6 +499,7 @@ void rodv_phmm_config_default(rodv_phmm_config *cfg) {
     cfg->short_lanes = 16;
     cfg->extended_range = 0;
     cfg->prior_table = 1;
+    cfg->use_tuning = 1;
 }
 
 int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
@@ -635,25 +689,33 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         p->any_inter = !sched_list.empty();
         p->n_regions = (uint32_t)nr;
     }
-    // key = (tab ? 81 : 0) + (L == 16 ? 0 : 64) + MR, + 162 for inter pairs
-    std::vector<int> bucket_of_key(4 * 81, -1);
+    std::vector<int> bucket_of_key(4 * 3 * 17, -1);
     std::vector<std::vector<uint32_t>> members;
     uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
     for (uint64_t k = 0; k < n_pairs; k++) {
         const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
         p->cells += (uint64_t)R * H;
         if (R > 63u * 4u) max_long_h = std::max(max_long_h, H);
+        const int cls = tune_class(R);
+        const TuneEntry *te = nullptr;
+        if (ctx->force_class == cls)
+            te = &ctx->force;
+        else if (cfg.use_tuning && ctx->tuned)
+            te = &ctx->tune[cls];
+        const bool tab_ok = cfg.prior_table && !hap_has_n[ph[k]];
         int L, MR;
-        choose_shape(R, cfg.short_lanes, &L, &MR);
-        // a very long haplotype does not fit four groups' codes in LDS
-        if (L == 16 && 4ull * (2ull * lds_stride_for(H, 16) + fwd_tab_bytes(16, MR)) > 60 * 1024) {
+        shape_for(R, cfg, te, tab_ok, &L, &MR);
+        // a very long haplotype does not fit the groups' codes in LDS
+        if (L < 64 && (64ull / L) * (2ull * lds_stride_for(H, L) + fwd_tab_bytes(L, MR)) > 60 * 1024) {
             L = 64;
             MR = 4;
             while (MR < 16 && 63 * MR < (int)R) MR += 4;
         }
-        const bool tab = cfg.prior_table && !hap_has_n[ph[k]] && strips_for(R, L, MR) == 1;
+        const bool tab = tab_ok && strips_for(R, L, MR) == 1;
+        if (!tab && L == 32) choose_shape(R, cfg.short_lanes, &L, &MR);
         const bool inter = !inter32.empty() && inter32[k];
-        const int key = (inter ? 162 : 0) + (tab ? 81 : 0) + (L == 16 ? 0 : 64) + MR;
+        const int lidx = L == 16 ? 0 : (L == 32 ? 1 : 2);
+        const int key = (((inter ? 1 : 0) * 2 + (tab ? 1 : 0)) * 3 + lidx) * 17 + MR;
         if (bucket_of_key[key] < 0) {
             bucket_of_key[key] = (int)p->buckets.size();
             Bucket bk;
@@ -661,6 +723,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             bk.MR = MR;
             bk.tab = tab;
             bk.inter = inter;
+            bk.wpc = te ? te->wpc : 0;
             p->buckets.push_back(bk);
             members.emplace_back();
         }
@@ -701,7 +764,8 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         } else {
             bk.lds_stride = lds_stride_for(bk.max_h, bk.L);
         }
-        const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu : kDefaultWavesPerCU;
+        const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu
+                                               : (bk.wpc > 0 ? bk.wpc : kDefaultWavesPerCU);
         bk.grid = (int)std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus * wpc);
         if (bk.grid < 1) bk.grid = 1;
         if (bk.max_strips > 1) {
@@ -1017,6 +1081,182 @@ int rodv_phmm_plan_profile(rodv_phmm_plan *p, void *stream) {
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+// ---------------------------------------------------------------- autotuner
+
+namespace {
+constexpr uint32_t kMaxTunePairs = 131072;
+
+int lanes_of(uint32_t pairs_per_wave) {
+    return pairs_per_wave == 4 ? 16 : pairs_per_wave == 2 ? 32 : pairs_per_wave == 1 ? 64 : 0;
+}
+
+// FP32 forward of the sub-batch under the context's current (forced) shape:
+// best of three profiled executions after a warm-up, in GCUPS.
+int time_candidate(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sub, const rodv_phmm_config &cfg,
+                   double *gcups) {
+    rodv_phmm_plan *p = nullptr;
+    int rc = rodv_phmm_plan_create(ctx, sub, &cfg, &p);
+    if (rc) return rc;
+    double best = 1e30;
+    for (int r = 0; r < 4 && !rc; r++) {
+        rc = rodv_phmm_plan_profile(p, nullptr);
+        if (!rc && r > 0) best = std::min(best, p->stats.fp32_kernel_ms);
+    }
+    const uint64_t cells = p->cells;
+    rodv_phmm_plan_destroy(p);
+    if (rc) return rc;
+    *gcups = best > 0 ? (double)cells / best / 1e6 : 0.0;
+    return RODV_PHMM_OK;
+}
+}  // namespace
+
+int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
+                       const rodv_phmm_config *cfg_in, rodv_phmm_tuning *out) {
+    if (!ctx || !sample || !out) return RODV_PHMM_ERR_INVALID_ARG;
+    int rc = validate(sample);
+    if (rc) return rc;
+    rodv_phmm_config cfg;
+    rodv_phmm_config_default(&cfg);
+    if (cfg_in) {
+        if (cfg_in->struct_size != sizeof(rodv_phmm_config)) return RODV_PHMM_ERR_INVALID_ARG;
+        cfg = *cfg_in;
+    }
+    cfg.semantics = RODV_PHMM_SEMANTICS_INTRA;  // shapes do not depend on routing
+    cfg.use_graph = 0;
+    cfg.use_tuning = 0;
+    cfg.waves_per_cu = 0;
+    // the sample's pairs by class, every k-th so that each class keeps at
+    // most kMaxTunePairs spread over the whole sample
+    std::vector<rodv_phmm_region> regions;
+    if (sample->num_regions == 0)
+        regions.push_back(rodv_phmm_region{0, sample->num_haps, 0, sample->num_reads});
+    else
+        regions.assign(sample->regions, sample->regions + sample->num_regions);
+    std::vector<uint64_t> count(RODV_PHMM_TUNE_CLASSES, 0);
+    for (const auto &rg : regions)
+        for (uint32_t c = 0; c < rg.num_reads; c++)
+            count[tune_class(sample->read_lengths[rg.read_begin + c])] += rg.num_haps;
+    std::vector<std::vector<rodv_phmm_region>> picked(RODV_PHMM_TUNE_CLASSES);
+    std::vector<uint64_t> seen(RODV_PHMM_TUNE_CLASSES, 0);
+    for (const auto &rg : regions)
+        for (uint32_t a = 0; a < rg.num_haps; a++)
+            for (uint32_t c = 0; c < rg.num_reads; c++) {
+                const int k = tune_class(sample->read_lengths[rg.read_begin + c]);
+                const uint64_t stride = (count[k] + kMaxTunePairs - 1) / kMaxTunePairs;
+                if (seen[k]++ % stride == 0 && picked[k].size() < kMaxTunePairs)
+                    picked[k].push_back(rodv_phmm_region{rg.hap_begin + a, 1, rg.read_begin + c, 1});
+            }
+    rodv_phmm_tuning res{};
+    res.struct_size = sizeof(res);
+    res.num_classes = RODV_PHMM_TUNE_CLASSES;
+    TuneEntry tuned[RODV_PHMM_TUNE_CLASSES];
+    for (int k = 0; k < RODV_PHMM_TUNE_CLASSES && !rc; k++) {
+        rodv_phmm_tune_entry &e = res.cls[k];
+        e.max_read_len = kClassMax[k];
+        if (picked[k].size() < 64) continue;  // too few pairs to time: keep the defaults
+        rodv_phmm_batch sub = *sample;
+        sub.num_regions = (uint32_t)picked[k].size();
+        sub.regions = picked[k].data();
+        e.sample_pairs = sub.num_regions;
+        // built-in shape
+        ctx->force_class = k;
+        ctx->force = TuneEntry{};
+        rc = time_candidate(ctx, &sub, cfg, &e.default_gcups);
+        if (rc) break;
+        TuneEntry best{};
+        double best_g = e.default_gcups;
+        std::vector<TuneEntry> cands;
+        if (kClassMax[k] <= 240) cands.push_back(TuneEntry{16, 0, 0});
+        if (kClassMax[k] <= 310) cands.push_back(TuneEntry{32, 0, 0});
+        if (kClassMax[k] <= 310) {
+            cands.push_back(TuneEntry{64, 0, 0});
+        } else {
+            for (int cap : {4, 8, 12, 16}) cands.push_back(TuneEntry{64, cap, 0});
+        }
+        for (const TuneEntry &c : cands) {
+            ctx->force = c;
+            double g = 0.0;
+            rc = time_candidate(ctx, &sub, cfg, &g);
+            if (rc) break;
+            if (g > best_g) {
+                best_g = g;
+                best = c;
+            }
+        }
+        if (rc) break;
+        if (best.L == 0) {  // the built-in shape won: name it
+            int L = 16, MR = 1;
+            choose_shape(kClassMax[k] == 0xffffffffu ? 1000u : kClassMax[k], cfg.short_lanes, &L, &MR);
+            best.L = L;
+        }
+        for (int w : {4, 8, 16}) {
+            TuneEntry c = best;
+            c.wpc = w;
+            ctx->force = c;
+            double g = 0.0;
+            rc = time_candidate(ctx, &sub, cfg, &g);
+            if (rc) break;
+            if (g > best_g * 1.01) {  // a clear gain over the default grid only
+                best_g = g;
+                best.wpc = w;
+            }
+        }
+        if (rc) break;
+        tuned[k] = best;
+        e.pairs_per_wave = 64u / (uint32_t)best.L;
+        e.max_rows_per_lane = (uint32_t)best.mr_cap;
+        e.waves_per_cu = (uint32_t)best.wpc;
+        e.gcups = best_g;
+    }
+    ctx->force_class = -1;
+    ctx->force = TuneEntry{};
+    if (rc) return rc;
+    for (int k = 0; k < RODV_PHMM_TUNE_CLASSES; k++) ctx->tune[k] = tuned[k];
+    ctx->tuned = true;
+    *out = res;
+    return RODV_PHMM_OK;
+}
+
+int rodv_phmm_set_tuning(rodv_phmm_ctx *ctx, const rodv_phmm_tuning *t) {
+    if (!ctx) return RODV_PHMM_ERR_INVALID_ARG;
+    if (!t) {
+        ctx->tuned = false;
+        for (auto &e : ctx->tune) e = TuneEntry{};
+        return RODV_PHMM_OK;
+    }
+    if (t->struct_size != sizeof(rodv_phmm_tuning) || t->num_classes != RODV_PHMM_TUNE_CLASSES)
+        return RODV_PHMM_ERR_INVALID_ARG;
+    TuneEntry tmp[RODV_PHMM_TUNE_CLASSES];
+    for (int k = 0; k < RODV_PHMM_TUNE_CLASSES; k++) {
+        const rodv_phmm_tune_entry &e = t->cls[k];
+        const uint32_t ppw = e.pairs_per_wave, cap = e.max_rows_per_lane;
+        if (ppw != 0 && lanes_of(ppw) == 0) return RODV_PHMM_ERR_INVALID_ARG;
+        if (cap != 0 && cap != 4 && cap != 8 && cap != 12 && cap != 16) return RODV_PHMM_ERR_INVALID_ARG;
+        if (e.waves_per_cu > 32) return RODV_PHMM_ERR_INVALID_ARG;
+        tmp[k] = TuneEntry{lanes_of(ppw), (int)cap, (int)e.waves_per_cu};
+    }
+    for (int k = 0; k < RODV_PHMM_TUNE_CLASSES; k++) ctx->tune[k] = tmp[k];
+    ctx->tuned = true;
+    return RODV_PHMM_OK;
+}
+
+int rodv_phmm_get_tuning(const rodv_phmm_ctx *ctx, rodv_phmm_tuning *out) {
+    if (!ctx || !out) return RODV_PHMM_ERR_INVALID_ARG;
+    rodv_phmm_tuning r{};
+    r.struct_size = sizeof(r);
+    r.num_classes = RODV_PHMM_TUNE_CLASSES;
+    for (int k = 0; k < RODV_PHMM_TUNE_CLASSES; k++) {
+        r.cls[k].max_read_len = kClassMax[k];
+        if (!ctx->tuned) continue;
+        const TuneEntry &e = ctx->tune[k];
+        r.cls[k].pairs_per_wave = e.L ? 64u / (uint32_t)e.L : 0u;
+        r.cls[k].max_rows_per_lane = (uint32_t)e.mr_cap;
+        r.cls[k].waves_per_cu = (uint32_t)e.wpc;
+    }
+    *out = r;
+    return RODV_PHMM_OK;
 }
 
 int rodv_phmm_plan_read(rodv_phmm_plan *p, double *out_log10, uint8_t *out_flags,

@@ -81,14 +81,18 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// min / max of a per-group value over the G groups of the wave (lanes 0,
+// L, 2L, ... hold each group's copy)
 __device__ __forceinline__ int wave_min4(int v, int G) {
     if (G == 1) return __builtin_amdgcn_readfirstlane(v);
+    if (G == 2) return min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
     int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
     int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
     return min(min(a, b), min(c, d));
 }
 __device__ __forceinline__ int wave_max4(int v, int G) {
     if (G == 1) return __builtin_amdgcn_readfirstlane(v);
+    if (G == 2) return max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
     int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
     int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
     return max(max(a, b), max(c, d));
@@ -696,27 +700,31 @@ void *fwd_f32_kernel(int L, int MR, bool inter) {
     return nullptr;
 }
 
-// Table-prior FP32 kernels (haplotypes without N, one strip).
+// Table-prior FP32 kernels (haplotypes without N, one strip): 16 lanes per
+// pair with 1..16 rows per lane, 32 lanes with 2..10, 64 lanes with 4, 8, 12, 16.
 void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
-#define T16(m) \
-    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<16, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<16, m, false>);
-#define T64(m) \
-    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<64, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<64, m, false>);
+#define TK(l, m) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<l, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<l, m, false>);
     if (L == 16) {
         switch (MR) {
-            T16(1) T16(2) T16(3) T16(4) T16(5) T16(6) T16(7) T16(8)
-            T16(9) T16(10) T16(11) T16(12) T16(13) T16(14) T16(15) T16(16)
+            TK(16, 1) TK(16, 2) TK(16, 3) TK(16, 4) TK(16, 5) TK(16, 6) TK(16, 7) TK(16, 8)
+            TK(16, 9) TK(16, 10) TK(16, 11) TK(16, 12) TK(16, 13) TK(16, 14) TK(16, 15) TK(16, 16)
+        default: return nullptr;
+        }
+    }
+    if (L == 32) {
+        switch (MR) {
+            TK(32, 2) TK(32, 3) TK(32, 4) TK(32, 5) TK(32, 6) TK(32, 7) TK(32, 8) TK(32, 9) TK(32, 10)
         default: return nullptr;
         }
     }
     if (L == 64) {
         switch (MR) {
-            T64(4) T64(8) T64(12) T64(16)
+            TK(64, 4) TK(64, 8) TK(64, 12) TK(64, 16)
         default: return nullptr;
         }
     }
-#undef T16
-#undef T64
+#undef TK
     return nullptr;
 }
 uint32_t fwd_tab_bytes(int L, int MR) { return 4u * (uint32_t)L * (uint32_t)((MR + 3) / 4 * 4) * 4u; }

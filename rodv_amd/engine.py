@@ -31,7 +31,8 @@ EXPORTS = (
     "rodv_phmm_plan_create", "rodv_phmm_plan_execute", "rodv_phmm_plan_execute_plain",
     "rodv_phmm_plan_profile", "rodv_phmm_plan_read", "rodv_phmm_plan_stats",
     "rodv_phmm_plan_num_pairs", "rodv_phmm_plan_destroy", "rodv_phmm_tables_f32",
-    "rodv_phmm_tables_f64", "rodv_phmm_adjoint",
+    "rodv_phmm_tables_f64", "rodv_phmm_adjoint", "rodv_phmm_autotune",
+    "rodv_phmm_set_tuning", "rodv_phmm_get_tuning",
 )
 
 
@@ -61,7 +62,26 @@ class Config(C.Structure):
         ("max_idle_ratio_float", C.c_double), ("max_idle_ratio_double", C.c_double),
         ("simd_width_float", C.c_uint32), ("simd_width_double", C.c_uint32),
         ("use_graph", C.c_int32), ("waves_per_cu", C.c_int32), ("short_lanes", C.c_int32),
-        ("extended_range", C.c_int32), ("prior_table", C.c_int32)]
+        ("extended_range", C.c_int32), ("prior_table", C.c_int32), ("use_tuning", C.c_int32)]
+
+
+TUNE_CLASSES = 18
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [
+        ("max_read_len", C.c_uint32), ("pairs_per_wave", C.c_uint32),
+        ("max_rows_per_lane", C.c_uint32), ("waves_per_cu", C.c_uint32),
+        ("sample_pairs", C.c_uint32), ("gcups", C.c_double), ("default_gcups", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("num_classes", C.c_uint32),
+                ("cls", TuneEntry * TUNE_CLASSES)]
+
+    def entries(self):
+        """The classes as dicts (pairs_per_wave 0: built-in shape)."""
+        return [{f: getattr(e, f) for f, _ in TuneEntry._fields_} for e in self.cls]
 
 
 class Stats(C.Structure):
@@ -109,6 +129,9 @@ def load_library(path: str = LIB_PATH):
     L.rodv_phmm_tables_f32.argtypes = [vp, vp, i32]
     L.rodv_phmm_tables_f64.argtypes = [vp, vp, i32]
     L.rodv_phmm_adjoint.argtypes = [vp, C.POINTER(CBatch), C.POINTER(Params), vp, i32, vp, vp, vp]
+    L.rodv_phmm_autotune.argtypes = [vp, C.POINTER(CBatch), C.POINTER(Config), C.POINTER(Tuning)]
+    L.rodv_phmm_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.rodv_phmm_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     for name in EXPORTS:
         getattr(L, name)  # every declared entry point must be exported
     _lib = L
@@ -222,6 +245,26 @@ class Engine:
         out, _ = self.forward(b, cfg)
         result[:] = [list(row) for row in out.reshape(M, N)]
         return True
+
+    def autotune(self, sample: Batch, config: Config | None = None) -> Tuning:
+        """Times every candidate shape per read-length class on `sample`
+        (rodv_phmm_autotune) and keeps the fastest in this context."""
+        hold = _CBatchHolder(sample)
+        t = Tuning()
+        cfg = config if config is not None else default_config()
+        _check(self._lib.rodv_phmm_autotune(self._h, C.byref(hold.c), C.byref(cfg), C.byref(t)),
+               "autotune")
+        return t
+
+    def set_tuning(self, tuning: Tuning | None):
+        """Installs a saved tuning; None restores the built-in shapes."""
+        _check(self._lib.rodv_phmm_set_tuning(self._h, None if tuning is None else C.byref(tuning)),
+               "set_tuning")
+
+    def get_tuning(self) -> Tuning:
+        t = Tuning()
+        _check(self._lib.rodv_phmm_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return t
 
     def adjoint(self, batch: Batch, theta, weights=None, ckpt_stride=32, timing=False):
         """log10 L per pair and grad[10] = sum_k w_k d log10 L_k / d theta for
