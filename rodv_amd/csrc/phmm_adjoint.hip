@@ -12,7 +12,9 @@
 //     its checkpoint, saving the state every 4 steps in LDS (level 2);
 //  3. for each 4-step window, last to first: recompute the 4 states into
 //     registers and run the adjoint of those steps in reverse, lanes reading
-//     the row below their last row from lane l+1 with DPP wave_rol:1.
+//     the row below their last row from lane l+1.
+// Values cross lanes through ds_bpermute (LaneXfer), not DPP, so that two
+// waves of a SIMD can co-issue the arithmetic.
 // Adjoint seeds are sigma = 2^-ilogb(S) so that adjoint x forward products
 // stay O(expected counts); the gradient of log10 L follows at the end.
 // Per real cell (i, j) with Tb = p * Mb (p the emission prior):
@@ -104,13 +106,14 @@ __device__ __forceinline__ void adj_step(const State<T, MR> &Sb, const State<T, 
                                          const Priors<T, MR> &P, const AdjRows<T, MR> &A,
                                          uint32_t hc, T seed, T (&Tb)[MR], T (&Ib)[MR],
                                          T (&Db)[MR], T &Tb0p, T (&gd)[MR], T (&gg)[MR],
-                                         T (&ge)[MR]) {
+                                         T (&ge)[MR], const LaneXfer<kL, 1> &from_above,
+                                         const LaneXfer<kL, -1> &from_below) {
     // forward inputs of slot 0 from lane l-1
-    const T dgM0 = ror1<kL>(Sb.Mp), dgI0 = ror1<kL>(Sb.Ip), dgD0 = ror1<kL>(Sb.Dp);
-    const T upM0 = ror1<kL>(Sb.M[MR - 1]), upI0 = ror1<kL>(Sb.I[MR - 1]);
+    const T dgM0 = from_above(Sb.Mp), dgI0 = from_above(Sb.Ip), dgD0 = from_above(Sb.Dp);
+    const T upM0 = from_above(Sb.M[MR - 1]), upI0 = from_above(Sb.I[MR - 1]);
     // adjoint inputs of the last slot from lane l+1
-    T bT = rol1<kL>(Tb0p);   // Tb(i+1, j+1)
-    T bI = rol1<kL>(Ib[0]);  // Ib(i+1, j)
+    T bT = from_below(Tb0p);   // Tb(i+1, j+1)
+    T bI = from_below(Ib[0]);  // Ib(i+1, j)
     T b_mm = A.nb_mm, b_g = A.nb_g, b_mx = A.nb_mx, b_xx = A.nb_xx;
     T oldT0 = Tb[0];
 #pragma unroll
@@ -158,6 +161,8 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
     const int l = lane;
     constexpr int NV = 3 * MR + 3;
     T *l2 = reinterpret_cast<T *>(lds + a.lds_stride);  // level-2 states
+    const LaneXfer<kL, 1> up;
+    const LaneXfer<kL, -1> down;
     const int e = ilogb((double)Ssum);
     const T sigma = (T)ldexp(1.0, -e);
 
@@ -175,7 +180,7 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
         for (int w = 0; w < nwin; w++) {
             put_state(l2, NV, w, S);
 #pragma unroll
-            for (int u = 0; u < kW; u++) step<T, kL, MR, false>(S, P, cp[d0 + w * kW + u + 1]);
+            for (int u = 0; u < kW; u++) step<T, kL, MR, false>(S, P, cp[d0 + w * kW + u + 1], up);
         }
         __syncthreads();
         for (int w = nwin - 1; w >= 0; w--) {
@@ -183,21 +188,21 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
             State<T, MR> W0, W1, W2, W3, W4;
             get_state(l2, NV, w, W0);
             W1 = W0;
-            step<T, kL, MR, false>(W1, P, cp[dw + 1]);
+            step<T, kL, MR, false>(W1, P, cp[dw + 1], up);
             W2 = W1;
-            step<T, kL, MR, false>(W2, P, cp[dw + 2]);
+            step<T, kL, MR, false>(W2, P, cp[dw + 2], up);
             W3 = W2;
-            step<T, kL, MR, false>(W3, P, cp[dw + 3]);
+            step<T, kL, MR, false>(W3, P, cp[dw + 3], up);
             W4 = W3;
-            step<T, kL, MR, false>(W4, P, cp[dw + 4]);
+            step<T, kL, MR, false>(W4, P, cp[dw + 4], up);
             auto seed_at = [&](int d) {
                 const int j = d - l;
                 return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
             };
-            adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge);
-            adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge);
-            adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge);
-            adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge);
+            adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+            adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+            adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+            adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
         }
     }
 
@@ -318,8 +323,8 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     S0.Dp = S0.D[MR - 1];
     {
         // transitions of the row below this lane's last row: lane l+1's slot 0
-        T m = rol1<kL>(P.mm[0]), g = rol1<kL>(P.g[0]), x = rol1<kL>(P.mx[0]),
-          xx = rol1<kL>(P.xx[0]);
+        const LaneXfer<kL, -1> down;
+        T m = down(P.mm[0]), g = down(P.g[0]), x = down(P.mx[0]), xx = down(P.xx[0]);
         const bool wrap = l == kL - 1;
         A.nb_mm = wrap ? T(0) : m;
         A.nb_g = wrap ? T(0) : g;
@@ -336,8 +341,9 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     State<T, MR> S = S0;
     T sm = T(0), si = T(0);
     put_state(ck, NV, 0, S);
+    const LaneXfer<kL, 1> up;
     for (int d = 1; d <= D4; d++) {
-        step<T, kL, MR, false>(S, P, cp[d]);
+        step<T, kL, MR, false>(S, P, cp[d], up);
         if (d <= Dend && d - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];

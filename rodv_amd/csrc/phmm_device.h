@@ -26,24 +26,33 @@ __device__ __forceinline__ double ror1(double x) {
     return __hiloint2double(ror1_i<L>(__double2hiint(x)), ror1_i<L>(__double2loint(x)));
 }
 
-// Lane rotation through the LDS crossbar (ds_bpermute): lane l receives lane
-// l-1 of its L-lane group, lane 0 the group's last lane -- the same movement
-// as ror1<L>.  On gfx950 two waves of a SIMD co-issue VGPR-only VALU
+// Lane rotation through the LDS crossbar (ds_bpermute).  DIR = +1: lane l
+// receives lane l-1 of its L-lane group and lane 0 the group's last lane (the
+// movement of ror1<L>); DIR = -1: lane l receives lane l+1 and the last lane
+// lane 0 (rol1<L>).  On gfx950 two waves of a SIMD co-issue VGPR-only VALU
 // instructions, but not DPP-modified ones: the hot loops rotate with
 // ds_bpermute, which runs in the LDS unit, so their arithmetic co-issues.
-template <int L>
-struct LaneRot {
+// Every lane of the wave must be active.
+template <int L, int DIR>
+struct LaneXfer {
     static_assert(L == 16 || L == 32 || L == 64, "lanes per pair");
+    static_assert(DIR == 1 || DIR == -1, "direction");
     int addr;
-    __device__ __forceinline__ LaneRot() {
+    __device__ __forceinline__ LaneXfer() {
         const int lane = threadIdx.x;
-        const int src = (lane & ~(L - 1)) | ((lane + L - 1) & (L - 1));
+        const int src = (lane & ~(L - 1)) | ((lane + (DIR > 0 ? L - 1 : 1)) & (L - 1));
         addr = src << 2;
     }
     __device__ __forceinline__ float operator()(float x) const {
         return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
     }
+    __device__ __forceinline__ double operator()(double x) const {
+        return __hiloint2double(__builtin_amdgcn_ds_bpermute(addr, __double2hiint(x)),
+                                __builtin_amdgcn_ds_bpermute(addr, __double2loint(x)));
+    }
 };
+template <int L>
+using LaneRot = LaneXfer<L, 1>;
 
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -106,13 +115,22 @@ struct Priors {
     uint32_t rpos[MR];
 };
 
-// One anti-diagonal step of one lane: all MR rows at the next column.
-template <typename T, int L, int MR, bool INTER>
-__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, uint32_t hc) {
+// The row above the lane's first row through DPP (ror1<L>).
+template <int L>
+struct DppRot {
+    template <typename T>
+    __device__ __forceinline__ T operator()(T x) const { return ror1<L>(x); }
+};
+
+// One anti-diagonal step of one lane: all MR rows at the next column.  `rot`
+// brings the previous lane's values (DppRot or LaneRot).
+template <typename T, int L, int MR, bool INTER, typename Rot = DppRot<L>>
+__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, uint32_t hc,
+                                     const Rot &rot = Rot()) {
     // row above the lane's first row: previous lane's last row, at this column
     // (up) and the column before (diagonal)
-    T dgM = ror1<L>(S.Mp), dgI = ror1<L>(S.Ip), dgD = ror1<L>(S.Dp);
-    T upM = ror1<L>(S.M[MR - 1]), upI = ror1<L>(S.I[MR - 1]);
+    T dgM = rot(S.Mp), dgI = rot(S.Ip), dgD = rot(S.Dp);
+    T upM = rot(S.M[MR - 1]), upI = rot(S.I[MR - 1]);
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
