@@ -27,8 +27,9 @@ FP_INSTR_PER_CELL = 8     # non-packed FP32 instructions of the bit-exact FMA sp
 # HBM bytes per FP32 forward pass (all length buckets), from two rocprofv3
 # --pmc runs of this script on the same workload: FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, the factors a 1 GiB device copy in the same run gives
-# (--copy-calibration-mib 1024).  Keyed by (config, scale).
-MEASURED_TRAFFIC = {(2, 1.0): 2.5104e10}
+# (--copy-calibration-mib 1024).  Keyed by (config, scale, semantics,
+# autotuned).
+MEASURED_TRAFFIC = {(2, 1.0, "schedule", True): 2.557e10}
 
 
 def log(msg):
@@ -256,8 +257,8 @@ def main():
         ceiling["forward_cells_per_s"] = round(cps, 1)
         ceiling["frac_of_cell_mix_ceiling"] = round(cps / ceiling["cells_per_s"], 4)
         log(f"VALU ceiling: {ceiling}")
-    traffic = MEASURED_TRAFFIC.get((args.config, args.scale)) \
-        if (args.waves_per_cu == 0 and args.prior_table and not args.autotune) else None
+    traffic = MEASURED_TRAFFIC.get((args.config, args.scale, args.semantics, bool(args.autotune))) \
+        if (args.waves_per_cu == 0 and args.prior_table) else None
     res = {
         "metric": "GCUPS fwd/adjoint, batched vs plain, 1 MI355X, |dlog10L|<=1e-4",
         "value": round(gcups, 2),
