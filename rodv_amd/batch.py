@@ -17,6 +17,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import re
+
 import numpy as np
 
 VALID_BASES = frozenset(b"ACGTN")
@@ -163,18 +165,22 @@ def parse_gkl_testdata(path):
     return Batch.from_lists(haps, reads, qs, ins, dels, gcps, regions), np.array(exp)
 
 
-def parse_assemble_log(path):
+def parse_assemble_log(path, with_names=False):
     """resouces/pairhmm_debug.txt region dump (test/assemble_log_parser.cpp:
-    13-149): '=== Region' headers, 'H<k>: bases', 'R<k>: bases' followed by
-    integer read-qual / read-ins-qual / read-del-qual / gcp lines."""
+    13-149): '=== Region: <name> ==' headers, 'H<k>: bases', 'R<k>: bases'
+    followed by integer read-qual / read-ins-qual / read-del-qual / gcp lines.
+    Returns one Batch per region (with with_names, (name, Batch) pairs)."""
     batches = []
+    names = []
     haps, reads, qs, ins, dels, gcps = [], [], [], [], [], []
     cur = None
+    name = [""]
 
     def flush():
         if haps and reads:
             batches.append(Batch.from_lists(list(haps), list(reads), list(qs), list(ins),
                                             list(dels), list(gcps)))
+            names.append(name[0])
         for lst in (haps, reads, qs, ins, dels, gcps):
             lst.clear()
 
@@ -183,6 +189,8 @@ def parse_assemble_log(path):
             line = raw.strip()
             if line.startswith("=== Region"):
                 flush()
+                m = re.search(r"=== Region: (.+) ==", line)
+                name[0] = m.group(1) if m else ""
                 continue
             if not line or ":" not in line:
                 continue
@@ -203,4 +211,4 @@ def parse_assemble_log(path):
                 gcps.append([int(x) for x in val.split()])
     flush()
     del cur
-    return batches
+    return list(zip(names, batches)) if with_names else batches
