@@ -47,11 +47,12 @@ def _cpu_model():
     return "unknown"
 
 
-def cpu_baseline(batch, seconds=12.0, seed=7):
+def cpu_baseline(batch, seconds=12.0, seed=7, all_core_seconds=6.0):
     """Time the reference's own intra kernel (compiled from its sources into
     oracle/_ref) on host core 0, one thread, over seeded random regions of
-    the same workload until about `seconds` of CPU work; falls back to the
-    oracle restatement when the reference build is unavailable here."""
+    the same workload until about `seconds` of CPU work, then on every core
+    for about `all_core_seconds`; falls back to the oracle restatement (one
+    core) when the reference build is unavailable here."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     order = np.random.default_rng(seed).permutation(len(batch.regions))
@@ -88,10 +89,29 @@ def cpu_baseline(batch, seconds=12.0, seed=7):
     dt = time.perf_counter() - t0
     what = (f"intra::computeLikelihoods{ref.isa.upper()} per pair (reference sources, "
             "g++ -O3)" if ref is not None else "oracle restatement, scalar")
-    return {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1,
-            "kind": "reference" if ref is not None else "port",
-            "sample": f"{what}; {regions} random regions of the workload, {pairs} pairs, "
-                      f"{cells:.3e} cells in {dt:.1f} s; host {_cpu_model()}"}
+    res = {"value": round(cells / dt / 1e9, 4), "unit": "GCUPS", "cores": 1,
+           "kind": "reference" if ref is not None else "port",
+           "sample": f"{what}; {regions} random regions of the workload, {pairs} pairs, "
+                     f"{cells:.3e} cells in {dt:.1f} s; host {_cpu_model()}"}
+    if ref is not None and all_core_seconds > 0:
+        # the same per-pair kernel on every host core of this process (at most
+        # 16: the share of a one-GPU box), threads over pairs
+        nt = max(1, min(16, len(os.sched_getaffinity(0))))
+        target = res["value"] * 1e9 * nt * all_core_seconds
+        hs, rs = [], []
+        c2 = 0
+        for h0, m, r0, n, c in sample_regions():
+            hs.append(np.repeat(np.arange(h0, h0 + m, dtype=np.uint32), n))
+            rs.append(np.tile(np.arange(r0, r0 + n, dtype=np.uint32), m))
+            c2 += c
+            if c2 >= target:
+                break
+        t1 = time.perf_counter()
+        ref.log10_batch(batch, (np.concatenate(hs), np.concatenate(rs)), nt)
+        d2 = time.perf_counter() - t1
+        res["all_cores"] = {"value": round(c2 / d2 / 1e9, 4), "cores": nt,
+                            "sample": f"{len(hs)} random regions, {c2:.3e} cells in {d2:.1f} s"}
+    return res
 
 
 def valu_ceiling():

@@ -236,6 +236,10 @@ class RefLib:
         self._log10 = getattr(L, pre + "intra_log10")
         self._log10.restype = C.c_double
         self._log10.argtypes = args + [C.c_int]
+        self._batch = getattr(L, pre + "intra_log10_batch")
+        self._batch.restype = C.c_int
+        self._batch.argtypes = [_u8p, _u64p, _u32p, _u8p, _u8p, _u8p, _u8p, _u8p, _u64p, _u32p,
+                                C.c_uint64, _u32p, _u32p, C.c_int, _f64p]
         self._t32 = getattr(L, pre + "tables_f32")
         self._t32.argtypes = [_f32p, _f32p, C.c_int]
         self._t64 = getattr(L, pre + "tables_f64")
@@ -266,6 +270,18 @@ class RefLib:
 
     def log10(self, batch, h, r, use_double=False):
         return self._log10(*self._pair_args(batch, h, r), int(use_double))
+
+    def log10_batch(self, batch, pairs, nthreads):
+        """computeLikelihoods over the (hap, read) index arrays on nthreads."""
+        ph = np.ascontiguousarray(pairs[0], np.uint32)
+        pr = np.ascontiguousarray(pairs[1], np.uint32)
+        out = np.zeros(len(ph), np.float64)
+        self._batch(_p(batch.hap_bases, _u8p), _p(batch.hap_off, _u64p), _p(batch.hap_len, _u32p),
+                    _p(batch.read_bases, _u8p), _p(batch.base_q, _u8p), _p(batch.ins_q, _u8p),
+                    _p(batch.del_q, _u8p), _p(batch.gcp_q, _u8p), _p(batch.read_off, _u64p),
+                    _p(batch.read_len, _u32p), len(ph), _p(ph, _u32p), _p(pr, _u32p),
+                    int(nthreads), _p(out, _f64p))
+        return out
 
     def tables_f32(self):
         ph = np.zeros(128, np.float32)

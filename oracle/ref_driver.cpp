@@ -6,9 +6,13 @@
 // it wraps the AVX-512 intra kernel, with -mavx2 the AVX2 intra kernel and
 // the AVX2 inter group kernel (the AVX-512 inter traits include a boost header
 // this image does not have, so that variant is not built).
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "pairhmm/common/common.h"
 #include "pairhmm/common/context.h"
@@ -76,6 +80,37 @@ double REF_SYM(intra_log10)(uint32_t rlen, uint32_t hlen, const uint8_t *hap, co
 #else
     return pairhmm::intra::computeLikelihoodsAVX2(tc, use_double != 0);
 #endif
+}
+
+// The per-pair API over many pairs on `nthreads` host threads (pairs handed
+// out one at a time), for the CPU baseline on every core.  Flat arrays as in
+// phmm_oracle_batch.
+int REF_SYM(intra_log10_batch)(const uint8_t *hap_bases, const uint64_t *hap_off,
+                               const uint32_t *hap_len, const uint8_t *read_bases,
+                               const uint8_t *q, const uint8_t *ins, const uint8_t *del,
+                               const uint8_t *gcp, const uint64_t *read_off,
+                               const uint32_t *read_len, uint64_t n, const uint32_t *pair_hap,
+                               const uint32_t *pair_read, int nthreads, double *out) {
+    std::atomic<uint64_t> next{0};
+    auto work = [&]() {
+        for (uint64_t k = next++; k < n; k = next++) {
+            const uint32_t h = pair_hap[k], r = pair_read[k];
+            TestCase tc = make_tc(read_len[r], hap_len[h], hap_bases + hap_off[h],
+                                  read_bases + read_off[r], q + read_off[r], ins + read_off[r],
+                                  del + read_off[r], gcp + read_off[r]);
+#if defined(__AVX512F__)
+            out[k] = pairhmm::intra::computeLikelihoodsAVX512(tc, false);
+#else
+            out[k] = pairhmm::intra::computeLikelihoodsAVX2(tc, false);
+#endif
+        }
+    };
+    const int nt = std::max(1, nthreads);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    return 0;
 }
 
 // Tables as Context<float>/Context<double> build them (context.h:90-166).

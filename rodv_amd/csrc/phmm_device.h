@@ -11,27 +11,12 @@ namespace dev {
 
 // ---------------------------------------------------------------- helpers
 
-template <int L>
-__device__ __forceinline__ int ror1_i(int x) {
-    // row_ror:1 (0x121) rotates within a 16-lane row, wave_ror:1 (0x13C)
-    // across the wave: lane l receives lane l-1, lane 0 receives the last.
-    return __builtin_amdgcn_mov_dpp(x, L == 16 ? 0x121 : 0x13C, 0xf, 0xf, true);
-}
-template <int L>
-__device__ __forceinline__ float ror1(float x) {
-    return __int_as_float(ror1_i<L>(__float_as_int(x)));
-}
-template <int L>
-__device__ __forceinline__ double ror1(double x) {
-    return __hiloint2double(ror1_i<L>(__double2hiint(x)), ror1_i<L>(__double2loint(x)));
-}
-
 // Lane rotation through the LDS crossbar (ds_bpermute).  DIR = +1: lane l
-// receives lane l-1 of its L-lane group and lane 0 the group's last lane (the
-// movement of ror1<L>); DIR = -1: lane l receives lane l+1 and the last lane
-// lane 0 (rol1<L>).  On gfx950 two waves of a SIMD co-issue VGPR-only VALU
-// instructions, but not DPP-modified ones: the hot loops rotate with
-// ds_bpermute, which runs in the LDS unit, so their arithmetic co-issues.
+// receives lane l-1 of its L-lane group and lane 0 the group's last lane;
+// DIR = -1: lane l receives lane l+1 and the last lane lane 0.  (DPP
+// row_ror / wave_ror would move the same values, but on gfx950 two waves of a
+// SIMD co-issue VGPR-only VALU instructions and not DPP-modified ones, so the
+// hot loops rotate in the LDS unit and keep their arithmetic co-issuing.)
 // Every lane of the wave must be active.
 template <int L, int DIR>
 struct LaneXfer {
@@ -46,9 +31,17 @@ struct LaneXfer {
     __device__ __forceinline__ float operator()(float x) const {
         return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
     }
+    // FP64 arithmetic does not co-issue, so the 64-lane FP64 kernels keep the
+    // cheaper DPP wave_ror:1 / wave_rol:1 (0x13C / 0x134) for each half
     __device__ __forceinline__ double operator()(double x) const {
-        return __hiloint2double(__builtin_amdgcn_ds_bpermute(addr, __double2hiint(x)),
-                                __builtin_amdgcn_ds_bpermute(addr, __double2loint(x)));
+        if constexpr (L == 64) {
+            constexpr int ctl = DIR > 0 ? 0x13C : 0x134;
+            return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), ctl, 0xf, 0xf, true),
+                                    __builtin_amdgcn_mov_dpp(__double2loint(x), ctl, 0xf, 0xf, true));
+        } else {
+            return __hiloint2double(__builtin_amdgcn_ds_bpermute(addr, __double2hiint(x)),
+                                    __builtin_amdgcn_ds_bpermute(addr, __double2loint(x)));
+        }
     }
 };
 template <int L>
@@ -85,21 +78,6 @@ template <> struct Tab<double> {
     static __device__ __forceinline__ double initd(const FwdArgs &a, uint32_t h) { return a.initd_d[h]; }
 };
 
-template <int L>
-__device__ __forceinline__ int rol1_i(int x) {
-    // row_rol:1 (0x101 + 15 = row_ror:15) / wave_rol:1 (0x134): lane l
-    // receives lane l+1, the last lane receives lane 0
-    return __builtin_amdgcn_mov_dpp(x, L == 16 ? 0x12F : 0x134, 0xf, 0xf, true);
-}
-template <int L>
-__device__ __forceinline__ float rol1(float x) {
-    return __int_as_float(rol1_i<L>(__float_as_int(x)));
-}
-template <int L>
-__device__ __forceinline__ double rol1(double x) {
-    return __hiloint2double(rol1_i<L>(__double2hiint(x)), rol1_i<L>(__double2loint(x)));
-}
-
 // Per-lane register state: MR rows at the lane's last column, plus the last
 // row one column earlier (read by the next lane as its diagonal).
 template <typename T, int MR>
@@ -115,18 +93,11 @@ struct Priors {
     uint32_t rpos[MR];
 };
 
-// The row above the lane's first row through DPP (ror1<L>).
-template <int L>
-struct DppRot {
-    template <typename T>
-    __device__ __forceinline__ T operator()(T x) const { return ror1<L>(x); }
-};
-
 // One anti-diagonal step of one lane: all MR rows at the next column.  `rot`
-// brings the previous lane's values (DppRot or LaneRot).
-template <typename T, int L, int MR, bool INTER, typename Rot = DppRot<L>>
+// (LaneRot<L>) brings the previous lane's values.
+template <typename T, int L, int MR, bool INTER>
 __device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, uint32_t hc,
-                                     const Rot &rot = Rot()) {
+                                     const LaneRot<L> &rot) {
     // row above the lane's first row: previous lane's last row, at this column
     // (up) and the column before (diagonal)
     T dgM = rot(S.Mp), dgI = rot(S.Ip), dgD = rot(S.Dp);

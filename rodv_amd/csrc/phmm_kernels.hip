@@ -12,21 +12,24 @@
 // order when asked for, the same tables and the same left-to-right sums.
 // Compiled with -ffp-contract=off so only the explicit fmas fuse.
 //
-// Mapping (MI355X, wave64).  A pair owns a group of L lanes (L = 16: one DPP
-// row, four pairs per wave; L = 64: one pair per wave).  Lane l holds MR
-// consecutive read rows in registers and sweeps the haplotype one column per
-// step, lagging lane l-1 by one column, so a step is an anti-diagonal of
-// lanes.  Rows inside a lane pass values in registers; the row above a lane's
-// first row is the previous lane's last row, read with one DPP row_ror:1
-// (wave_ror:1 for L = 64).  The rotation makes lane 0 read lane L-1's last
-// slot, which is never a read row: it is an identity row that reproduces the
-// row-0 boundary (M = I = 0, D = K/H) for free, or -- for reads longer than
-// one strip of L*MR-1 rows -- the feeder that injects the previous strip's
-// last row from HBM.  Read rows are shifted down by `off` identity rows so
-// that row R always sits in the last register slot of its lane.
-// Haplotype codes are staged in LDS (one-hot, one byte per column); each
-// step reads one byte per lane and selects the emission prior with
-// v_bfe_i32 + v_bfi_b32.  No MFMA: the recurrence is not a contraction.
+// Mapping (MI355X, wave64).  A pair owns a group of L lanes (L = 16: four
+// pairs per wave; 32: two; 64: one).  Lane l holds MR consecutive read rows
+// in registers and sweeps the haplotype one column per step, lagging lane
+// l-1 by one column, so a step is an anti-diagonal of lanes.  Rows inside a
+// lane pass values in registers; the row above a lane's first row is the
+// previous lane's last row, brought over with ds_bpermute (LaneRot; DPP would
+// stop two waves from co-issuing).  The rotation makes lane 0 read lane L-1's
+// last slot, which is never a read row: it is an identity row that
+// reproduces the row-0 boundary (M = I = 0, D = K/H) for free, or -- for
+// reads longer than one strip of (L-1)*MR rows -- the feeder that injects the
+// previous strip's last row from HBM.  Read rows are shifted down by `off`
+// identity rows so that row R always sits in the last register slot of its
+// lane.
+// Two FP32 variants: fwd_tab_kernel reads each row's emission prior from a
+// per-pair LDS table indexed by the haplotype base (N-free haplotypes, one
+// strip); fwd_kernel stages one-hot haplotype codes and selects the prior
+// per cell with v_bfe_i32 + a bit-select (haplotypes with N, multi-strip
+// reads, and the FP64 rescue).  No MFMA: the recurrence is not a contraction.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -224,6 +227,7 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
         };
 
         int d = 1;
+        const LaneRot<L> rot;
         // main loop: every lane's sum slot is at a column <= H
         for (; d + kU - 1 <= Dmain; d += kU) {
             uint32_t hc[kU];
@@ -231,7 +235,7 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
             for (int u = 0; u < kU; u++) hc[u] = cp[d + u];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                step<T, L, MR, INTER>(S, P, hc[u]);
+                step<T, L, MR, INTER>(S, P, hc[u], rot);
                 sm += S.M[MR - 1];
                 si += S.I[MR - 1];
                 after_step(d + u);
@@ -239,7 +243,7 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
         }
         // remaining steps: sums only while the lane's column is <= H
         for (; d <= Dend; d++) {
-            step<T, L, MR, INTER>(S, P, cp[d]);
+            step<T, L, MR, INTER>(S, P, cp[d], rot);
             if (last && d - l <= H) {
                 sm += S.M[MR - 1];
                 si += S.I[MR - 1];
