@@ -9,10 +9,11 @@
 //  1. forward over all anti-diagonal steps, saving the whole lane state every
 //     `ckpt` steps to HBM (level-1 checkpoints);
 //  2. for each checkpoint segment, last to first: recompute the segment from
-//     its checkpoint, saving the state every 4 steps in LDS (level 2);
-//  3. for each 4-step window, last to first: recompute the 4 states into
-//     registers and run the adjoint of those steps in reverse, lanes reading
-//     the row below their last row from lane l+1.
+//     its checkpoint, saving the state every 8 steps in LDS (level 2);
+//  3. for each 8-step window, last to first, and inside it for each 4-step
+//     half, last to first: recompute the 4 states into registers (the second
+//     half from a copy advanced 4 steps) and run the adjoint of those steps in
+//     reverse, lanes reading the row below their last row from lane l+1.
 // Values cross lanes through ds_bpermute (LaneXfer), not DPP, so that two
 // waves of a SIMD can co-issue the arithmetic.
 // Adjoint seeds are sigma = 2^-ilogb(S) so that adjoint x forward products
@@ -171,38 +172,53 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
 #pragma unroll
     for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gg[s] = ge[s] = T(0);
     T Tb0p = T(0);
+    // the 4 steps after state W0 (state after step dw): recompute them into
+    // registers, then their adjoints last to first
+    auto window4 = [&](const State<T, MR> &W0, int dw) {
+        State<T, MR> W1, W2, W3, W4;
+        W1 = W0;
+        step<T, kL, MR, false>(W1, P, cp[dw + 1], up);
+        W2 = W1;
+        step<T, kL, MR, false>(W2, P, cp[dw + 2], up);
+        W3 = W2;
+        step<T, kL, MR, false>(W3, P, cp[dw + 3], up);
+        W4 = W3;
+        step<T, kL, MR, false>(W4, P, cp[dw + 4], up);
+        auto seed_at = [&](int d) {
+            const int j = d - l;
+            return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
+        };
+        adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+        adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+        adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+        adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+    };
     for (int k = nseg - 1; k >= 0; k--) {
         const int d0 = k * K1;                       // state after step d0
-        const int nsteps = min(K1, D4 - d0);
-        const int nwin = nsteps / kW;
+        const int nsteps = min(K1, D4 - d0);         // a multiple of 4
+        const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
+        // level 2: the state at the start of every 8 steps, in LDS
         get_state(ck, NV, k, S);
         __syncthreads();
         for (int w = 0; w < nwin; w++) {
             put_state(l2, NV, w, S);
-#pragma unroll
-            for (int u = 0; u < kW; u++) step<T, kL, MR, false>(S, P, cp[d0 + w * kW + u + 1], up);
+            const int n = min(2 * kW, nsteps - w * 2 * kW);
+            for (int u = 0; u < n; u++) step<T, kL, MR, false>(S, P, cp[d0 + w * 2 * kW + u + 1], up);
         }
         __syncthreads();
         for (int w = nwin - 1; w >= 0; w--) {
-            const int dw = d0 + w * kW;              // window covers steps dw+1 .. dw+4
-            State<T, MR> W0, W1, W2, W3, W4;
+            const int dw = d0 + w * 2 * kW;          // window covers steps dw+1 .. dw+n
+            const int n = min(2 * kW, nsteps - w * 2 * kW);
+            State<T, MR> W0;
             get_state(l2, NV, w, W0);
-            W1 = W0;
-            step<T, kL, MR, false>(W1, P, cp[dw + 1], up);
-            W2 = W1;
-            step<T, kL, MR, false>(W2, P, cp[dw + 2], up);
-            W3 = W2;
-            step<T, kL, MR, false>(W3, P, cp[dw + 3], up);
-            W4 = W3;
-            step<T, kL, MR, false>(W4, P, cp[dw + 4], up);
-            auto seed_at = [&](int d) {
-                const int j = d - l;
-                return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
-            };
-            adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
-            adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
-            adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
-            adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+            if (n == 2 * kW) {
+                // second half first: advance a copy 4 steps, then its window
+                State<T, MR> X = W0;
+#pragma unroll
+                for (int u = 1; u <= kW; u++) step<T, kL, MR, false>(X, P, cp[dw + u], up);
+                window4(X, dw + kW);
+            }
+            window4(W0, dw);
         }
     }
 

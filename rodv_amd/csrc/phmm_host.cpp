@@ -1374,7 +1374,9 @@ int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         all_items.insert(all_items.end(), bucket[mr].begin(), bucket[mr].end());
     }
     item_begin[maxmr + 1] = all_items.size();
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t)ctx->num_cus * 4));
+    // two waves per SIMD: the adjoint kernels use up to 256 VGPRs, so more
+    // would not be resident at once
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t)ctx->num_cus * 8));
     const uint32_t stride = lds_stride_for(max_h, 64);
     const uint64_t d4max = ((uint64_t)max_h + 63 + 3) / 4 * 4;
     const uint64_t nseg_max = d4max / 4 + 2;  // enough for any stride >= 4
@@ -1445,12 +1447,13 @@ int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
             const size_t esz = pass == 0 ? sizeof(float) : sizeof(double);
             // level-2 windows live in LDS: shrink the checkpoint stride if needed
             int K1 = ckpt_stride;
-            while (K1 > 4 && stride + (size_t)(K1 / 4) * nv * 64 * esz > (size_t)maxlds) K1 /= 2;
+            // (one state per 8 steps of a segment)
+            while (K1 > 4 && stride + (size_t)((K1 + 7) / 8) * nv * 64 * esz > (size_t)maxlds) K1 /= 2;
             K1 = K1 / 4 * 4;
             a.ckpt = K1;
             a.counter = d_cnt + 3 * mr + 1 + pass;
             a.n_items_dev = pass == 0 ? nullptr : a.rescue_count;
-            const size_t lds = stride + (size_t)(K1 / 4) * nv * 64 * esz;
+            const size_t lds = stride + (size_t)((K1 + 7) / 8) * nv * 64 * esz;
             if (launch_adj(adj_kernel_ptr(mr, pass == 1), grid, lds, st, a) != hipSuccess)
                 rc = RODV_PHMM_ERR_DEVICE;
         }
