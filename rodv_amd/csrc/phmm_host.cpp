@@ -871,8 +871,9 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     }
     p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size();
     if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
-    // FP64 rescue: one pair per wave; boundary rows only if some read > 252
-    const int wpc64 = 4;
+    // FP64 rescue: one pair per wave, two waves per SIMD (its ~180 VGPRs
+    // allow no more); boundary rows only if some read > 252
+    const int wpc64 = 8;
     p->rescue_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n_pairs, (uint64_t)ctx->num_cus * wpc64));
     uint32_t max_h_all = 0;
     for (uint32_t h = 0; h < M; h++) max_h_all = std::max(max_h_all, b->hap_lengths[h]);
