@@ -99,7 +99,8 @@ typedef struct rodv_phmm_config {
                                      at most 64 */
     int32_t use_graph;            /* capture the batched launch in a hipGraph   */
     int32_t waves_per_cu;         /* persistent grid size; 0 = engine default   */
-    int32_t short_lanes;          /* lanes per pair for reads <= 240: 16 or 64  */
+    int32_t short_lanes;          /* lanes per pair for reads <= 255 (N-free
+                                     haplotypes; <= 240 with N): 16 or 64      */
     int32_t extended_range;       /* write finite log10 where FP64 underflows   */
     int32_t prior_table;          /* 1: per-pair LDS table of emission priors for
                                      N-free haplotypes (default); 0: select per cell */
@@ -169,7 +170,7 @@ int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n);
 /*
  * Autotuner (no counterpart in the reference: its SIMD widths are fixed by
  * the host CPU).  Reads fall in RODV_PHMM_TUNE_CLASSES length classes
- * (1..15, 16..30, ..., 226..240 rows, then 241..310, then longer).  For each
+ * (1..15, 16..31, ..., 240..255 rows, then 256..319, then longer).  For each
  * class present in `sample`, rodv_phmm_autotune times the FP32 forward of
  * (up to 131,072 of) the sample's pairs of that class under every candidate
  * shape -- pairs per wavefront 4, 2 or 1 (16, 32 or 64 lanes per pair, rows

@@ -174,10 +174,11 @@ constexpr int kUnroll = 8;  // must match kU in phmm_kernels.hip
 constexpr int kDefaultWavesPerCU = 12;
 
 // Read-length classes of the autotuner: class k (0-based) holds reads of
-// 15k+1 .. 15(k+1) rows for k < 16 (the 16-lane shapes' rows-per-lane
-// steps), then 241..310 and longer.
+// 16k .. 16k+15 rows for k < 16 (the 16-lane table shapes' rows-per-lane
+// steps: 16 lanes of MR rows hold up to 16*MR - 1), then 256..319 (the
+// longest 32-lane table shape) and longer.
 constexpr uint32_t kClassMax[RODV_PHMM_TUNE_CLASSES] = {
-    15, 30, 45, 60, 75, 90, 105, 120, 135, 150, 165, 180, 195, 210, 225, 240, 310, 0xffffffffu};
+    15, 31, 47, 63, 79, 95, 111, 127, 143, 159, 175, 191, 207, 223, 239, 255, 319, 0xffffffffu};
 
 int tune_class(uint32_t R) {
     int c = 0;
@@ -309,8 +310,9 @@ void free_plan(rodv_phmm_plan *p) {
     delete p;
 }
 
-// (L, MR) of the FP32 kernel for a read of R rows.  A single strip holds
-// (L-1)*MR read rows; longer reads run strip by strip in the L = 64 kernels.
+// (L, MR) of the select-prior FP32 kernel for a read of R rows.  A strip
+// holds (L-1)*MR read rows (lane L-1 is the identity/feeder lane); longer
+// reads run strip by strip in the L = 64 kernels.
 void choose_shape(uint32_t R, int short_lanes, int *L, int *MR) {
     if (short_lanes != 64 && R <= 15u * 16u) {
         *L = 16;
@@ -323,31 +325,70 @@ void choose_shape(uint32_t R, int short_lanes, int *L, int *MR) {
     *MR = std::min(16, std::max(4, mr));
 }
 
-// Shape of one pair: the class's tuned lanes per pair when the context has
-// one (rows per lane then as few as the read needs), else choose_shape.
-// 32-lane shapes exist only as table kernels (prior_tab), so pairs the table
-// kernel cannot take fall back to the default shape.
-void shape_for(uint32_t R, const rodv_phmm_config &cfg, const TuneEntry *te, bool tab_ok,
-               int *L, int *MR) {
-    if (te && te->L == 16 && R <= 240u) {
+// Rows per lane of the table-prior kernel with L lanes for a read of R rows,
+// or 0 when no instantiated shape (16 lanes: 1..16 rows per lane, 32: 2..10,
+// 64: 4, 8, 12, 16 up to `cap`) holds it in one strip.  Every lane carries
+// read rows and the row-0 boundary needs one identity slot, so a strip of
+// L*MR slots holds R <= L*MR - 1 rows.
+int tab_rows(uint32_t R, int L, int cap) {
+    int mr = (int)((R + (uint32_t)L) / (uint32_t)L);
+    if (L == 16) return mr <= 16 ? mr : 0;
+    if (L == 32) {
+        mr = std::max(2, mr);
+        return mr <= 10 ? mr : 0;
+    }
+    mr = std::max(4, (mr + 3) / 4 * 4);
+    return mr <= (cap > 0 ? cap : 16) ? mr : 0;
+}
+
+// Lanes per pair of the built-in table shape: 16 while 16 lanes hold the
+// read in one strip, else 64.
+int default_tab_lanes(uint32_t R, int short_lanes) {
+    return short_lanes != 64 && tab_rows(R, 16, 0) ? 16 : 64;
+}
+
+// The groups of one wave stage their haplotype codes (and prior tables) in
+// LDS; a very long haplotype does not fit four or two of them.
+bool codes_fit(uint32_t H, int L, int MR);
+
+// Shape and kernel of one pair: the class's tuned lanes per pair when the
+// context has one (rows per lane then as few as the read needs), else the
+// built-in shape.  The table kernel takes the pair when `tab_ok` (N-free
+// haplotype, prior tables on) and the read fits one table strip; otherwise
+// the select-prior kernel, whose 32-lane shapes do not exist, takes it.
+void shape_for(uint32_t R, uint32_t H, const rodv_phmm_config &cfg, const TuneEntry *te, bool tab_ok,
+               int *L, int *MR, bool *tab) {
+    const int want = te ? te->L : 0;
+    const int cap = want == 64 ? te->mr_cap : 0;
+    if (tab_ok) {
+        int l = want ? want : default_tab_lanes(R, cfg.short_lanes);
+        int mr = tab_rows(R, l, cap);
+        if (mr == 0 && l != 64) mr = tab_rows(R, l = 64, 0);
+        if (mr && !codes_fit(H, l, mr)) mr = tab_rows(R, l = 64, cap);
+        if (mr) {
+            *L = l;
+            *MR = mr;
+            *tab = true;
+            return;
+        }
+    }
+    *tab = false;
+    if (want == 16 && R <= 240u) {
         *L = 16;
         *MR = std::max(1, (int)((R + 14) / 15));
-        return;
-    }
-    if (te && te->L == 32 && R <= 310u && tab_ok) {
-        *L = 32;
-        *MR = std::max(2, (int)((R + 30) / 31));
-        return;
-    }
-    if (te && te->L == 64) {
-        const int cap = te->mr_cap > 0 ? te->mr_cap : 16;
+    } else if (want == 64) {
         int mr = (int)((R + 62) / 63);
         mr = ((mr + 3) / 4) * 4;
         *L = 64;
-        *MR = std::min(cap, std::max(4, mr));
-        return;
+        *MR = std::min(cap > 0 ? cap : 16, std::max(4, mr));
+    } else {
+        choose_shape(R, cfg.short_lanes, L, MR);
     }
-    choose_shape(R, cfg.short_lanes, L, MR);
+    if (!codes_fit(H, *L, *MR)) {
+        *L = 64;
+        *MR = 4;
+        while (*MR < 16 && 63 * *MR < (int)R) *MR += 4;
+    }
 }
 
 uint32_t lds_stride_for(uint32_t max_h, int L) {
@@ -355,7 +396,14 @@ uint32_t lds_stride_for(uint32_t max_h, int L) {
     return (s + 15u) & ~15u;
 }
 
-// Strips of (L-1)*MR read rows, after `off` < MR leading identity rows.
+bool codes_fit(uint32_t H, int L, int MR) {
+    if (L == 64) return true;
+    const uint64_t per = 2ull * lds_stride_for(H, L) + fwd_tab_bytes(L, MR);
+    return (64ull / (uint64_t)L) * per <= 60u * 1024u;
+}
+
+// Strips of (L-1)*MR read rows (select-prior kernel), after `off` < MR
+// leading identity rows.
 int strips_for(uint32_t R, int L, int MR) {
     const int C = (L - 1) * MR;
     const int off = (MR - (int)R % MR) % MR;
@@ -704,15 +752,8 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             te = &ctx->tune[cls];
         const bool tab_ok = cfg.prior_table && !hap_has_n[ph[k]];
         int L, MR;
-        shape_for(R, cfg, te, tab_ok, &L, &MR);
-        // a very long haplotype does not fit the groups' codes in LDS
-        if (L < 64 && (64ull / L) * (2ull * lds_stride_for(H, L) + fwd_tab_bytes(L, MR)) > 60 * 1024) {
-            L = 64;
-            MR = 4;
-            while (MR < 16 && 63 * MR < (int)R) MR += 4;
-        }
-        const bool tab = tab_ok && strips_for(R, L, MR) == 1;
-        if (!tab && L == 32) choose_shape(R, cfg.short_lanes, &L, &MR);
+        bool tab;
+        shape_for(R, H, cfg, te, tab_ok, &L, &MR, &tab);
         const bool inter = !inter32.empty() && inter32[k];
         const int lidx = L == 16 ? 0 : (L == 32 ? 1 : 2);
         const int key = (((inter ? 1 : 0) * 2 + (tab ? 1 : 0)) * 3 + lidx) * 17 + MR;
@@ -752,7 +793,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             bk.items[k] = (int32_t)mem[k];
             const uint32_t R = b->read_lengths[pr[mem[k]]], H = b->hap_lengths[ph[mem[k]]];
             bk.max_h = std::max(bk.max_h, H);
-            bk.max_strips = std::max(bk.max_strips, (uint32_t)strips_for(R, bk.L, bk.MR));
+            if (!bk.tab) bk.max_strips = std::max(bk.max_strips, (uint32_t)strips_for(R, bk.L, bk.MR));
             bk.cells += (uint64_t)R * H;
         }
         if (bk.tab) {
@@ -1169,9 +1210,9 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
         TuneEntry best{};
         double best_g = e.default_gcups;
         std::vector<TuneEntry> cands;
-        if (kClassMax[k] <= 240) cands.push_back(TuneEntry{16, 0, 0});
-        if (kClassMax[k] <= 310) cands.push_back(TuneEntry{32, 0, 0});
-        if (kClassMax[k] <= 310) {
+        if (kClassMax[k] <= 255) cands.push_back(TuneEntry{16, 0, 0});
+        if (kClassMax[k] <= 319) cands.push_back(TuneEntry{32, 0, 0});
+        if (kClassMax[k] <= 319) {
             cands.push_back(TuneEntry{64, 0, 0});
         } else {
             for (int cap : {4, 8, 12, 16}) cands.push_back(TuneEntry{64, cap, 0});
@@ -1187,11 +1228,8 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
             }
         }
         if (rc) break;
-        if (best.L == 0) {  // the built-in shape won: name it
-            int L = 16, MR = 1;
-            choose_shape(kClassMax[k] == 0xffffffffu ? 1000u : kClassMax[k], cfg.short_lanes, &L, &MR);
-            best.L = L;
-        }
+        if (best.L == 0)  // the built-in shape won: name it
+            best.L = default_tab_lanes(kClassMax[k], cfg.short_lanes);
         for (int w : {4, 8, 16}) {
             TuneEntry c = best;
             c.wpc = w;

@@ -30,6 +30,11 @@
 // strip); fwd_kernel stages one-hot haplotype codes and selects the prior
 // per cell with v_bfe_i32 + a bit-select (haplotypes with N, multi-strip
 // reads, and the FP64 rescue).  No MFMA: the recurrence is not a contraction.
+// fwd_tab_kernel needs no identity lane: all L lanes carry read rows, and
+// the row-0 boundary is an identity slot at the top of lane 0 (off >= 1
+// when R % MR != 0) or lane L-1 left without read rows (R % MR == 0), so a
+// strip holds any R <= L*MR - 1.  An identity slot maps every finite input
+// to (0, 0, D unchanged), so what lane L-1 rotates into lane 0 is harmless.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -431,7 +436,7 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
     for (int s = 0; s < MR; s++) {
         const int pos = l * MR + s;
         const int i = pos - off + 1;
-        const bool real = valid && l < L - 1 && i >= 1 && i <= R;
+        const bool real = valid && i >= 1 && i <= R;  // lane L-1 included
         load_row<float, MR, INTER>(a, P, s, real, ro + (uint64_t)(real ? i - 1 : 0));
         X[s] = P.X[s];
         Y[s] = P.Y[s];
