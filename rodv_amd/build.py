@@ -20,10 +20,13 @@ CEILING_SOURCES = [os.path.join(HERE, "csrc", "valu_ceiling.hip")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: only the explicit fmas of the kernels fuse (bit-exact
 # FP32 parity).  Denormals stay IEEE like the reference library.  Packed FP32
-# is disabled for all GPU code.
+# is disabled for all GPU code.  The atomic optimizer is off: it turns a
+# one-lane atomicAdd into a form that waits for the result right away, where
+# the table kernel fetches its next work item early and reads it late.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero", "-fno-fast-math",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
          "-Wall", "-Wno-unused-function", f"-I{os.path.join(ROOT, 'include')}"]
 
 

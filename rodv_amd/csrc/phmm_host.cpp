@@ -27,6 +27,8 @@ int fwd_f64_rescue_mr();
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
+hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, uint64_t n,
+                       hipStream_t st);
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st);
 int adj_max_mr();
 void *adj_kernel_ptr(int MR, bool rescue);
@@ -206,6 +208,7 @@ struct Bucket {
     uint32_t max_strips = 1;
     uint32_t lds_stride = 0;
     int32_t *d_items = nullptr;
+    ItemDesc *d_desc = nullptr;  // table kernel: one descriptor per item slot
     float *d_bnd = nullptr;
     uint64_t bnd_elems = 0;
     int grid = 0;
@@ -272,6 +275,8 @@ struct rodv_phmm_plan {
     std::vector<PlainLaunch> plain;
     std::vector<int32_t> plain_items_all;  // per bucket, region-ordered items
     std::vector<int32_t *> d_plain_items;
+    std::vector<size_t> plain_items_n;
+    std::vector<ItemDesc *> d_plain_desc;
     std::vector<uint32_t> pair_R, pair_H;
     bool any_inter = false;
     hipGraph_t graph = nullptr;
@@ -303,9 +308,12 @@ void free_plan(rodv_phmm_plan *p) {
         if (b) (void)hipFree(b);
     for (auto &b : p->buckets) {
         if (b.d_items) (void)hipFree(b.d_items);
+        if (b.d_desc) (void)hipFree(b.d_desc);
         if (b.d_bnd) (void)hipFree(b.d_bnd);
     }
     for (auto *d : p->d_plain_items)
+        if (d) (void)hipFree(d);
+    for (auto *d : p->d_plain_desc)
         if (d) (void)hipFree(d);
     delete p;
 }
@@ -432,6 +440,8 @@ void host_simd_widths(uint32_t *wf, uint32_t *wd) {
         *wd = 4;
     }
 }
+
+FwdArgs base_args(rodv_phmm_plan *p);
 
 // ---------------------------------------------------------------- routing
 // schedule_pairhmm's FP32 stage (pairhmm_schedule.cpp:141-196): the region's
@@ -868,6 +878,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             int32_t *d = nullptr;
             rc = upload(&d, items.data(), items.size());
             p->d_plain_items.push_back(d);
+            p->plain_items_n.push_back(items.size());
             if (rc) {
                 free_plan(p);
                 return rc;
@@ -924,6 +935,25 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         if (!rc) rc = dev_alloc(&p->d_bnd64, (size_t)p->rescue_grid * 2 * p->bnd64_elems);
         if (!rc && hipMemset(p->d_bnd64, 0, (size_t)p->rescue_grid * 2 * p->bnd64_elems * sizeof(double)) != hipSuccess)
             rc = RODV_PHMM_ERR_DEVICE;
+    }
+    // descriptors of the table kernels' item slots (batched and plain order)
+    if (!rc) {
+        const FwdArgs a = base_args(p);
+        p->d_plain_desc.assign(p->d_plain_items.size(), nullptr);
+        for (size_t bi = 0; bi < p->buckets.size() && !rc; bi++) {
+            Bucket &bk = p->buckets[bi];
+            if (!bk.tab) continue;
+            rc = dev_alloc(&bk.d_desc, bk.items.size());
+            if (!rc && launch_desc(a, bk.d_items, bk.d_desc, bk.items.size(), nullptr) != hipSuccess)
+                rc = RODV_PHMM_ERR_DEVICE;
+            if (!rc && bi < p->d_plain_items.size()) {
+                rc = dev_alloc(&p->d_plain_desc[bi], p->plain_items_n[bi]);
+                if (!rc && launch_desc(a, p->d_plain_items[bi], p->d_plain_desc[bi], p->plain_items_n[bi],
+                                       nullptr) != hipSuccess)
+                    rc = RODV_PHMM_ERR_DEVICE;
+            }
+        }
+        if (!rc && hipDeviceSynchronize() != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     }
     if (rc) {
         free_plan(p);
@@ -984,6 +1014,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 Bucket &bk = p->buckets[bi];
                 FwdArgs a = base_args(p);
                 a.items = bk.d_items;
+                a.desc = bk.d_desc;
                 a.n_items = bk.n_items;
                 a.counter = p->d_counters + 2 + bi;
                 a.lds_stride = bk.lds_stride;
@@ -1001,6 +1032,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 const int G = 64 / bk.L;
                 FwdArgs a = base_args(p);
                 a.items = p->d_plain_items[pl.bucket] + (size_t)pl.item_begin * G;
+                if (bk.tab) a.desc = p->d_plain_desc[pl.bucket] + (size_t)pl.item_begin * G;
                 a.n_items = pl.n_items;
                 a.counter = p->d_counters + 2 + p->buckets.size() + k;
                 a.lds_stride = bk.lds_stride;

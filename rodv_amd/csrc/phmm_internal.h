@@ -29,6 +29,16 @@ constexpr uint8_t kHapN = 0x0f;
 // Read codes: bit position of the base in the hap code (A0 C1 G2 T3), 4 = N.
 constexpr uint8_t kReadN = 4;
 
+// One item slot of the table kernel: the pair and what its set-up needs,
+// gathered once per plan so that a wave starts an item with one load.
+struct alignas(16) ItemDesc {
+    int32_t p;      // pair index, -1 = empty slot (then all fields are 0)
+    uint32_t R, H;  // read and haplotype length
+    float initd;    // 2^120 / (float)H
+    uint64_t ro;    // offset of the read's first base
+    uint64_t hoff;  // offset of the haplotype's codes
+};
+
 // Arguments of every forward launch (passed by value).
 struct FwdArgs {
     const DevTables *tab;
@@ -51,6 +61,7 @@ struct FwdArgs {
     // work: items of G pair indices (-1 = empty); or, for the rescue pass,
     // a list of pair indices whose length is *n_items_dev
     const int32_t *items;
+    const ItemDesc *desc;  // table kernel: descriptors of the item slots
     uint32_t n_items;
     const uint32_t *n_items_dev;
     uint32_t *counter;  // persistent work counter (zeroed by the reset kernel)

@@ -48,15 +48,30 @@ using namespace dev;
 constexpr int kU = 8;  // steps per unrolled block (even: the register rotation)
 
 
+// The raw bytes of one read row: qualities (masked as pairhmm_impl.cpp:
+// 89-92 masks them) and the base code.
+struct RowRaw {
+    uint32_t qi, qd, qc, qq, code;
+};
+
+__device__ __forceinline__ RowRaw load_raw(const FwdArgs &a, uint64_t idx) {
+    RowRaw w;
+    w.qi = a.ins[idx] & 127u;
+    w.qd = a.del[idx] & 127u;
+    w.qc = a.gcp[idx] & 127u;
+    w.qq = a.q[idx] & 127u;
+    w.code = a.rcode[idx];
+    return w;
+}
+
+// The row's transition and emission priors from its raw bytes; identity
+// rows (real == false) select constants.
 template <typename T, int MR, bool INTER>
-__device__ __forceinline__ void load_row(const FwdArgs &a, Priors<T, MR> &P, int s, bool real,
-                                         uint64_t idx) {
-    // branch-free: the address is always valid (idx points at the read's
-    // first base for identity rows) and identity rows select constants
+__device__ __forceinline__ void row_priors(const FwdArgs &a, Priors<T, MR> &P, int s, bool real,
+                                           const RowRaw &w) {
     const DevTables *t = a.tab;
-    uint32_t qi = a.ins[idx] & 127u, qd = a.del[idx] & 127u, qc = a.gcp[idx] & 127u;
-    uint32_t qq = a.q[idx] & 127u;
-    uint32_t code = a.rcode[idx];
+    const uint32_t qi = w.qi, qd = w.qd, qc = w.qc, qq = w.qq;
+    uint32_t code = w.code;
     uint32_t lo = qi <= qd ? qi : qd, hi = qi <= qd ? qd : qi;
     T mm = Tab<T>::mm(t)[((hi * (hi + 1)) >> 1) + lo];
     T g = Tab<T>::omp(t)[qc];
@@ -79,6 +94,22 @@ __device__ __forceinline__ void load_row(const FwdArgs &a, Priors<T, MR> &P, int
     P.Y[s] = real ? Y : T(0);
     P.rpos[s] = real ? code : 0u;
 }
+
+// branch-free: the address is always valid (idx points at the read's first
+// base for identity rows) and identity rows select constants
+template <typename T, int MR, bool INTER>
+__device__ __forceinline__ void load_row(const FwdArgs &a, Priors<T, MR> &P, int s, bool real,
+                                         uint64_t idx) {
+    row_priors<T, MR, INTER>(a, P, s, real, load_raw(a, idx));
+}
+
+// Orders one wave's LDS accesses across its lanes without a workgroup
+// barrier: the compiler moves no memory access across it, and a wave's LDS
+// operations execute in issue order, so waiting for its own ones suffices.
+// A __syncthreads() would also wait for every global load and store in
+// flight, which the table kernel keeps in flight across items on purpose.
+// Valid only for one-wave workgroups.
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
 template <typename T>
 __device__ __forceinline__ T ld_agent(const T *p) {
@@ -328,6 +359,9 @@ template <int L, int MR, bool INTER>
 __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Priors<float, MR> &P,
                                          const float (&pr)[MR], const LaneRot<L> &rot,
                                          float (&pb)[MR], const uint8_t *pnext) {
+    // keep the previous step's M and I rotations ahead of this step's LDS
+    // reads: the I chain below then waits for them alone
+    __builtin_amdgcn_sched_barrier(0);
     const float nextD = rot(S.D[MR - 1]);
     load_priors<MR>(pb, pnext);
     __builtin_amdgcn_sched_barrier(0);
@@ -371,29 +405,25 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
     C.uI = rot(S.I[MR - 1]);
 }
 
+// One item of the table kernel: G pairs, one per group of L lanes.  `d` is
+// the group's descriptor; the return value is the index of the wave's next
+// item, fetched from the work counter once the set-up is done so that the
+// atomic's round trip hides behind the sweep.
 template <int L, int MR, bool INTER>
-__device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
+__device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemDesc &d) {
     constexpr int G = 64 / L;
     constexpr int MRT = TabLayout<MR>::MRT;
     constexpr uint32_t kBlk = (uint32_t)L * MRT * 4;  // bytes per hap base
     const int lane = threadIdx.x;
     const int g = lane / L;
     const int l = lane % L;
+    const int p = d.p;
     const bool valid = p >= 0;
-
-    uint32_t h = 0, r = 0;
-    int R = 0, H = 0;
-    uint64_t ro = 0;
-    if (valid) {
-        h = a.pair_hap[p];
-        r = a.pair_read[p];
-        R = (int)a.read_len[r];
-        H = (int)a.hap_len[h];
-        ro = a.read_off[r];
-    }
+    const int R = (int)d.R, H = (int)d.H;  // 0 for an empty slot
+    const uint64_t ro = d.ro;
+    const float initd = d.initd;
     const int off = valid ? (MR - R % MR) % MR : 0;
     const int lR = valid ? (R - 1 + off) / MR : 0;
-    const float initd = valid ? a.initd_f[h] : 0.0f;
 
     // LDS per group: codes (u16, lds_stride bytes) then the prior table
     const uint32_t gstride = a.lds_stride + 4 * kBlk;
@@ -401,47 +431,63 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
     uint16_t *codes = reinterpret_cast<uint16_t *>(gb);
     uint8_t *tab = gb + a.lds_stride;
 
-    __syncthreads();
-    for (uint32_t k = lane * 16; k < (uint32_t)G * gstride; k += 64 * 16)
-        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
-    __syncthreads();
+    // codes outside the haplotype are zero: each group clears its code area
+    // (the previous item's LDS reads have all returned: their values were used)
+    for (uint32_t k = (uint32_t)l * 16; k < a.lds_stride; k += L * 16)
+        *reinterpret_cast<uint4 *>(gb + k) = make_uint4(0, 0, 0, 0);
+    wave_lds_sync();
+
+    // the rows' raw bytes first, so that they and the haplotype codes are
+    // in flight together
+    RowRaw raw[MR];
+    bool realv[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int i = l * MR + s - off + 1;
+        realv[s] = valid && i >= 1 && i <= R;  // lane L-1 included
+        raw[s] = load_raw(a, ro + (uint64_t)(realv[s] ? i - 1 : 0));
+    }
     if (valid) {
         // column j (1-based) at codes[L-1+j]: one-hot 1,2,4,8 -> block offset;
-        // 16 columns per lane and iteration from the 16-byte aligned codes
-        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
+        // 16 columns per chunk from the 16-byte aligned codes, two chunks per
+        // lane and pass, both loads issued before either is used (without a
+        // second chunk the first is converted and written twice)
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + d.hoff);
         const int nch = (H + 15) >> 4;
-        for (int c = l; c < nch; c += L) {
-            const uint4 v = src[c];
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-            uint32_t o[8];
+        for (int c = l; c < nch; c += 2 * L) {
+            const int c2 = c + L < nch ? c + L : c;
+            const uint4 v0 = src[c];
+            const uint4 v1 = src[c2];
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t b0 = (w[k >> 1] >> (16 * (k & 1))) & 0xffu;
-                const uint32_t b1 = (w[k >> 1] >> (16 * (k & 1) + 8)) & 0xffu;
-                const uint32_t i0 = (b0 >> 1) - (b0 >> 3);  // 1->0 2->1 4->2 8->3
-                const uint32_t i1 = (b1 >> 1) - (b1 >> 3);
-                o[k] = (i0 * kBlk) | ((i1 * kBlk) << 16);
+            for (int h2 = 0; h2 < 2; h2++) {
+                const uint4 v = h2 ? v1 : v0;
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+                uint32_t o[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t b0 = (w[k >> 1] >> (16 * (k & 1))) & 0xffu;
+                    const uint32_t b1 = (w[k >> 1] >> (16 * (k & 1) + 8)) & 0xffu;
+                    const uint32_t i0 = (b0 >> 1) - (b0 >> 3);  // 1->0 2->1 4->2 8->3
+                    const uint32_t i1 = (b1 >> 1) - (b1 >> 3);
+                    o[k] = (i0 * kBlk) | ((i1 * kBlk) << 16);
+                }
+                uint4 *dst = reinterpret_cast<uint4 *>(codes + L + 16 * (h2 ? c2 : c));
+                dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
+                dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
             }
-            uint4 *dst = reinterpret_cast<uint4 *>(codes + L + 16 * c);
-            dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-            dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
         }
     }
 
     Priors<float, MR> P;
     float X[MR], Y[MR];
     uint32_t rc[MR];
-    bool realv[MR];
 #pragma unroll
     for (int s = 0; s < MR; s++) {
-        const int pos = l * MR + s;
-        const int i = pos - off + 1;
-        const bool real = valid && i >= 1 && i <= R;  // lane L-1 included
-        load_row<float, MR, INTER>(a, P, s, real, ro + (uint64_t)(real ? i - 1 : 0));
+        row_priors<float, MR, INTER>(a, P, s, realv[s], raw[s]);
         X[s] = P.X[s];
         Y[s] = P.Y[s];
         rc[s] = P.rpos[s];
-        realv[s] = real;
     }
     // table: block c, lane l, slot s -> prior of row s when the hap base is c
     float *tl = reinterpret_cast<float *>(tab + (uint32_t)l * MRT * 4);
@@ -451,7 +497,9 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
         for (int s = 0; s < MR; s++)
             tl[c * (kBlk / 4) + s] = (rc[s] == (uint32_t)c) ? X[s] : Y[s];
     }
-    __syncthreads();
+    wave_lds_sync();
+    uint32_t next = 0;
+    if (lane == 0) next = atomicAdd(a.counter, 1u);
 
     State<float, MR> S;
 #pragma unroll
@@ -479,16 +527,16 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
     // computes, the code of column d+2 one step earlier still.  Codes are read
     // one per step with 2-byte loads: lanes start at different columns, so a
     // wider load would be unaligned for most of them.  Codes past the
-    // haplotype are zero (LDS cleared per item), so every prefetch address
-    // stays inside the group's table.
-    int d = 1;
+    // haplotype are zero (cleared per item), so every prefetch address stays
+    // inside the group's table.
+    int dd = 1;
     float pa[MR];
-    load_priors<MR>(pa, tlane + ld_code(cl + d));
-    uint32_t c1 = ld_code(cl + d + 1);
-    for (; d + kU - 1 <= Dmain; d += kU) {
+    load_priors<MR>(pa, tlane + ld_code(cl + dd));
+    uint32_t c1 = ld_code(cl + dd + 1);
+    for (; dd + kU - 1 <= Dmain; dd += kU) {
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            const uint32_t c2 = ld_code(cl + d + u + 2);
+            const uint32_t c2 = ld_code(cl + dd + u + 2);
             float pb[MR];
             step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1);
             sm += S.M[MR - 1];
@@ -498,11 +546,11 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
             c1 = c2;
         }
     }
-    for (; d <= Dend; d++) {
-        const uint32_t c2 = ld_code(cl + d + 2);
+    for (; dd <= Dend; dd++) {
+        const uint32_t c2 = ld_code(cl + dd + 2);
         float pb[MR];
         step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1);
-        if (d - l <= H) {
+        if (dd - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
         }
@@ -521,20 +569,27 @@ __device__ void process_item_tab(const FwdArgs &a, uint8_t *lds, int p) {
         }
         a.flags[p] = fl;
     }
+    // every lane has finished its LDS reads (their values were used above)
+    // before the next item clears and rewrites the areas
+    wave_lds_sync();
+    return __builtin_amdgcn_readfirstlane(next);
 }
 
+// Persistent: each wave takes items (G pairs) from the work counter, one
+// item ahead (process_item_tab fetches the next index), and reads each
+// item's descriptors with one load.  One wave per workgroup (the LDS
+// ordering in process_item_tab relies on it).
 template <int L, int MR, bool INTER>
 __global__ __launch_bounds__(64) void fwd_tab_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int G = 64 / L;
     const uint32_t n = a.n_items;
-    for (uint32_t iter = 0; iter <= n; iter++) {
-        uint32_t it = 0;
-        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= n) break;
-        const int p = a.items[(uint64_t)it * G + threadIdx.x / L];
-        process_item_tab<L, MR, INTER>(a, lds, p);
+    uint32_t it = 0;
+    if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+    it = __builtin_amdgcn_readfirstlane(it);
+    for (uint32_t iter = 0; iter < n && it < n; iter++) {
+        const ItemDesc d = a.desc[(uint64_t)it * G + threadIdx.x / L];
+        it = process_item_tab<L, MR, INTER>(a, lds, d);
     }
 }
 
@@ -762,6 +817,37 @@ hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st) {
     const int block = 256;
     const uint64_t grid = (f.n + block - 1) / block;
     hipLaunchKernelGGL(finalize_kernel, dim3((uint32_t)grid), dim3(block), 0, st, f);
+    return hipGetLastError();
+}
+
+// Item descriptors: one thread per item slot.
+__global__ void desc_kernel(FwdArgs a, const int32_t *items, ItemDesc *out, uint64_t n) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t p = items[k];
+    ItemDesc d;
+    d.p = p;
+    d.R = 0;
+    d.H = 0;
+    d.initd = 0.0f;
+    d.ro = 0;
+    d.hoff = 0;
+    if (p >= 0) {
+        const uint32_t h = a.pair_hap[p], r = a.pair_read[p];
+        d.R = a.read_len[r];
+        d.H = a.hap_len[h];
+        d.initd = a.initd_f[h];
+        d.ro = a.read_off[r];
+        d.hoff = a.hapc_off[h];
+    }
+    out[k] = d;
+}
+
+hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, uint64_t n,
+                       hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const uint64_t grid = (n + 255) / 256;
+    hipLaunchKernelGGL(desc_kernel, dim3((uint32_t)grid), dim3(256), 0, st, a, items, out, n);
     return hipGetLastError();
 }
 
