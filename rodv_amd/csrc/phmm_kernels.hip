@@ -342,19 +342,23 @@ __device__ __forceinline__ void load_priors(float (&pr)[MR], const uint8_t *p) {
 // The row above a lane's first row comes from the previous lane's last slot:
 // `up` (this step's column, rotated from the slot's value after the previous
 // step) and `dg` (the column before).  dg of step t is up of step t-1 for M
-// and I, so each step rotates three values: the new M and I of the last slot
-// (up of the next step) and its old D (dg of D for the next step).
+// and I, so each step rotates two values: the new M and I of the last slot
+// (up of the next step).  D runs along the row, so the lane recomputes the
+// slot's D itself -- D(j) = fma(D(j-1), yy, M(j-1) * my), the slot's own
+// instruction on the slot's own inputs (dg of the previous step), hence the
+// same bits -- from the slot's yy and my, rotated once per item.
 struct Carry {
     float dM, dI, dD;  // dg of this step
     float uM, uI;      // up of this step
+    float yyA, myA;    // yy and my of the row above the lane's first row
 };
 
 // One step of one lane (all MR rows at the next column), ordered for the LDS
-// unit: the old-D rotation and the next column's priors are issued first,
-// then M and D of every slot (which need only the previous column), then the
-// I chain down the slots (which needs `up`, rotated at the end of the
-// previous step), then the rotations for the next step.  LDS results return
-// in issue order, so each wait covers only what the next phase needs.
+// unit: the next column's priors are read first, then M and D of every slot
+// (which need only the previous column), then the I chain down the slots
+// (which needs `up`, rotated at the end of the previous step), then the
+// rotations for the next step.  LDS results return in issue order, so each
+// wait covers only what the next phase needs.
 template <int L, int MR, bool INTER>
 __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Priors<float, MR> &P,
                                          const float (&pr)[MR], const LaneRot<L> &rot,
@@ -362,7 +366,6 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
     // keep the previous step's M and I rotations ahead of this step's LDS
     // reads: the I chain below then waits for them alone
     __builtin_amdgcn_sched_barrier(0);
-    const float nextD = rot(S.D[MR - 1]);
     load_priors<MR>(pb, pnext);
     __builtin_amdgcn_sched_barrier(0);
     float nM[MR], nD[MR];
@@ -397,9 +400,9 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
         S.M[s] = nM[s];
         S.D[s] = nD[s];
     }
+    C.dD = fmaT(C.dD, C.yyA, C.dM * C.myA);  // the slot above, one column on
     C.dM = C.uM;
     C.dI = C.uI;
-    C.dD = nextD;
     __builtin_amdgcn_sched_barrier(0);
     C.uM = rot(S.M[MR - 1]);
     C.uI = rot(S.I[MR - 1]);
@@ -515,6 +518,8 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     C.dD = rot(S.D[MR - 1]);
     C.uM = 0.0f;  // rotations of the all-zero M and I of the first column
     C.uI = 0.0f;
+    C.yyA = rot(P.yy[MR - 1]);
+    C.myA = rot(P.my[MR - 1]);
 
     const int key = valid ? H + lR : 0x7fffffff;
     const int keymax = valid ? H + lR : -1;
