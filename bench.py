@@ -23,13 +23,16 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
 FLOP_PER_CELL = 12        # 8 mul + 4 add per forward cell (pairhmm_impl.cpp:266-289)
 FP_INSTR_PER_CELL = 8     # non-packed FP32 instructions of the bit-exact FMA split
+# adjoint, algorithmic work per cell: 12 forward + 24 reverse + 10 parameter
+# accumulation (the kernel's checkpoint recomputation is not counted)
+ADJ_FLOP_PER_CELL = 46
 
 # HBM bytes per FP32 forward pass (all length buckets), from two rocprofv3
 # --pmc runs of this script on the same workload: FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, the factors a 1 GiB device copy in the same run gives
 # (--copy-calibration-mib 1024).  Keyed by (config, scale, semantics,
 # autotuned).
-MEASURED_TRAFFIC = {(2, 1.0, "schedule", True): 2.557e10}
+MEASURED_TRAFFIC = {(2, 1.0, "schedule", True): 2.411e10}
 
 
 def log(msg):
@@ -265,9 +268,11 @@ def main():
         eng.adjoint(b3, theta, timing=True)
         times = [eng.adjoint(b3, theta, timing=True)[2] for _ in range(3)]
         adj_ms = min(times)
+        adj_tflops = ADJ_FLOP_PER_CELL * b3.cells() / (adj_ms / 1e3) / 1e12
         adj = {"gcups": round(b3.cells() / (adj_ms / 1e3) / 1e9, 2), "kernel_ms": round(adj_ms, 3),
                "workload": "config3: 4096 pairs, R=150, H=400, ckpt every 32 steps",
-               "cells": b3.cells()}
+               "cells": b3.cells(), "flop_per_cell": ADJ_FLOP_PER_CELL,
+               "tflops": round(adj_tflops, 2), "frac_of_fp32_peak": round(adj_tflops / FP32_PEAK_TFLOPS, 4)}
         log(f"adjoint: {adj_ms:.2f} ms, {adj['gcups']} GCUPS")
 
     achieved = FLOP_PER_CELL * cells / (fp32_ms / 1e3) / 1e12
