@@ -46,6 +46,8 @@ struct AdjRows {
     uint32_t isN[MR];  // N in the read: always the match prior
     // transitions of the row below the lane's last row (0 across the wrap)
     T nb_mm, nb_g, nb_mx, nb_xx;
+    // yy and my of the row above the lane's first row (step_c's D recurrence)
+    T ab_yy, ab_my;
 };
 
 template <typename T> struct PTab;
@@ -100,18 +102,19 @@ __device__ __forceinline__ void get_state(const T *base, int nvec, int k, State<
 }
 
 // Adjoint of one forward step d for one lane.  Sb = state after step d-1,
-// Sa = state after step d.  Adjoint registers hold the lane's values at its
+// Sa = state after step d (states of step_c: Mp, Ip, Dp hold the row above
+// one column back).  Adjoint registers hold the lane's values at its
 // previous reverse step (column j+1); Tb0p holds slot 0's Tb one step older.
 template <typename T, int MR>
 __device__ __forceinline__ void adj_step(const State<T, MR> &Sb, const State<T, MR> &Sa,
                                          const Priors<T, MR> &P, const AdjRows<T, MR> &A,
                                          uint32_t hc, T seed, T (&Tb)[MR], T (&Ib)[MR],
                                          T (&Db)[MR], T &Tb0p, T (&gd)[MR], T (&gg)[MR],
-                                         T (&ge)[MR], const LaneXfer<kL, 1> &from_above,
-                                         const LaneXfer<kL, -1> &from_below) {
-    // forward inputs of slot 0 from lane l-1
-    const T dgM0 = from_above(Sb.Mp), dgI0 = from_above(Sb.Ip), dgD0 = from_above(Sb.Dp);
-    const T upM0 = from_above(Sb.M[MR - 1]), upI0 = from_above(Sb.I[MR - 1]);
+                                         T (&ge)[MR], const LaneXfer<kL, -1> &from_below) {
+    // forward inputs of slot 0 (the row above): at column j-1 they are what
+    // step d read (Sb's carry), at column j what step d carried on (Sa's)
+    const T dgM0 = Sb.Mp, dgI0 = Sb.Ip, dgD0 = Sb.Dp;
+    const T upM0 = Sa.Mp, upI0 = Sa.Ip;
     // adjoint inputs of the last slot from lane l+1
     T bT = from_below(Tb0p);   // Tb(i+1, j+1)
     T bI = from_below(Ib[0]);  // Ib(i+1, j)
@@ -177,21 +180,21 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
     auto window4 = [&](const State<T, MR> &W0, int dw) {
         State<T, MR> W1, W2, W3, W4;
         W1 = W0;
-        step<T, kL, MR, false>(W1, P, cp[dw + 1], up);
+        step_c<T, kL, MR, false>(W1, P, A.ab_yy, A.ab_my, cp[dw + 1], up);
         W2 = W1;
-        step<T, kL, MR, false>(W2, P, cp[dw + 2], up);
+        step_c<T, kL, MR, false>(W2, P, A.ab_yy, A.ab_my, cp[dw + 2], up);
         W3 = W2;
-        step<T, kL, MR, false>(W3, P, cp[dw + 3], up);
+        step_c<T, kL, MR, false>(W3, P, A.ab_yy, A.ab_my, cp[dw + 3], up);
         W4 = W3;
-        step<T, kL, MR, false>(W4, P, cp[dw + 4], up);
+        step_c<T, kL, MR, false>(W4, P, A.ab_yy, A.ab_my, cp[dw + 4], up);
         auto seed_at = [&](int d) {
             const int j = d - l;
             return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
         };
-        adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
-        adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
-        adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
-        adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge, up, down);
+        adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
+        adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
+        adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
+        adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
     };
     for (int k = nseg - 1; k >= 0; k--) {
         const int d0 = k * K1;                       // state after step d0
@@ -203,7 +206,7 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
         for (int w = 0; w < nwin; w++) {
             put_state(l2, NV, w, S);
             const int n = min(2 * kW, nsteps - w * 2 * kW);
-            for (int u = 0; u < n; u++) step<T, kL, MR, false>(S, P, cp[d0 + w * 2 * kW + u + 1], up);
+            for (int u = 0; u < n; u++) step_c<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d0 + w * 2 * kW + u + 1], up);
         }
         __syncthreads();
         for (int w = nwin - 1; w >= 0; w--) {
@@ -215,7 +218,7 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
                 // second half first: advance a copy 4 steps, then its window
                 State<T, MR> X = W0;
 #pragma unroll
-                for (int u = 1; u <= kW; u++) step<T, kL, MR, false>(X, P, cp[dw + u], up);
+                for (int u = 1; u <= kW; u++) step_c<T, kL, MR, false>(X, P, A.ab_yy, A.ab_my, cp[dw + u], up);
                 window4(X, dw + kW);
             }
             window4(W0, dw);
@@ -334,9 +337,14 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         S0.I[s] = T(0);
         S0.D[s] = real ? T(0) : initd;
     }
+    const LaneXfer<kL, 1> up;
+    // step_c's carry: the row above before the first column (M = I = 0, its
+    // initial D), and that row's yy and my
     S0.Mp = T(0);
     S0.Ip = T(0);
-    S0.Dp = S0.D[MR - 1];
+    S0.Dp = up(S0.D[MR - 1]);
+    A.ab_yy = up(P.yy[MR - 1]);
+    A.ab_my = up(P.my[MR - 1]);
     {
         // transitions of the row below this lane's last row: lane l+1's slot 0
         const LaneXfer<kL, -1> down;
@@ -357,9 +365,8 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     State<T, MR> S = S0;
     T sm = T(0), si = T(0);
     put_state(ck, NV, 0, S);
-    const LaneXfer<kL, 1> up;
     for (int d = 1; d <= D4; d++) {
-        step<T, kL, MR, false>(S, P, cp[d], up);
+        step_c<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
         if (d <= Dend && d - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
@@ -386,7 +393,7 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
 }
 
 template <typename T, int MR, bool RESCUE>
-__global__ __launch_bounds__(64) void adj_kernel(AdjArgs a) {
+__global__ __launch_bounds__(64, (MR <= 4 && !RESCUE ? 2 : 1)) void adj_kernel(AdjArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t n = RESCUE ? *a.n_items_dev : a.n_items;
     // a wave takes at most n items: the bound makes every wave exit

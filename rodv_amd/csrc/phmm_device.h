@@ -134,5 +134,50 @@ __device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, ui
     S.Dp = dgD;
 }
 
+// The same step with the row above's previous-column values carried instead
+// of rotated: S.Mp, S.Ip, S.Dp hold the previous lane's last row one column
+// before this step's (dg).  dg of M and I is the previous step's up; D runs
+// along the row, so the lane recomputes it with the slot's own instruction,
+// fma(D, yy, M * my), from the slot's yy and my (yyA, myA, rotated once per
+// pair) -- the same bits.  Two rotations per step instead of five.  Single
+// strips only: a feeder lane's injected D would not follow the recurrence.
+template <typename T, int L, int MR, bool INTER>
+__device__ __forceinline__ void step_c(State<T, MR> &S, const Priors<T, MR> &P, T yyA, T myA,
+                                       uint32_t hc, const LaneRot<L> &rot) {
+    const T upM0 = rot(S.M[MR - 1]), upI0 = rot(S.I[MR - 1]);
+    T dgM = S.Mp, dgI = S.Ip, dgD = S.Dp;
+    T upM = upM0, upI = upI0;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
+        T t;
+        if (INTER) {
+            t = dgI * P.g[s];
+            t = fmaT(dgM, P.mm[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        } else {
+            t = dgM * P.mm[s];
+            t = fmaT(dgI, P.g[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        }
+        T nM = t * pr;
+        T u = upM * P.mx[s];
+        T nI = fmaT(upI, P.xx[s], u);
+        T v = S.M[s] * P.my[s];
+        T nD = fmaT(S.D[s], P.yy[s], v);
+        dgM = S.M[s];
+        dgI = S.I[s];
+        dgD = S.D[s];
+        upM = nM;
+        upI = nI;
+        S.M[s] = nM;
+        S.I[s] = nI;
+        S.D[s] = nD;
+    }
+    S.Dp = fmaT(S.Dp, yyA, S.Mp * myA);
+    S.Mp = upM0;
+    S.Ip = upI0;
+}
+
 }  // namespace dev
 }  // namespace rodv
