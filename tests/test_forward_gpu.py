@@ -110,6 +110,15 @@ def test_rescue_unrelated_reads(engine, oracle_mod):
     assert (fl & FLAG_RESCUED).sum() > 10
 
 
+def test_long_haplotypes_short_reads(engine, oracle_mod):
+    # haplotypes too long for four or two groups' codes in LDS take the
+    # 64-lane shapes; reads of every length class around them
+    rng = np.random.default_rng(11)
+    b = synth.random_batch(rng, 4, 12, hap_len=(2500, 9000), read_len=(10, 300))
+    check_against_oracle(oracle_mod, engine, b)
+    check_against_oracle(oracle_mod, engine, b, prior_table=0)
+
+
 def test_fp64_underflow_long_reads(engine, oracle_mod):
     b = synth.config(4, scale=4 / 256)
     out, fl, o = check_against_oracle(oracle_mod, engine, b)
