@@ -46,7 +46,7 @@ struct AdjRows {
     uint32_t isN[MR];  // N in the read: always the match prior
     // transitions of the row below the lane's last row (0 across the wrap)
     T nb_mm, nb_g, nb_mx, nb_xx;
-    // yy and my of the row above the lane's first row (step_c's D recurrence)
+    // yy and my of the row above the lane's first row (step's D recurrence)
     T ab_yy, ab_my;
 };
 
@@ -102,7 +102,7 @@ __device__ __forceinline__ void get_state(const T *base, int nvec, int k, State<
 }
 
 // Adjoint of one forward step d for one lane.  Sb = state after step d-1,
-// Sa = state after step d (states of step_c: Mp, Ip, Dp hold the row above
+// Sa = state after step d (states of step: Mp, Ip, Dp hold the row above
 // one column back).  Adjoint registers hold the lane's values at its
 // previous reverse step (column j+1); Tb0p holds slot 0's Tb one step older.
 template <typename T, int MR>
@@ -180,13 +180,13 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
     auto window4 = [&](const State<T, MR> &W0, int dw) {
         State<T, MR> W1, W2, W3, W4;
         W1 = W0;
-        step_c<T, kL, MR, false>(W1, P, A.ab_yy, A.ab_my, cp[dw + 1], up);
+        step<T, kL, MR, false>(W1, P, A.ab_yy, A.ab_my, cp[dw + 1], up);
         W2 = W1;
-        step_c<T, kL, MR, false>(W2, P, A.ab_yy, A.ab_my, cp[dw + 2], up);
+        step<T, kL, MR, false>(W2, P, A.ab_yy, A.ab_my, cp[dw + 2], up);
         W3 = W2;
-        step_c<T, kL, MR, false>(W3, P, A.ab_yy, A.ab_my, cp[dw + 3], up);
+        step<T, kL, MR, false>(W3, P, A.ab_yy, A.ab_my, cp[dw + 3], up);
         W4 = W3;
-        step_c<T, kL, MR, false>(W4, P, A.ab_yy, A.ab_my, cp[dw + 4], up);
+        step<T, kL, MR, false>(W4, P, A.ab_yy, A.ab_my, cp[dw + 4], up);
         auto seed_at = [&](int d) {
             const int j = d - l;
             return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
@@ -206,7 +206,7 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
         for (int w = 0; w < nwin; w++) {
             put_state(l2, NV, w, S);
             const int n = min(2 * kW, nsteps - w * 2 * kW);
-            for (int u = 0; u < n; u++) step_c<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d0 + w * 2 * kW + u + 1], up);
+            for (int u = 0; u < n; u++) step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d0 + w * 2 * kW + u + 1], up);
         }
         __syncthreads();
         for (int w = nwin - 1; w >= 0; w--) {
@@ -218,7 +218,7 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
                 // second half first: advance a copy 4 steps, then its window
                 State<T, MR> X = W0;
 #pragma unroll
-                for (int u = 1; u <= kW; u++) step_c<T, kL, MR, false>(X, P, A.ab_yy, A.ab_my, cp[dw + u], up);
+                for (int u = 1; u <= kW; u++) step<T, kL, MR, false>(X, P, A.ab_yy, A.ab_my, cp[dw + u], up);
                 window4(X, dw + kW);
             }
             window4(W0, dw);
@@ -338,7 +338,7 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         S0.D[s] = real ? T(0) : initd;
     }
     const LaneXfer<kL, 1> up;
-    // step_c's carry: the row above before the first column (M = I = 0, its
+    // step's carry: the row above before the first column (M = I = 0, its
     // initial D), and that row's yy and my
     S0.Mp = T(0);
     S0.Ip = T(0);
@@ -366,7 +366,7 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     T sm = T(0), si = T(0);
     put_state(ck, NV, 0, S);
     for (int d = 1; d <= D4; d++) {
-        step_c<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
+        step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
         if (d <= Dend && d - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];

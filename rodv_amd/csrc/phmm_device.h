@@ -78,8 +78,8 @@ template <> struct Tab<double> {
     static __device__ __forceinline__ double initd(const FwdArgs &a, uint32_t h) { return a.initd_d[h]; }
 };
 
-// Per-lane register state: MR rows at the lane's last column, plus the last
-// row one column earlier (read by the next lane as its diagonal).
+// Per-lane register state: MR rows at the lane's last column, plus the row
+// above the lane's first row one column earlier (its diagonal inputs).
 template <typename T, int MR>
 struct State {
     T M[MR], I[MR], D[MR];
@@ -93,56 +93,15 @@ struct Priors {
     uint32_t rpos[MR];
 };
 
-// One anti-diagonal step of one lane: all MR rows at the next column.  `rot`
-// (LaneRot<L>) brings the previous lane's values.
+// One anti-diagonal step of one lane: all MR rows at the next column.  The
+// row above the lane's first row is the previous lane's last row: at this
+// column (up, rotated in by `rot` from the previous lane's state) and one
+// column before (dg, carried in S.Mp, S.Ip, S.Dp).  dg of M and I is the
+// previous step's up; D runs along the row, so the lane recomputes it with
+// that row's own instruction, fma(D, yy, M * my), from the row's yy and my
+// (yyA, myA, set once per strip) -- the same bits.  Two rotations per step.
 template <typename T, int L, int MR, bool INTER>
-__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, uint32_t hc,
-                                     const LaneRot<L> &rot) {
-    // row above the lane's first row: previous lane's last row, at this column
-    // (up) and the column before (diagonal)
-    T dgM = rot(S.Mp), dgI = rot(S.Ip), dgD = rot(S.Dp);
-    T upM = rot(S.M[MR - 1]), upI = rot(S.I[MR - 1]);
-#pragma unroll
-    for (int s = 0; s < MR; s++) {
-        T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
-        T t;
-        if (INTER) {  // pairhmm_inter.cpp:279-283 as GCC contracts it
-            t = dgI * P.g[s];
-            t = fmaT(dgM, P.mm[s], t);
-            t = fmaT(dgD, P.g[s], t);
-        } else {  // pairhmm_impl.cpp:266-275
-            t = dgM * P.mm[s];
-            t = fmaT(dgI, P.g[s], t);
-            t = fmaT(dgD, P.g[s], t);
-        }
-        T nM = t * pr;
-        T u = upM * P.mx[s];
-        T nI = fmaT(upI, P.xx[s], u);
-        T v = S.M[s] * P.my[s];
-        T nD = fmaT(S.D[s], P.yy[s], v);
-        dgM = S.M[s];
-        dgI = S.I[s];
-        dgD = S.D[s];
-        upM = nM;
-        upI = nI;
-        S.M[s] = nM;
-        S.I[s] = nI;
-        S.D[s] = nD;
-    }
-    S.Mp = dgM;
-    S.Ip = dgI;
-    S.Dp = dgD;
-}
-
-// The same step with the row above's previous-column values carried instead
-// of rotated: S.Mp, S.Ip, S.Dp hold the previous lane's last row one column
-// before this step's (dg).  dg of M and I is the previous step's up; D runs
-// along the row, so the lane recomputes it with the slot's own instruction,
-// fma(D, yy, M * my), from the slot's yy and my (yyA, myA, rotated once per
-// pair) -- the same bits.  Two rotations per step instead of five.  Single
-// strips only: a feeder lane's injected D would not follow the recurrence.
-template <typename T, int L, int MR, bool INTER>
-__device__ __forceinline__ void step_c(State<T, MR> &S, const Priors<T, MR> &P, T yyA, T myA,
+__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, T yyA, T myA,
                                        uint32_t hc, const LaneRot<L> &rot) {
     const T upM0 = rot(S.M[MR - 1]), upI0 = rot(S.I[MR - 1]);
     T dgM = S.Mp, dgI = S.Ip, dgD = S.Dp;

@@ -202,9 +202,21 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
             S.I[s] = T(0);
             S.D[s] = real ? T(0) : initd;
         }
+        // step's carry: the row above before the first column and that
+        // row's yy and my.  Lane 0's row above is lane L-1's identity slot in
+        // the first strip, the previous strip's last read row in the others
+        // (column 0 of a read row is zero; its yy and my from the read).
+        const LaneRot<L> rot;
         S.Mp = T(0);
         S.Ip = T(0);
-        S.Dp = S.D[MR - 1];
+        S.Dp = rot(S.D[MR - 1]);
+        T yyA = rot(P.yy[MR - 1]), myA = rot(P.my[MR - 1]);
+        if (sidx > 0 && l == 0) {
+            const uint64_t ip = ro + (uint64_t)(sidx * C - off - 1);  // 0-based row above
+            yyA = Tab<T>::ph2pr(a.tab)[a.gcp[ip] & 127u];
+            myA = Tab<T>::ph2pr(a.tab)[a.del[ip] & 127u];
+            S.Dp = T(0);
+        }
 
         // boundary rows between strips (L = 64 kernels only)
         T *bnd_in = nullptr, *bnd_out = nullptr;
@@ -228,7 +240,6 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
                     S.M[MR - 1] = bm;
                     S.I[MR - 1] = bi;
                     S.D[MR - 1] = bd;
-                    S.Dp = T(0);
                 }
             }
         }
@@ -263,7 +274,6 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
         };
 
         int d = 1;
-        const LaneRot<L> rot;
         // main loop: every lane's sum slot is at a column <= H
         for (; d + kU - 1 <= Dmain; d += kU) {
             uint32_t hc[kU];
@@ -271,7 +281,7 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
             for (int u = 0; u < kU; u++) hc[u] = cp[d + u];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
-                step<T, L, MR, INTER>(S, P, hc[u], rot);
+                step<T, L, MR, INTER>(S, P, yyA, myA, hc[u], rot);
                 sm += S.M[MR - 1];
                 si += S.I[MR - 1];
                 after_step(d + u);
@@ -279,7 +289,7 @@ __device__ void process_item(const FwdArgs &a, uint8_t *lds, int p) {
         }
         // remaining steps: sums only while the lane's column is <= H
         for (; d <= Dend; d++) {
-            step<T, L, MR, INTER>(S, P, cp[d], rot);
+            step<T, L, MR, INTER>(S, P, yyA, myA, cp[d], rot);
             if (last && d - l <= H) {
                 sm += S.M[MR - 1];
                 si += S.I[MR - 1];
