@@ -334,14 +334,19 @@ __device__ __forceinline__ uint32_t ld_code(const uint16_t *p) {
     return __atomic_load_n(p, __ATOMIC_RELAXED);
 }
 
-// the MR priors of one lane for one column: MRT/4 ds_read_b128
-template <int MR>
+// The prior table of a group: [4 hap bases][MRT/4 chunks][L lanes][4 rows]
+// of floats, so the 16 lanes of an LDS pass read 16 consecutive 16-byte
+// chunks (all 64 banks once) whatever MRT is; a lane-major layout put lanes
+// MRT*4 bytes apart, which is 2- and 4-way bank conflicts at MRT 8 and 16.
+// The MR priors of one lane for one column: MRT/4 ds_read_b128 from the
+// lane's chunk of the column's block.
+template <int L, int MR>
 __device__ __forceinline__ void load_priors(float (&pr)[MR], const uint8_t *p) {
     constexpr int MRT = TabLayout<MR>::MRT;
     const float4 *tp = reinterpret_cast<const float4 *>(p);
 #pragma unroll
     for (int q = 0; q < MRT / 4; q++) {
-        const float4 v = tp[q];
+        const float4 v = tp[q * L];
         if (4 * q + 0 < MR) pr[4 * q + 0] = v.x;
         if (4 * q + 1 < MR) pr[4 * q + 1] = v.y;
         if (4 * q + 2 < MR) pr[4 * q + 2] = v.z;
@@ -376,7 +381,7 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
     // keep the previous step's M and I rotations ahead of this step's LDS
     // reads: the I chain below then waits for them alone
     __builtin_amdgcn_sched_barrier(0);
-    load_priors<MR>(pb, pnext);
+    load_priors<L, MR>(pb, pnext);
     __builtin_amdgcn_sched_barrier(0);
     float nM[MR], nD[MR];
     float dgM = C.dM, dgI = C.dI, dgD = C.dD;
@@ -502,13 +507,14 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
         Y[s] = P.Y[s];
         rc[s] = P.rpos[s];
     }
-    // table: block c, lane l, slot s -> prior of row s when the hap base is c
-    float *tl = reinterpret_cast<float *>(tab + (uint32_t)l * MRT * 4);
+    // table: block c, chunk s/4, lane l, row s%4 -> prior of row s when the
+    // hap base is c
+    float *tl = reinterpret_cast<float *>(tab + (uint32_t)l * 16);
 #pragma unroll
     for (int c = 0; c < 4; c++) {
 #pragma unroll
         for (int s = 0; s < MR; s++)
-            tl[c * (kBlk / 4) + s] = (rc[s] == (uint32_t)c) ? X[s] : Y[s];
+            tl[c * (kBlk / 4) + (s >> 2) * (L * 4) + (s & 3)] = (rc[s] == (uint32_t)c) ? X[s] : Y[s];
     }
     wave_lds_sync();
     uint32_t next = 0;
@@ -536,7 +542,7 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     const int Dmain = wave_min4(key, G);
     const int Dend = wave_max4(keymax, G);
     const uint16_t *cl = codes + (L - 1) - l;
-    const uint8_t *tlane = tab + (uint32_t)l * MRT * 4;
+    const uint8_t *tlane = tab + (uint32_t)l * 16;
     float sm = 0.0f, si = 0.0f;
     // Software pipeline: the priors of column d+1 are read while step d
     // computes, the code of column d+2 one step earlier still.  Codes are read
@@ -546,7 +552,7 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     // inside the group's table.
     int dd = 1;
     float pa[MR];
-    load_priors<MR>(pa, tlane + ld_code(cl + dd));
+    load_priors<L, MR>(pa, tlane + ld_code(cl + dd));
     uint32_t c1 = ld_code(cl + dd + 1);
     for (; dd + kU - 1 <= Dmain; dd += kU) {
 #pragma unroll
