@@ -100,7 +100,9 @@ typedef struct rodv_phmm_config {
     int32_t use_graph;            /* capture the batched launch in a hipGraph   */
     int32_t waves_per_cu;         /* persistent grid size; 0 = engine default   */
     int32_t short_lanes;          /* lanes per pair for reads <= 255 (N-free
-                                     haplotypes; <= 240 with N): 16 or 64      */
+                                     haplotypes; <= 240 with N): 16 (built-in
+                                     shapes: 8 lanes for 40..95 rows, else 16)
+                                     or 64                                     */
     int32_t extended_range;       /* write finite log10 where FP64 underflows   */
     int32_t prior_table;          /* 1: per-pair LDS table of emission priors for
                                      N-free haplotypes (default); 0: select per cell */
@@ -174,7 +176,8 @@ int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n);
  * class present in `sample`, rodv_phmm_autotune times the FP32 forward of
  * (up to 131,072 of) the sample's pairs of that class under every candidate
  * shape -- pairs per wavefront 4, 2 or 1 (16, 32 or 64 lanes per pair, rows
- * per lane as few as the read needs), for the longest class the 64-lane
+ * per lane as few as the read needs), 8 (8 lanes, 6..12 rows per lane) for
+ * the classes from 32..47 to 80..95 rows, for the longest class the 64-lane
  * strip width (rows per lane 4..16, shorter strips for longer reads) -- then
  * the persistent waves per CU of the best shape, and keeps the fastest
  * in the context.  Plans created afterwards use it (config use_tuning).  All
@@ -186,7 +189,7 @@ int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n);
 
 typedef struct rodv_phmm_tune_entry {
     uint32_t max_read_len;      /* the class's longest read (UINT32_MAX: open)  */
-    uint32_t pairs_per_wave;    /* 4, 2 or 1; 0: class not tuned (defaults)     */
+    uint32_t pairs_per_wave;    /* 8, 4, 2 or 1; 0: class not tuned (defaults)  */
     uint32_t max_rows_per_lane; /* strip width cap of 64-lane shapes; 0 = 16    */
     uint32_t waves_per_cu;      /* persistent grid per CU; 0 = engine default   */
     uint32_t sample_pairs;      /* pairs of the sample timed for this class     */

@@ -20,7 +20,7 @@ namespace dev {
 // Every lane of the wave must be active.
 template <int L, int DIR>
 struct LaneXfer {
-    static_assert(L == 16 || L == 32 || L == 64, "lanes per pair");
+    static_assert(L == 8 || L == 16 || L == 32 || L == 64, "lanes per pair");
     static_assert(DIR == 1 || DIR == -1, "direction");
     int addr;
     __device__ __forceinline__ LaneXfer() {

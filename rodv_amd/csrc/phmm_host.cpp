@@ -334,12 +334,13 @@ void choose_shape(uint32_t R, int short_lanes, int *L, int *MR) {
 }
 
 // Rows per lane of the table-prior kernel with L lanes for a read of R rows,
-// or 0 when no instantiated shape (16 lanes: 1..16 rows per lane, 32: 2..10,
-// 64: 4, 8, 12, 16 up to `cap`) holds it in one strip.  Every lane carries
-// read rows and the row-0 boundary needs one identity slot, so a strip of
-// L*MR slots holds R <= L*MR - 1 rows.
+// or 0 when no instantiated shape (8 lanes: 6..12 rows per lane, 16: 1..16,
+// 32: 2..10, 64: 4, 8, 12, 16 up to `cap`) holds it in one strip.  Every lane
+// carries read rows and the row-0 boundary needs one identity slot, so a
+// strip of L*MR slots holds R <= L*MR - 1 rows.
 int tab_rows(uint32_t R, int L, int cap) {
     int mr = (int)((R + (uint32_t)L) / (uint32_t)L);
+    if (L == 8) return mr >= 6 && mr <= 12 ? mr : 0;
     if (L == 16) return mr <= 16 ? mr : 0;
     if (L == 32) {
         mr = std::max(2, mr);
@@ -349,11 +350,19 @@ int tab_rows(uint32_t R, int L, int cap) {
     return mr <= (cap > 0 ? cap : 16) ? mr : 0;
 }
 
-// Lanes per pair of the built-in table shape: 16 while 16 lanes hold the
-// read in one strip, else 64.
-int default_tab_lanes(uint32_t R, int short_lanes) {
-    return short_lanes != 64 && tab_rows(R, 16, 0) ? 16 : 64;
+// Lanes per pair of the built-in table shape: 8 for reads of 40..95 rows when
+// there are many of them (6..12 rows per lane: 7-25 % faster than 16 lanes
+// of 3..6 rows, but an item holds 8 pairs, so a small batch fills the device
+// worse), 16 while 16 lanes hold the read in one strip, else 64.
+int default_tab_lanes(uint32_t R, int short_lanes, bool many_short) {
+    if (short_lanes == 64) return 64;
+    if (many_short && R >= 40u && R <= 95u) return 8;
+    return tab_rows(R, 16, 0) ? 16 : 64;
 }
+
+// Pairs of reads of 40..95 rows that make 8-lane items worth it: four items
+// per persistent wave of the default grid.
+uint64_t many_short_pairs(int num_cus) { return (uint64_t)num_cus * kDefaultWavesPerCU * 8u * 4u; }
 
 // The groups of one wave stage their haplotype codes (and prior tables) in
 // LDS; a very long haplotype does not fit four or two of them.
@@ -365,12 +374,14 @@ bool codes_fit(uint32_t H, int L, int MR);
 // haplotype, prior tables on) and the read fits one table strip; otherwise
 // the select-prior kernel, whose 32-lane shapes do not exist, takes it.
 void shape_for(uint32_t R, uint32_t H, const rodv_phmm_config &cfg, const TuneEntry *te, bool tab_ok,
-               int *L, int *MR, bool *tab) {
+               bool many_short, int *L, int *MR, bool *tab) {
     const int want = te ? te->L : 0;
     const int cap = want == 64 ? te->mr_cap : 0;
     if (tab_ok) {
-        int l = want ? want : default_tab_lanes(R, cfg.short_lanes);
+        const int l0 = default_tab_lanes(R, cfg.short_lanes, many_short);
+        int l = want ? want : l0;
         int mr = tab_rows(R, l, cap);
+        if (mr == 0 && l != l0) mr = tab_rows(R, l = l0, 0);
         if (mr == 0 && l != 64) mr = tab_rows(R, l = 64, 0);
         if (mr && !codes_fit(H, l, mr)) mr = tab_rows(R, l = 64, cap);
         if (mr) {
@@ -406,7 +417,7 @@ uint32_t lds_stride_for(uint32_t max_h, int L) {
 
 bool codes_fit(uint32_t H, int L, int MR) {
     if (L == 64) return true;
-    const uint64_t per = 2ull * lds_stride_for(H, L) + fwd_tab_bytes(L, MR);
+    const uint64_t per = lds_stride_for(H, L) + 256u + fwd_tab_bytes(L, MR);
     return (64ull / (uint64_t)L) * per <= 60u * 1024u;
 }
 
@@ -747,9 +758,15 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         p->any_inter = !sched_list.empty();
         p->n_regions = (uint32_t)nr;
     }
-    std::vector<int> bucket_of_key(4 * 3 * 17, -1);
+    std::vector<int> bucket_of_key(4 * 4 * 17, -1);
     std::vector<std::vector<uint32_t>> members;
     uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
+    uint64_t n_short = 0;
+    for (uint64_t k = 0; k < n_pairs; k++) {
+        const uint32_t R = b->read_lengths[pr[k]];
+        n_short += (R >= 40u && R <= 95u) ? 1u : 0u;
+    }
+    const bool many_short = n_short >= many_short_pairs(ctx->num_cus);
     for (uint64_t k = 0; k < n_pairs; k++) {
         const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
         p->cells += (uint64_t)R * H;
@@ -763,10 +780,10 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         const bool tab_ok = cfg.prior_table && !hap_has_n[ph[k]];
         int L, MR;
         bool tab;
-        shape_for(R, H, cfg, te, tab_ok, &L, &MR, &tab);
+        shape_for(R, H, cfg, te, tab_ok, many_short, &L, &MR, &tab);
         const bool inter = !inter32.empty() && inter32[k];
-        const int lidx = L == 16 ? 0 : (L == 32 ? 1 : 2);
-        const int key = (((inter ? 1 : 0) * 2 + (tab ? 1 : 0)) * 3 + lidx) * 17 + MR;
+        const int lidx = L == 16 ? 0 : (L == 32 ? 1 : (L == 64 ? 2 : 3));
+        const int key = (((inter ? 1 : 0) * 2 + (tab ? 1 : 0)) * 4 + lidx) * 17 + MR;
         if (bucket_of_key[key] < 0) {
             bucket_of_key[key] = (int)p->buckets.size();
             Bucket bk;
@@ -807,10 +824,10 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             bk.cells += (uint64_t)R * H;
         }
         if (bk.tab) {
-            // u16 code per column, then the prior table.  A multiple of 256
-            // bytes: every group's table then starts on LDS bank 0, which
-            // keeps the four groups' 16-byte prior reads free of bank conflicts
-            bk.lds_stride = (2u * lds_stride_for(bk.max_h, bk.L) + 255u) & ~255u;
+            // one code byte per column, then the prior table.  A multiple of
+            // 256 bytes: every group's table then starts on LDS bank 0, which
+            // keeps the groups' 16-byte prior reads free of bank conflicts
+            bk.lds_stride = (lds_stride_for(bk.max_h, bk.L) + 255u) & ~255u;
             bk.tab_bytes = fwd_tab_bytes(bk.L, bk.MR);
         } else {
             bk.lds_stride = lds_stride_for(bk.max_h, bk.L);
@@ -1163,7 +1180,8 @@ namespace {
 constexpr uint32_t kMaxTunePairs = 131072;
 
 int lanes_of(uint32_t pairs_per_wave) {
-    return pairs_per_wave == 4 ? 16 : pairs_per_wave == 2 ? 32 : pairs_per_wave == 1 ? 64 : 0;
+    return pairs_per_wave == 8 ? 8 : pairs_per_wave == 4 ? 16 : pairs_per_wave == 2 ? 32
+                                                                 : pairs_per_wave == 1 ? 64 : 0;
 }
 
 // FP32 forward of the sub-batch under the context's current (forced) shape:
@@ -1242,6 +1260,7 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
         TuneEntry best{};
         double best_g = e.default_gcups;
         std::vector<TuneEntry> cands;
+        if (kClassMax[k] <= 95 && kClassMax[k] >= 47) cands.push_back(TuneEntry{8, 0, 0});
         if (kClassMax[k] <= 255) cands.push_back(TuneEntry{16, 0, 0});
         if (kClassMax[k] <= 319) cands.push_back(TuneEntry{32, 0, 0});
         if (kClassMax[k] <= 319) {
@@ -1261,7 +1280,8 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
         }
         if (rc) break;
         if (best.L == 0)  // the built-in shape won: name it
-            best.L = default_tab_lanes(kClassMax[k], cfg.short_lanes);
+            best.L = default_tab_lanes(kClassMax[k], cfg.short_lanes,
+                                       picked[k].size() >= many_short_pairs(ctx->num_cus));
         for (int w : {4, 8, 16}) {
             TuneEntry c = best;
             c.wpc = w;

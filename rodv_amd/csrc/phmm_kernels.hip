@@ -123,6 +123,12 @@ __device__ __forceinline__ void st_agent(T *p, T v) {
 // min / max of a per-group value over the G groups of the wave (lanes 0,
 // L, 2L, ... hold each group's copy)
 __device__ __forceinline__ int wave_min4(int v, int G) {
+    if (G == 8) {
+        int m = __builtin_amdgcn_readlane(v, 0);
+#pragma unroll
+        for (int k = 1; k < 8; k++) m = min(m, __builtin_amdgcn_readlane(v, 8 * k));
+        return m;
+    }
     if (G == 1) return __builtin_amdgcn_readfirstlane(v);
     if (G == 2) return min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
     int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
@@ -130,6 +136,12 @@ __device__ __forceinline__ int wave_min4(int v, int G) {
     return min(min(a, b), min(c, d));
 }
 __device__ __forceinline__ int wave_max4(int v, int G) {
+    if (G == 8) {
+        int m = __builtin_amdgcn_readlane(v, 0);
+#pragma unroll
+        for (int k = 1; k < 8; k++) m = max(m, __builtin_amdgcn_readlane(v, 8 * k));
+        return m;
+    }
     if (G == 1) return __builtin_amdgcn_readfirstlane(v);
     if (G == 2) return max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
     int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
@@ -327,10 +339,10 @@ struct TabLayout {
     static constexpr int MRT = (MR + 3) / 4 * 4;  // slots padded to 16 bytes
 };
 
-// One haplotype code (a u16 byte offset into the prior table).  A relaxed
-// atomic load is a plain ds_read_u16 that the compiler does not merge with
-// its neighbours into a wider, unaligned read.
-__device__ __forceinline__ uint32_t ld_code(const uint16_t *p) {
+// One haplotype code (the base's block 0..3 of the prior table).  A relaxed
+// atomic load is a plain ds_read_u8 that the compiler does not merge with its
+// neighbours into a wider, unaligned read.
+__device__ __forceinline__ uint32_t ld_code(const uint8_t *p) {
     return __atomic_load_n(p, __ATOMIC_RELAXED);
 }
 
@@ -443,10 +455,11 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     const int off = valid ? (MR - R % MR) % MR : 0;
     const int lR = valid ? (R - 1 + off) / MR : 0;
 
-    // LDS per group: codes (u16, lds_stride bytes) then the prior table
+    // LDS per group: codes (one byte per column, lds_stride bytes) then the
+    // prior table
     const uint32_t gstride = a.lds_stride + 4 * kBlk;
     uint8_t *gb = lds + g * gstride;
-    uint16_t *codes = reinterpret_cast<uint16_t *>(gb);
+    uint8_t *codes = gb;
     uint8_t *tab = gb + a.lds_stride;
 
     // codes outside the haplotype are zero: each group clears its code area
@@ -466,10 +479,11 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
         raw[s] = load_raw(a, ro + (uint64_t)(realv[s] ? i - 1 : 0));
     }
     if (valid) {
-        // column j (1-based) at codes[L-1+j]: one-hot 1,2,4,8 -> block offset;
-        // 16 columns per chunk from the 16-byte aligned codes, two chunks per
-        // lane and pass, both loads issued before either is used (without a
-        // second chunk the first is converted and written twice)
+        // column j (1-based) at codes[L-1+j]: one-hot 1,2,4,8 -> block 0..3,
+        // (b >> 1) - (b >> 3) on four bytes at once; 16 columns per chunk
+        // from the 16-byte aligned codes, two chunks per lane and pass, both
+        // loads issued before either is used (without a second chunk the
+        // first is converted and written twice)
         const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + d.hoff);
         const int nch = (H + 15) >> 4;
         for (int c = l; c < nch; c += 2 * L) {
@@ -481,18 +495,11 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
             for (int h2 = 0; h2 < 2; h2++) {
                 const uint4 v = h2 ? v1 : v0;
                 const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-                uint32_t o[8];
+                uint32_t o[4];
 #pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const uint32_t b0 = (w[k >> 1] >> (16 * (k & 1))) & 0xffu;
-                    const uint32_t b1 = (w[k >> 1] >> (16 * (k & 1) + 8)) & 0xffu;
-                    const uint32_t i0 = (b0 >> 1) - (b0 >> 3);  // 1->0 2->1 4->2 8->3
-                    const uint32_t i1 = (b1 >> 1) - (b1 >> 3);
-                    o[k] = (i0 * kBlk) | ((i1 * kBlk) << 16);
-                }
-                uint4 *dst = reinterpret_cast<uint4 *>(codes + L + 16 * (h2 ? c2 : c));
-                dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-                dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+                for (int k = 0; k < 4; k++)  // 1->0 2->1 4->2 8->3 per byte
+                    o[k] = ((w[k] >> 1) & 0x7f7f7f7fu) - ((w[k] >> 3) & 0x1f1f1f1fu);
+                *reinterpret_cast<uint4 *>(codes + L + 16 * (h2 ? c2 : c)) = make_uint4(o[0], o[1], o[2], o[3]);
             }
         }
     }
@@ -541,25 +548,25 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     const int keymax = valid ? H + lR : -1;
     const int Dmain = wave_min4(key, G);
     const int Dend = wave_max4(keymax, G);
-    const uint16_t *cl = codes + (L - 1) - l;
+    const uint8_t *cl = codes + (L - 1) - l;
     const uint8_t *tlane = tab + (uint32_t)l * 16;
     float sm = 0.0f, si = 0.0f;
     // Software pipeline: the priors of column d+1 are read while step d
     // computes, the code of column d+2 one step earlier still.  Codes are read
-    // one per step with 2-byte loads: lanes start at different columns, so a
+    // one per step with byte loads: lanes start at different columns, so a
     // wider load would be unaligned for most of them.  Codes past the
     // haplotype are zero (cleared per item), so every prefetch address stays
     // inside the group's table.
     int dd = 1;
     float pa[MR];
-    load_priors<L, MR>(pa, tlane + ld_code(cl + dd));
+    load_priors<L, MR>(pa, tlane + ld_code(cl + dd) * kBlk);
     uint32_t c1 = ld_code(cl + dd + 1);
     for (; dd + kU - 1 <= Dmain; dd += kU) {
 #pragma unroll
         for (int u = 0; u < kU; u++) {
             const uint32_t c2 = ld_code(cl + dd + u + 2);
             float pb[MR];
-            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1);
+            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
 #pragma unroll
@@ -570,7 +577,7 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     for (; dd <= Dend; dd++) {
         const uint32_t c2 = ld_code(cl + dd + 2);
         float pb[MR];
-        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1);
+        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
         if (dd - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
@@ -785,11 +792,18 @@ void *fwd_f32_kernel(int L, int MR, bool inter) {
     return nullptr;
 }
 
-// Table-prior FP32 kernels (haplotypes without N, one strip): 16 lanes per
-// pair with 1..16 rows per lane, 32 lanes with 2..10, 64 lanes with 4, 8, 12, 16.
+// Table-prior FP32 kernels (haplotypes without N, one strip): 8 lanes per
+// pair with 6..12 rows per lane, 16 lanes with 1..16, 32 lanes with 2..10,
+// 64 lanes with 4, 8, 12, 16.
 void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
 #define TK(l, m) \
     case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<l, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<l, m, false>);
+    if (L == 8) {
+        switch (MR) {
+            TK(8, 6) TK(8, 7) TK(8, 8) TK(8, 9) TK(8, 10) TK(8, 11) TK(8, 12)
+        default: return nullptr;
+        }
+    }
     if (L == 16) {
         switch (MR) {
             TK(16, 1) TK(16, 2) TK(16, 3) TK(16, 4) TK(16, 5) TK(16, 6) TK(16, 7) TK(16, 8)

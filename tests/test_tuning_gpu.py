@@ -49,15 +49,16 @@ def test_two_pairs_per_wave_every_rows_per_lane(tuned_engine, oracle_mod):
     check_against_oracle(oracle_mod, tuned_engine, b)
 
 
-@pytest.mark.parametrize("ppw,L", [(4, 16), (2, 32), (1, 64)])
+@pytest.mark.parametrize("ppw,L", [(8, 8), (4, 16), (2, 32), (1, 64)])
 def test_table_strip_uses_every_lane(tuned_engine, oracle_mod, ppw, L):
     # a table strip of L lanes x MR rows holds up to L*MR - 1 rows: the row-0
     # boundary is an identity slot at the top of lane 0 (R % MR != 0) or lane
-    # L-1 left without read rows (R % MR == 0); both ends of every step
-    mrs = {16: range(1, 17), 32: range(2, 11), 64: (4, 8, 12, 16)}[L]
+    # L-1 left without read rows (R % MR == 0); both ends of every step.
+    # 8 lanes exist for 6..12 rows per lane; other reads fall back.
+    mrs = {8: range(6, 13), 16: range(1, 17), 32: range(2, 11), 64: (4, 8, 12, 16)}[L]
     lens = sorted({x for m in mrs
                    for x in (L * m - 1, L * m - 2, (L - 1) * m, (L - 1) * m + 1, L * (m - 1))
-                   if x >= 1})
+                   if x >= 1} | ({1, 5, 20, 39, 96, 130} if L == 8 else set()))
     b = shape_batch(30 + L, lens, hap_lens=(1, 5, 40, 300))
     tuned_engine.set_tuning(forced(ppw))
     check_against_oracle(oracle_mod, tuned_engine, b)
