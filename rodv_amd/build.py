@@ -23,9 +23,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # is disabled for all GPU code.  The atomic optimizer is off: it turns a
 # one-lane atomicAdd into a form that waits for the result right away, where
 # the table kernel fetches its next work item early and reads it late.
+# Unaligned access mode is off so that the table kernel's per-step byte loads
+# of haplotype codes are not merged into unaligned wider reads.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero", "-fno-fast-math",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Xclang", "-target-feature", "-Xclang", "-unaligned-access-mode",
          "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
          "-Wall", "-Wno-unused-function", f"-I{os.path.join(ROOT, 'include')}"]
 
@@ -39,7 +42,8 @@ def build(verbose: bool = False) -> str:
     failed = []
     for out, proc in jobs:
         _, stderr = proc.communicate()
-        errs = "\n".join(l for l in (stderr or "").splitlines() if "packed-fp32-ops" not in l)
+        errs = "\n".join(l for l in (stderr or "").splitlines()
+                         if "packed-fp32-ops" not in l and "unaligned-access-mode" not in l)
         if proc.returncode != 0:
             failed.append(f"hipcc -> {os.path.basename(out)} failed ({proc.returncode}):\n{errs}")
             continue

@@ -339,12 +339,12 @@ struct TabLayout {
     static constexpr int MRT = (MR + 3) / 4 * 4;  // slots padded to 16 bytes
 };
 
-// One haplotype code (the base's block 0..3 of the prior table).  A relaxed
-// atomic load is a plain ds_read_u8 that the compiler does not merge with its
-// neighbours into a wider, unaligned read.
-__device__ __forceinline__ uint32_t ld_code(const uint8_t *p) {
-    return __atomic_load_n(p, __ATOMIC_RELAXED);
-}
+// One haplotype code (the base's block 0..3 of the prior table): one
+// ds_read_u8.  The library is built without unaligned-access-mode, so the
+// compiler does not merge the eight byte loads of a block into wider reads
+// that would be unaligned for most lanes, and it knows the load zero-extends
+// (an atomic load would cost an extra v_and per step).
+__device__ __forceinline__ uint32_t ld_code(const uint8_t *p) { return *p; }
 
 // The prior table of a group: [4 hap bases][MRT/4 chunks][L lanes][4 rows]
 // of floats, so the 16 lanes of an LDS pass read 16 consecutive 16-byte
