@@ -100,9 +100,7 @@ typedef struct rodv_phmm_config {
     int32_t use_graph;            /* capture the batched launch in a hipGraph   */
     int32_t waves_per_cu;         /* persistent grid size; 0 = engine default   */
     int32_t short_lanes;          /* lanes per pair for reads <= 255 (N-free
-                                     haplotypes; <= 240 with N): 16 (built-in
-                                     shapes: 8 lanes for 40..95 rows, else 16)
-                                     or 64                                     */
+                                     haplotypes; <= 240 with N): 16 or 64      */
     int32_t extended_range;       /* write finite log10 where FP64 underflows   */
     int32_t prior_table;          /* 1: per-pair LDS table of emission priors for
                                      N-free haplotypes (default); 0: select per cell */

@@ -350,19 +350,20 @@ int tab_rows(uint32_t R, int L, int cap) {
     return mr <= (cap > 0 ? cap : 16) ? mr : 0;
 }
 
-// Lanes per pair of the built-in table shape: 8 for reads of 40..95 rows when
-// there are many of them (6..12 rows per lane: 7-25 % faster than 16 lanes
-// of 3..6 rows, but an item holds 8 pairs, so a small batch fills the device
-// worse), 16 while 16 lanes hold the read in one strip, else 64.
-int default_tab_lanes(uint32_t R, int short_lanes, bool many_short) {
+// Lanes per pair of the built-in table shape: 16 while 16 lanes hold the
+// read in one strip, else 64.  (8 lanes for reads of 40..95 rows are an
+// autotuner candidate only: faster on uniform batches, but on batches with
+// mixed haplotype lengths as fast as 16 lanes at best and slower when the
+// batch is small, since an item then holds 8 pairs.)
+int default_tab_lanes(uint32_t R, int short_lanes) {
     if (short_lanes == 64) return 64;
-    if (many_short && R >= 40u && R <= 95u) return 8;
     return tab_rows(R, 16, 0) ? 16 : 64;
 }
 
-// Pairs of reads of 40..95 rows that make 8-lane items worth it: four items
-// per persistent wave of the default grid.
-uint64_t many_short_pairs(int num_cus) { return (uint64_t)num_cus * kDefaultWavesPerCU * 8u * 4u; }
+// Pairs the autotuner times per length class: enough to give every
+// persistent wave of the default grid four items of the largest (8-pair)
+// kind.
+uint64_t tune_pairs_min(int num_cus) { return (uint64_t)num_cus * kDefaultWavesPerCU * 8u * 4u; }
 
 // The groups of one wave stage their haplotype codes (and prior tables) in
 // LDS; a very long haplotype does not fit four or two of them.
@@ -374,11 +375,11 @@ bool codes_fit(uint32_t H, int L, int MR);
 // haplotype, prior tables on) and the read fits one table strip; otherwise
 // the select-prior kernel, whose 32-lane shapes do not exist, takes it.
 void shape_for(uint32_t R, uint32_t H, const rodv_phmm_config &cfg, const TuneEntry *te, bool tab_ok,
-               bool many_short, int *L, int *MR, bool *tab) {
+               int *L, int *MR, bool *tab) {
     const int want = te ? te->L : 0;
     const int cap = want == 64 ? te->mr_cap : 0;
     if (tab_ok) {
-        const int l0 = default_tab_lanes(R, cfg.short_lanes, many_short);
+        const int l0 = default_tab_lanes(R, cfg.short_lanes);
         int l = want ? want : l0;
         int mr = tab_rows(R, l, cap);
         if (mr == 0 && l != l0) mr = tab_rows(R, l = l0, 0);
@@ -761,12 +762,6 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     std::vector<int> bucket_of_key(4 * 4 * 17, -1);
     std::vector<std::vector<uint32_t>> members;
     uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
-    uint64_t n_short = 0;
-    for (uint64_t k = 0; k < n_pairs; k++) {
-        const uint32_t R = b->read_lengths[pr[k]];
-        n_short += (R >= 40u && R <= 95u) ? 1u : 0u;
-    }
-    const bool many_short = n_short >= many_short_pairs(ctx->num_cus);
     for (uint64_t k = 0; k < n_pairs; k++) {
         const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
         p->cells += (uint64_t)R * H;
@@ -780,7 +775,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         const bool tab_ok = cfg.prior_table && !hap_has_n[ph[k]];
         int L, MR;
         bool tab;
-        shape_for(R, H, cfg, te, tab_ok, many_short, &L, &MR, &tab);
+        shape_for(R, H, cfg, te, tab_ok, &L, &MR, &tab);
         const bool inter = !inter32.empty() && inter32[k];
         const int lidx = L == 16 ? 0 : (L == 32 ? 1 : (L == 64 ? 2 : 3));
         const int key = (((inter ? 1 : 0) * 2 + (tab ? 1 : 0)) * 4 + lidx) * 17 + MR;
@@ -1248,10 +1243,17 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
         rodv_phmm_tune_entry &e = res.cls[k];
         e.max_read_len = kClassMax[k];
         if (picked[k].size() < 64) continue;  // too few pairs to time: keep the defaults
+        e.sample_pairs = (uint32_t)picked[k].size();
+        // time every candidate on enough pairs to give each persistent wave
+        // several items of the largest (8-pair) kind, repeating the class's
+        // sample if needed: on fewer, shapes with fewer pairs per item win
+        // by filling the device better, which a full-size batch does not need
+        const size_t n_pick = picked[k].size();
+        for (size_t i = n_pick; i < tune_pairs_min(ctx->num_cus); i++)
+            picked[k].push_back(picked[k][i % n_pick]);
         rodv_phmm_batch sub = *sample;
         sub.num_regions = (uint32_t)picked[k].size();
         sub.regions = picked[k].data();
-        e.sample_pairs = sub.num_regions;
         // built-in shape
         ctx->force_class = k;
         ctx->force = TuneEntry{};
@@ -1280,8 +1282,7 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
         }
         if (rc) break;
         if (best.L == 0)  // the built-in shape won: name it
-            best.L = default_tab_lanes(kClassMax[k], cfg.short_lanes,
-                                       picked[k].size() >= many_short_pairs(ctx->num_cus));
+            best.L = default_tab_lanes(kClassMax[k], cfg.short_lanes);
         for (int w : {4, 8, 16}) {
             TuneEntry c = best;
             c.wpc = w;
