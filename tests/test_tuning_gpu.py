@@ -89,7 +89,7 @@ def test_autotune_changes_speed_not_results(tuned_engine):
     assert len(ents) == TUNE_CLASSES
     assert any(e["pairs_per_wave"] for e in ents)
     for e in ents:
-        assert e["pairs_per_wave"] in (0, 1, 2, 4)
+        assert e["pairs_per_wave"] in (0, 1, 2, 4, 8)
         if e["pairs_per_wave"]:
             assert e["gcups"] >= e["default_gcups"] > 0
     got = tuned_engine.get_tuning().entries()
