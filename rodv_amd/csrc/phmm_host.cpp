@@ -224,6 +224,9 @@ struct PlainLaunch {
 
 }  // namespace
 
+// FP32 buckets run on the launch stream and this many more streams, in turn
+constexpr int kSideStreams = 2;
+
 struct rodv_phmm_ctx {
     int device = 0;
     int num_cus = 256;
@@ -234,6 +237,10 @@ struct rodv_phmm_ctx {
     // forced entry while the tuner measures a candidate
     int force_class = -1;
     TuneEntry force{};
+    // more streams for the FP32 buckets (default priority) and the events
+    // that fork them from and join them to the launch stream
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
 };
 
 struct rodv_phmm_plan {
@@ -590,8 +597,14 @@ int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
         c->num_cus = cus;
     const HostTables &t = tables();
     int rc = upload(&c->d_tab, &t.dev, 1);
+    if (!rc && hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    for (int k = 0; k < kSideStreams && !rc; k++)
+        if (hipStreamCreateWithFlags(&c->side[k], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->join_ev[k], hipEventDisableTiming) != hipSuccess)
+            rc = RODV_PHMM_ERR_DEVICE;
+    if (rc) (void)hipGetLastError();
     if (rc) {
-        delete c;
+        rodv_phmm_ctx_destroy(c);
         return rc;
     }
     *out = c;
@@ -600,6 +613,11 @@ int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
 
 void rodv_phmm_ctx_destroy(rodv_phmm_ctx *ctx) {
     if (!ctx) return;
+    for (int k = 0; k < kSideStreams; k++) {
+        if (ctx->side[k]) (void)hipStreamDestroy(ctx->side[k]);
+        if (ctx->join_ev[k]) (void)hipEventDestroy(ctx->join_ev[k]);
+    }
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->d_tab) (void)hipFree(ctx->d_tab);
     delete ctx;
 }
@@ -1022,8 +1040,16 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
     // 259-262, 307-335): under intra semantics it changes nothing.
     {
         if (mode == 0) {
+            // buckets go round the launch stream and the context's side
+            // streams, so that one bucket's tail overlaps the next buckets'
+            // start (each persistent grid fills the device on its own)
+            const int ns = (int)std::min<size_t>(p->buckets.size(), 1 + kSideStreams);
+            if (ns > 1) HIPCHK(hipEventRecord(p->ctx->fork_ev, st));
+            for (int k = 0; k + 1 < ns; k++) HIPCHK(hipStreamWaitEvent(p->ctx->side[k], p->ctx->fork_ev, 0));
             for (size_t bi = 0; bi < p->buckets.size(); bi++) {
                 Bucket &bk = p->buckets[bi];
+                const int si = (int)(bi % (size_t)ns);
+                const hipStream_t bs = si == 0 ? st : p->ctx->side[si - 1];
                 FwdArgs a = base_args(p);
                 a.items = bk.d_items;
                 a.desc = bk.d_desc;
@@ -1034,8 +1060,12 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
                 const size_t lds = (size_t)(64 / bk.L) * (bk.lds_stride + bk.tab_bytes);
-                HIPCHK(launch_fwd(bk.kern, bk.grid, lds, st, a));
+                HIPCHK(launch_fwd(bk.kern, bk.grid, lds, bs, a));
                 launches++;
+            }
+            for (int k = 0; k + 1 < ns; k++) {
+                HIPCHK(hipEventRecord(p->ctx->join_ev[k], p->ctx->side[k]));
+                HIPCHK(hipStreamWaitEvent(st, p->ctx->join_ev[k], 0));
             }
         } else {
             for (size_t k = 0; k < p->plain.size(); k++) {
