@@ -224,8 +224,11 @@ struct PlainLaunch {
 
 }  // namespace
 
-// FP32 buckets run on the launch stream and this many more streams, in turn
-constexpr int kSideStreams = 2;
+// FP32 buckets run on the launch stream and this many more streams, in turn.
+// One: with two, the captured graph's replays slowed down by up to 3 % on
+// some plans (more streams than the device's hardware queues comfortably
+// serve), while one more stream gives the same overlap with stable replays.
+constexpr int kSideStreams = 1;
 
 struct rodv_phmm_ctx {
     int device = 0;
