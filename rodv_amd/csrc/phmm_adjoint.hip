@@ -43,7 +43,6 @@ struct AdjRows {
     T real[MR];     // 1 for a read row, 0 for an identity row
     T wq[MR];       // d e_i / d scale = 10^(-q_i/10)
     uint32_t bin[MR];
-    uint32_t isN[MR];  // N in the read: always the match prior
     // transitions of the row below the lane's last row (0 across the wrap)
     T nb_mm, nb_g, nb_mx, nb_xx;
     // yy and my of the row above the lane's first row (step's D recurrence)
@@ -109,8 +108,9 @@ template <typename T, int MR>
 __device__ __forceinline__ void adj_step(const State<T, MR> &Sb, const State<T, MR> &Sa,
                                          const Priors<T, MR> &P, const AdjRows<T, MR> &A,
                                          uint32_t hc, T seed, T (&Tb)[MR], T (&Ib)[MR],
-                                         T (&Db)[MR], T &Tb0p, T (&gd)[MR], T (&gg)[MR],
-                                         T (&ge)[MR], const LaneXfer<kL, -1> &from_below) {
+                                         T (&Db)[MR], T &Tb0p, T (&gd)[MR], T (&gdT)[MR],
+                                         T (&gg)[MR], T (&ge)[MR],
+                                         const LaneXfer<kL, -1> &from_below) {
     // forward inputs of slot 0 (the row above): at column j-1 they are what
     // step d read (Sb's carry), at column j what step d carried on (Sa's)
     const T dgM0 = Sb.Mp, dgI0 = Sb.Ip, dgD0 = Sb.Dp;
@@ -135,12 +135,25 @@ __device__ __forceinline__ void adj_step(const State<T, MR> &Sb, const State<T, 
         const T upI = s == 0 ? upI0 : Sa.I[s - 1];
         const T lfM = Sb.M[s], lfD = Sb.D[s];
         const T t = fmaT(P.g[s], dgD, fmaT(P.g[s], dgI, dgM * P.mm[s]));
-        const T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
-        const bool match = (((hc >> P.rpos[s]) & 1u) | A.isN[s]) != 0u;
+        // match: the hap code has the read base's bit (bit 4, always set in
+        // the staged codes, for N in the read); the prior and d e / d e_i
+        // (-1 on a match, 1/3 otherwise) are both selected by that bit
+        T pr, de;
+        if constexpr (sizeof(T) == 4) {
+            const int m = __builtin_amdgcn_sbfe((int)hc, P.rpos[s], 1);
+            pr = __int_as_float((__float_as_int(P.X[s]) & m) | (__float_as_int(P.Y[s]) & ~m));
+            de = __int_as_float((__float_as_int(-1.0f) & m) | (__float_as_int(1.0f / 3.0f) & ~m));
+        } else {
+            const bool match = ((hc >> P.rpos[s]) & 1u) != 0u;
+            pr = match ? P.X[s] : P.Y[s];
+            de = match ? T(-1) : T(1) / T(3);
+        }
         const T Tbn = pr * Mb;
-        gd[s] = fmaT(Dbn, lfM, fmaT(Ibn, upM, fmaT(T(-2) * Tbn, dgM, gd[s])));
+        // dS/ddelta = gd - 2 gdT (the -2 applied once, in the reduction)
+        gdT[s] = fmaT(Tbn, dgM, gdT[s]);
+        gd[s] = fmaT(Dbn, lfM, fmaT(Ibn, upM, gd[s]));
         gg[s] = fmaT(Dbn, lfD, fmaT(Ibn, upI, fmaT(-Tbn, dgI + dgD, gg[s])));
-        ge[s] = fmaT(Mb * t, match ? T(-1) : T(1) / T(3), ge[s]);
+        ge[s] = fmaT(Mb * t, de, ge[s]);
         Tb[s] = Tbn;
         Ib[s] = Ibn;
         Db[s] = Dbn;
@@ -171,9 +184,9 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
     const T sigma = (T)ldexp(1.0, -e);
 
     // ---- passes 2 and 3: reverse sweep
-    T Tb[MR], Ib[MR], Db[MR], gd[MR], gg[MR], ge[MR];
+    T Tb[MR], Ib[MR], Db[MR], gd[MR], gdT[MR], gg[MR], ge[MR];
 #pragma unroll
-    for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gg[s] = ge[s] = T(0);
+    for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gdT[s] = gg[s] = ge[s] = T(0);
     T Tb0p = T(0);
     // the 4 steps after state W0 (state after step dw): recompute them into
     // registers, then their adjoints last to first
@@ -191,10 +204,10 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
             const int j = d - l;
             return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
         };
-        adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
-        adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
-        adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
-        adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gg, ge, down);
+        adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
+        adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
+        adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
+        adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
     };
     for (int k = nseg - 1; k >= 0; k--) {
         const int d0 = k * K1;                       // state after step d0
@@ -231,7 +244,7 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
     for (int b = 0; b < 8; b++) vb[b] = T(0);
 #pragma unroll
     for (int s = 0; s < MR; s++) {
-        vd = fmaT(A.real[s], gd[s], vd);
+        vd = fmaT(A.real[s], fmaT(T(-2), gdT[s], gd[s]), vd);
         vg = fmaT(A.real[s], gg[s], vg);
         const T c = A.real[s] * ge[s] * A.wq[s];
 #pragma unroll
@@ -301,7 +314,13 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     {
         const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
         const int nch = (H + 15) >> 4;
-        for (int c = l; c < nch; c += kL) *reinterpret_cast<uint4 *>(lds + kL + 16 * c) = src[c];
+        // one-hot codes with bit 4 set: a read N (its row's bit is 4) matches
+        // every column, as in the reference
+        for (int c = l; c < nch; c += kL) {
+            const uint4 v = src[c];
+            *reinterpret_cast<uint4 *>(lds + kL + 16 * c) =
+                make_uint4(v.x | 0x10101010u, v.y | 0x10101010u, v.z | 0x10101010u, v.w | 0x10101010u);
+        }
     }
     __syncthreads();
     const uint8_t *cp = lds + (kL - 1) - l;
@@ -318,8 +337,7 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         uint32_t code = a.rcode[idx];
         T X = PTab<T>::X(a, qq), Y = PTab<T>::Y(a, qq);
         const bool isN = code == kReadN;
-        Y = isN ? X : Y;
-        code = isN ? 0u : code;
+        Y = isN ? X : Y;  // N in the read: always the match prior (bit 4 of the codes)
         P.mm[s] = real ? PTab<T>::mm(a) : T(0);
         P.g[s] = real ? PTab<T>::g(a) : T(0);
         P.mx[s] = real ? PTab<T>::dl(a) : T(0);
@@ -332,7 +350,6 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         A.real[s] = real ? T(1) : T(0);
         A.wq[s] = (T)a.wq[qq];
         A.bin[s] = a.qbin[qq];
-        A.isN[s] = isN ? 1u : 0u;
         S0.M[s] = T(0);
         S0.I[s] = T(0);
         S0.D[s] = real ? T(0) : initd;
