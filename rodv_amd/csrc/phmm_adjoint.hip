@@ -40,9 +40,6 @@ constexpr int kW = 4;  // steps per register window
 
 template <typename T, int MR>
 struct AdjRows {
-    T real[MR];     // 1 for a read row, 0 for an identity row
-    T wq[MR];       // d e_i / d scale = 10^(-q_i/10)
-    uint32_t bin[MR];
     // transitions of the row below the lane's last row (0 across the wrap)
     T nb_mm, nb_g, nb_mx, nb_xx;
     // yy and my of the row above the lane's first row (step's D recurrence)
@@ -173,7 +170,7 @@ template <typename T, int MR, bool RESCUE>
 __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int p, int H, int lR, int D4,
                                          int nseg, int K1, T *ck, State<T, MR> S,
                                          const Priors<T, MR> &P, const AdjRows<T, MR> &A, T Ssum,
-                                         const uint8_t *cp) {
+                                         const uint8_t *cp, int R, int off, uint64_t ro) {
     const int lane = threadIdx.x;
     const int l = lane;
     constexpr int NV = 3 * MR + 3;
@@ -244,11 +241,18 @@ __device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int 
     for (int b = 0; b < 8; b++) vb[b] = T(0);
 #pragma unroll
     for (int s = 0; s < MR; s++) {
-        vd = fmaT(A.real[s], fmaT(T(-2), gdT[s], gd[s]), vd);
-        vg = fmaT(A.real[s], gg[s], vg);
-        const T c = A.real[s] * ge[s] * A.wq[s];
+        // the row's realness, d e_i / d scale = 10^(-q_i/10) and quality bin,
+        // read again here rather than kept in registers through the sweeps
+        const int i = l * MR + s - off + 1;
+        const bool rl = l < kL - 1 && i >= 1 && i <= R;
+        const uint32_t qq = a.q[ro + (uint64_t)(rl ? i - 1 : 0)] & 127u;
+        const T real = rl ? T(1) : T(0);
+        const uint32_t bin = a.qbin[qq];
+        vd = fmaT(real, fmaT(T(-2), gdT[s], gd[s]), vd);
+        vg = fmaT(real, gg[s], vg);
+        const T c = real * ge[s] * (T)a.wq[qq];
 #pragma unroll
-        for (int b = 0; b < 8; b++) vb[b] += (A.bin[s] == (uint32_t)b) ? c : T(0);
+        for (int b = 0; b < 8; b++) vb[b] += (bin == (uint32_t)b) ? c : T(0);
     }
     double acc[10];
     acc[0] = (double)vd;
@@ -347,9 +351,6 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         P.X[s] = real ? X : T(0);
         P.Y[s] = real ? Y : T(0);
         P.rpos[s] = real ? code : 0u;
-        A.real[s] = real ? T(1) : T(0);
-        A.wq[s] = (T)a.wq[qq];
-        A.bin[s] = a.qbin[qq];
         S0.M[s] = T(0);
         S0.I[s] = T(0);
         S0.D[s] = real ? T(0) : initd;
@@ -405,7 +406,7 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
             }
         }
     } else {
-        adj_reverse<T, MR, RESCUE>(a, lds, p, H, lR, D4, nseg, K1, ck, S, P, A, Ssum, cp);
+        adj_reverse<T, MR, RESCUE>(a, lds, p, H, lR, D4, nseg, K1, ck, S, P, A, Ssum, cp, R, off, ro);
     }
 }
 
