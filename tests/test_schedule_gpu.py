@@ -114,3 +114,40 @@ def test_plain_and_graph_agree_under_routing(engine, oracle_mod):
         np.testing.assert_array_equal(got[2].view(np.uint32), ref[2].view(np.uint32))
         np.testing.assert_array_equal(got[0], ref[0])
     plan.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_shapes_tunings_and_widths(engine, oracle_mod, seed):
+    # regions of random lengths (unrelated reads, N in reads and haplotypes),
+    # random group widths and idle bounds, and a random forced tuning per
+    # length class (8/4/2/1 pairs per wave, strip caps, grids): every shape
+    # gives the oracle's routing and bits
+    import ctypes
+    from rodv_amd.engine import TUNE_CLASSES, Tuning
+    rng = np.random.default_rng(700 + seed)
+    regs = []
+    for _ in range(int(rng.integers(1, 5))):
+        nh, nr = int(rng.integers(1, 9)), int(rng.integers(1, 30))
+        hl = rng.integers(1, int(rng.choice([80, 300, 700, 1500])), nh)
+        rl = rng.integers(1, int(rng.choice([60, 250, 320, 600])), nr)
+        regs.append(synth.region(rng, hl, rl,
+                                 unrelated=[i for i in range(nr) if rng.random() < 0.3],
+                                 n_read=[i for i in range(nr) if rng.random() < 0.05],
+                                 n_hap=[i for i in range(nh) if rng.random() < 0.05],
+                                 qual=int(rng.integers(5, 41))))
+    b = Batch.concat(regs)
+    t = Tuning()
+    t.struct_size = ctypes.sizeof(Tuning)
+    t.num_classes = TUNE_CLASSES
+    for k in range(TUNE_CLASSES):
+        t.cls[k].pairs_per_wave = int(rng.choice([0, 1, 2, 4, 8]))
+        t.cls[k].max_rows_per_lane = int(rng.choice([0, 4, 8, 12, 16]))
+        t.cls[k].waves_per_cu = int(rng.choice([0, 2, 8, 16]))
+    wf, wd = [(16, 8), (8, 4), (4, 2), (4, 3)][seed % 4]
+    engine.set_tuning(t)
+    try:
+        check_schedule(engine, oracle_mod, b, wf=wf, wd=wd,
+                       idle_f=float(rng.choice([0.0, 0.1, 0.5, 1.0])),
+                       idle_d=float(rng.choice([0.1, 0.5, 1.0])))
+    finally:
+        engine.set_tuning(None)
