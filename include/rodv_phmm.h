@@ -106,6 +106,13 @@ typedef struct rodv_phmm_config {
                                      N-free haplotypes (default); 0: select per cell */
     int32_t use_tuning;           /* 1: shapes from rodv_phmm_autotune when the
                                      context has been tuned (default)          */
+    int32_t haplotype_chains;     /* reads of a region that share their
+                                     haplotypes swept across up to 16 of them in
+                                     one item, without a pipeline drain between
+                                     haplotypes: 1 (default) where a length
+                                     bucket still fills the device with such
+                                     items, 2 always, 0 never (one pair per
+                                     slot)                                      */
 } rodv_phmm_config;
 
 /* Counters of the last execution of a plan. */

@@ -22,6 +22,7 @@ namespace rodv {
 void *fwd_f32_kernel(int L, int MR, bool inter);
 void *fwd_f64_rescue_kernel(bool inter);
 void *fwd_tab_kernel_ptr(int L, int MR, bool inter);
+void *fwd_chain_kernel_ptr(int L, int MR, bool inter);
 uint32_t fwd_tab_bytes(int L, int MR);
 int fwd_f64_rescue_mr();
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
@@ -200,9 +201,16 @@ struct Bucket {
     int wpc = 0;
     bool inter = false;
     bool tab = false;        // table-prior kernel (N-free haplotypes, one strip)
+    bool chain = false;      // chained items (fwd_chain_kernel); tab is set too
     uint32_t tab_bytes = 0;  // LDS prior table per group
     void *kern = nullptr;
     std::vector<int32_t> items;  // G entries per item
+    // chained items: item i's haplotypes are chain_hap[chain_off[i] ..
+    // chain_off[i+1]), items holds the G pairs of its first haplotype
+    std::vector<uint32_t> chain_off;
+    std::vector<ChainHap> chain_hap;
+    uint32_t *d_chain_off = nullptr;
+    ChainHap *d_chain_hap = nullptr;
     uint32_t n_items = 0;
     uint32_t max_h = 0;
     uint32_t max_strips = 1;
@@ -212,6 +220,7 @@ struct Bucket {
     float *d_bnd = nullptr;
     uint64_t bnd_elems = 0;
     int grid = 0;
+    int grid_cap = 1;  // persistent grid of a full launch (plain launches are capped by it)
     uint64_t cells = 0;
 };
 
@@ -319,6 +328,8 @@ void free_plan(rodv_phmm_plan *p) {
     for (auto &b : p->buckets) {
         if (b.d_items) (void)hipFree(b.d_items);
         if (b.d_desc) (void)hipFree(b.d_desc);
+        if (b.d_chain_off) (void)hipFree(b.d_chain_off);
+        if (b.d_chain_hap) (void)hipFree(b.d_chain_hap);
         if (b.d_bnd) (void)hipFree(b.d_bnd);
     }
     for (auto *d : p->d_plain_items)
@@ -430,6 +441,144 @@ bool codes_fit(uint32_t H, int L, int MR) {
     if (L == 64) return true;
     const uint64_t per = lds_stride_for(H, L) + 256u + fwd_tab_bytes(L, MR);
     return (64ull / (uint64_t)L) * per <= 60u * 1024u;
+}
+
+// Chained items of one table bucket: reads of a region whose pairs in the
+// bucket cover the same haplotypes (two or more, each at least L + 2 long,
+// so that the kernel's hand-overs never overlap), G reads per item, swept
+// across those haplotypes longest first.  A chain is cut into segments of
+// at most kseg haplotypes, kseg as large as leaves at least `min_items`
+// items (0: no such bound) and at most kChainMaxHaps; a bucket too small
+// for kseg >= 2 keeps single pairs (long items in few waves would leave the
+// device idle).
+// `mem` holds the bucket's pairs; on return only those left as single
+// pairs.  Fills the chain bucket's items (the G pairs of each item's first
+// haplotype), chain_off, chain_hap, max_h and cells; items are ordered by
+// columns, most first.
+constexpr uint64_t kChainMaxHaps = 16;
+constexpr uint64_t kChainItemsPerWave = 16;
+void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, const uint32_t *pr,
+                 const std::vector<uint32_t> &pair_region, const uint32_t *hap_len,
+                 const uint32_t *read_len, const std::vector<uint64_t> &hapc_off,
+                 const std::vector<float> &initd_f, uint64_t min_items) {
+    const size_t G = (size_t)(64 / bk.L);
+    const uint32_t hmin = (uint32_t)bk.L + 2u;
+    // LDS of the chain kernel: G prior tables and two code buffers, <= 60 KiB
+    const int64_t room = ((61440 - (int64_t)G * fwd_tab_bytes(bk.L, bk.MR)) / 2) & ~(int64_t)255;
+    const int64_t hmax = room - (int64_t)lds_stride_for(0, bk.L) - 15;
+    std::sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
+        if (pair_region[x] != pair_region[y]) return pair_region[x] < pair_region[y];
+        if (pr[x] != pr[y]) return pr[x] < pr[y];
+        const uint32_t hx = hap_len[ph[x]], hy = hap_len[ph[y]];
+        if (hx != hy) return hx > hy;
+        return ph[x] < ph[y];
+    });
+    // runs: one read's pairs, [b, e) in mem; sig hashes the haplotype list
+    struct Run {
+        uint32_t b, e;
+        uint64_t sig;
+    };
+    std::vector<Run> runs;
+    std::vector<uint32_t> single;
+    for (size_t i = 0; i < mem.size();) {
+        size_t j = i + 1;
+        while (j < mem.size() && pr[mem[j]] == pr[mem[i]] && pair_region[mem[j]] == pair_region[mem[i]]) j++;
+        bool ok = j - i >= 2;
+        uint64_t sig = 1469598103934665603ull ^ (uint64_t)pair_region[mem[i]];
+        for (size_t k = i; k < j && ok; k++) {
+            ok = hap_len[ph[mem[k]]] >= hmin && (int64_t)hap_len[ph[mem[k]]] <= hmax;
+            sig = (sig ^ ph[mem[k]]) * 1099511628211ull;
+            sig = (sig ^ (uint32_t)(mem[k] - mem[i])) * 1099511628211ull;
+        }
+        if (ok)
+            runs.push_back({(uint32_t)i, (uint32_t)j, sig});
+        else
+            single.insert(single.end(), mem.begin() + i, mem.begin() + j);
+        i = j;
+    }
+    std::sort(runs.begin(), runs.end(), [](const Run &x, const Run &y) {
+        return x.sig != y.sig ? x.sig < y.sig : x.b < y.b;
+    });
+    // same haplotypes in the same order, and the same pair offsets
+    auto same = [&](const Run &x, const Run &y) {
+        if (x.e - x.b != y.e - y.b) return false;
+        for (uint32_t k = 0; k < x.e - x.b; k++) {
+            if (ph[mem[x.b + k]] != ph[mem[y.b + k]]) return false;
+            if (mem[x.b + k] - mem[x.b] != mem[y.b + k] - mem[y.b]) return false;
+        }
+        return true;
+    };
+    // groups of G reads with the same haplotypes: (first run in `chained`)
+    std::vector<size_t> groups;
+    std::vector<Run> chained;
+    uint64_t chain_pairs = 0;
+    for (size_t i = 0; i < runs.size();) {
+        size_t j = i + 1;
+        while (j < runs.size() && runs[j].sig == runs[i].sig) j++;
+        std::vector<Run> grp;
+        for (size_t k = i; k < j; k++) {
+            if (same(runs[k], runs[i]))
+                grp.push_back(runs[k]);
+            else
+                single.insert(single.end(), mem.begin() + runs[k].b, mem.begin() + runs[k].e);
+        }
+        const size_t full = grp.size() / G * G;
+        for (size_t k = 0; k < full; k += G) {
+            groups.push_back(chained.size());
+            chained.insert(chained.end(), grp.begin() + k, grp.begin() + k + G);
+            chain_pairs += (uint64_t)G * (grp[0].e - grp[0].b);
+        }
+        for (size_t k = full; k < grp.size(); k++)
+            single.insert(single.end(), mem.begin() + grp[k].b, mem.begin() + grp[k].e);
+        i = j;
+    }
+    const uint64_t kseg = min_items ? std::min<uint64_t>(kChainMaxHaps, chain_pairs / ((uint64_t)G * min_items))
+                                    : kChainMaxHaps;
+    if (kseg < 2) {
+        for (const Run &r : chained) single.insert(single.end(), mem.begin() + r.b, mem.begin() + r.e);
+        mem.swap(single);
+        return;
+    }
+    // segments: (columns, first run, first and last haplotype), K haplotypes
+    // cut into ceil(K / kseg) near-equal parts
+    struct Seg {
+        uint64_t cols;
+        size_t run;
+        uint32_t k0, k1;
+    };
+    std::vector<Seg> segs;
+    for (size_t gi : groups) {
+        const Run &r0 = chained[gi];
+        const uint32_t K = r0.e - r0.b;
+        const uint32_t ns = (uint32_t)((K + kseg - 1) / kseg);
+        for (uint32_t q = 0; q < ns; q++) {
+            Seg sg{0, gi, (uint32_t)((uint64_t)K * q / ns), (uint32_t)((uint64_t)K * (q + 1) / ns)};
+            for (uint32_t k = sg.k0; k < sg.k1; k++) sg.cols += hap_len[ph[mem[r0.b + k]]];
+            segs.push_back(sg);
+        }
+    }
+    std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cols > y.cols; });
+    for (const Seg &sg : segs) {
+        const Run *r = &chained[sg.run];
+        bk.chain_off.push_back((uint32_t)bk.chain_hap.size());
+        for (uint32_t k = sg.k0; k < sg.k1; k++) {
+            const uint32_t h = ph[mem[r[0].b + k]];
+            ChainHap e{};
+            e.H = hap_len[h];
+            e.initd = initd_f[h];
+            e.hoff = hapc_off[h];
+            e.pdelta = (int32_t)(mem[r[0].b + k] - mem[r[0].b + sg.k0]);
+            bk.chain_hap.push_back(e);
+            bk.max_h = std::max(bk.max_h, e.H);
+        }
+        for (size_t g = 0; g < G; g++) {
+            bk.items.push_back((int32_t)mem[r[g].b + sg.k0]);
+            for (uint32_t k = r[g].b + sg.k0; k < r[g].b + sg.k1; k++)
+                bk.cells += (uint64_t)read_len[pr[mem[k]]] * hap_len[ph[mem[k]]];
+        }
+    }
+    if (!bk.chain_off.empty()) bk.chain_off.push_back((uint32_t)bk.chain_hap.size());
+    mem.swap(single);
 }
 
 // Strips of (L-1)*MR read rows (select-prior kernel), after `off` < MR
@@ -580,6 +729,7 @@ void rodv_phmm_config_default(rodv_phmm_config *cfg) {
     cfg->extended_range = 0;
     cfg->prior_table = 1;
     cfg->use_tuning = 1;
+    cfg->haplotype_chains = 1;
 }
 
 int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
@@ -658,6 +808,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (cfg.short_lanes != 16 && cfg.short_lanes != 64) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.waves_per_cu < 0 || cfg.waves_per_cu > 32) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.simd_width_float > 64 || cfg.simd_width_double > 64) return RODV_PHMM_ERR_INVALID_ARG;
+    if (cfg.haplotype_chains < 0 || cfg.haplotype_chains > 2) return RODV_PHMM_ERR_INVALID_ARG;
     if (!(cfg.max_idle_ratio_float == cfg.max_idle_ratio_float) ||
         !(cfg.max_idle_ratio_double == cfg.max_idle_ratio_double))
         return RODV_PHMM_ERR_INVALID_ARG;
@@ -813,16 +964,57 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         }
         members[bucket_of_key[key]].push_back((uint32_t)k);
     }
+    // the plain schedule keeps every pair of a bucket in one-pair slots
+    std::vector<std::vector<uint32_t>> plain_mem = members;
+    // ---- chained items: a bucket of their own next to each table bucket
+    if (cfg.haplotype_chains) {
+        const size_t nb0 = p->buckets.size();
+        for (size_t bi = 0; bi < nb0; bi++) {
+            if (!p->buckets[bi].tab) continue;
+            void *ck = fwd_chain_kernel_ptr(p->buckets[bi].L, p->buckets[bi].MR, p->buckets[bi].inter);
+            if (!ck) continue;
+            Bucket cb;
+            cb.L = p->buckets[bi].L;
+            cb.MR = p->buckets[bi].MR;
+            cb.wpc = p->buckets[bi].wpc;
+            cb.inter = p->buckets[bi].inter;
+            cb.tab = true;
+            cb.chain = true;
+            cb.kern = ck;
+            // at least kChainItemsPerWave items per wave of a full grid
+            const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu
+                                                   : (cb.wpc > 0 ? cb.wpc : kDefaultWavesPerCU);
+            form_chains(members[bi], cb, ph.data(), pr.data(), pair_region, b->hap_lengths,
+                        b->read_lengths, hapc_off, initd_f,
+                        cfg.haplotype_chains == 2 ? 0 : kChainItemsPerWave * (uint64_t)ctx->num_cus * (uint64_t)wpc);
+            if (cb.chain_off.empty()) continue;
+            p->buckets.push_back(std::move(cb));
+            members.emplace_back();
+            plain_mem.emplace_back();
+        }
+    }
 
     // ---- items: length-sorted packing of G pairs per wave
     for (size_t bi = 0; bi < p->buckets.size(); bi++) {
         Bucket &bk = p->buckets[bi];
+        const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu
+                                               : (bk.wpc > 0 ? bk.wpc : kDefaultWavesPerCU);
+        bk.grid_cap = (int)std::max<uint64_t>(1, (uint64_t)ctx->num_cus * wpc);
+        if (bk.chain) {
+            bk.n_items = (uint32_t)(bk.chain_off.size() - 1);
+            bk.lds_stride = (lds_stride_for(bk.max_h, bk.L) + 255u) & ~255u;
+            bk.tab_bytes = fwd_tab_bytes(bk.L, bk.MR);
+            bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus * wpc));
+            continue;
+        }
         bk.kern = bk.tab ? fwd_tab_kernel_ptr(bk.L, bk.MR, bk.inter)
                          : fwd_f32_kernel(bk.L, bk.MR, bk.inter);
         if (!bk.kern) {
             free_plan(p);
             return RODV_PHMM_ERR_UNSUPPORTED;
         }
+        // LDS for every pair the bucket's kernel may take, plain launches included
+        for (uint32_t k : plain_mem[bi]) bk.max_h = std::max(bk.max_h, b->hap_lengths[ph[k]]);
         auto &mem = members[bi];
         std::stable_sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
             uint32_t hx = b->hap_lengths[ph[x]], hy = b->hap_lengths[ph[y]];
@@ -848,8 +1040,6 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         } else {
             bk.lds_stride = lds_stride_for(bk.max_h, bk.L);
         }
-        const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu
-                                               : (bk.wpc > 0 ? bk.wpc : kDefaultWavesPerCU);
         bk.grid = (int)std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus * wpc);
         if (bk.grid < 1) bk.grid = 1;
         if (bk.max_strips > 1) {
@@ -864,13 +1054,15 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     });
     {
         std::vector<Bucket> sorted;
-        std::vector<std::vector<uint32_t>> smem;
+        std::vector<std::vector<uint32_t>> smem, spm;
         for (size_t k : order) {
             sorted.push_back(std::move(p->buckets[k]));
             smem.push_back(std::move(members[k]));
+            spm.push_back(std::move(plain_mem[k]));
         }
         p->buckets = std::move(sorted);
         members = std::move(smem);
+        plain_mem = std::move(spm);
     }
 
     // ---- plain schedule: per region, per bucket, region-ordered items
@@ -878,7 +1070,7 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         for (size_t bi = 0; bi < p->buckets.size(); bi++) {
             Bucket &bk = p->buckets[bi];
             const int G = 64 / bk.L;
-            auto mem = members[bi];
+            auto mem = plain_mem[bi];
             std::stable_sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
                 if (pair_region[x] != pair_region[y]) return pair_region[x] < pair_region[y];
                 uint32_t hx = b->hap_lengths[ph[x]], hy = b->hap_lengths[ph[y]];
@@ -948,6 +1140,10 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (!rc) rc = dev_alloc(&p->d_rescue, n_pairs);
     for (auto &bk : p->buckets) {
         if (!rc) rc = upload(&bk.d_items, bk.items.data(), bk.items.size());
+        if (!rc && bk.chain) {
+            rc = upload(&bk.d_chain_off, bk.chain_off.data(), bk.chain_off.size());
+            if (!rc) rc = upload(&bk.d_chain_hap, bk.chain_hap.data(), bk.chain_hap.size());
+        }
         if (!rc && bk.bnd_elems) {
             rc = dev_alloc(&bk.d_bnd, (size_t)bk.grid * 2 * bk.bnd_elems);
             if (!rc && hipMemset(bk.d_bnd, 0, (size_t)bk.grid * 2 * bk.bnd_elems * sizeof(float)) != hipSuccess)
@@ -1051,18 +1247,23 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
             for (int k = 0; k + 1 < ns; k++) HIPCHK(hipStreamWaitEvent(p->ctx->side[k], p->ctx->fork_ev, 0));
             for (size_t bi = 0; bi < p->buckets.size(); bi++) {
                 Bucket &bk = p->buckets[bi];
+                if (bk.n_items == 0) continue;  // every pair of the bucket chained
                 const int si = (int)(bi % (size_t)ns);
                 const hipStream_t bs = si == 0 ? st : p->ctx->side[si - 1];
                 FwdArgs a = base_args(p);
                 a.items = bk.d_items;
                 a.desc = bk.d_desc;
+                a.chain_off = bk.d_chain_off;
+                a.chain_hap = bk.d_chain_hap;
                 a.n_items = bk.n_items;
                 a.counter = p->d_counters + 2 + bi;
                 a.lds_stride = bk.lds_stride;
                 a.bnd = bk.d_bnd;
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
-                const size_t lds = (size_t)(64 / bk.L) * (bk.lds_stride + bk.tab_bytes);
+                // chained items: G prior tables and two code buffers per wave
+                const size_t lds = bk.chain ? (size_t)(64 / bk.L) * bk.tab_bytes + 2u * bk.lds_stride
+                                            : (size_t)(64 / bk.L) * (bk.lds_stride + bk.tab_bytes);
                 HIPCHK(launch_fwd(bk.kern, bk.grid, lds, bs, a));
                 launches++;
             }
@@ -1085,7 +1286,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
                 const size_t lds = (size_t)G * (bk.lds_stride + bk.tab_bytes);
-                HIPCHK(launch_fwd(bk.kern, std::min(pl.grid, bk.grid), lds, st, a));
+                HIPCHK(launch_fwd(bk.kern, std::min(pl.grid, bk.grid_cap), lds, st, a));
                 launches++;
             }
         }

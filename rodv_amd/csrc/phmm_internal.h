@@ -39,6 +39,18 @@ struct alignas(16) ItemDesc {
     uint64_t hoff;  // offset of the haplotype's codes
 };
 
+// One haplotype of a chained item (fwd_chain_kernel): the same for every
+// group of the item, so the wave keeps it in scalar registers.  A group's
+// pair for this haplotype is its pair for the chain's first haplotype plus
+// pdelta (pairs are numbered haplotype-major within a region).
+struct alignas(16) ChainHap {
+    uint32_t H;
+    float initd;    // 2^120 / (float)H
+    uint64_t hoff;  // offset of the haplotype's codes
+    int32_t pdelta;
+    uint32_t pad[3];
+};
+
 // Arguments of every forward launch (passed by value).
 struct FwdArgs {
     const DevTables *tab;
@@ -62,6 +74,10 @@ struct FwdArgs {
     // a list of pair indices whose length is *n_items_dev
     const int32_t *items;
     const ItemDesc *desc;  // table kernel: descriptors of the item slots
+    // chained items: item i's haplotypes are chain_hap[chain_off[i] ..
+    // chain_off[i+1]); desc holds the G slots of each item's first haplotype
+    const uint32_t *chain_off;
+    const ChainHap *chain_hap;
     uint32_t n_items;
     const uint32_t *n_items_dev;
     uint32_t *counter;  // persistent work counter (zeroed by the reset kernel)

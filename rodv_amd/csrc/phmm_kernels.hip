@@ -621,6 +621,259 @@ __global__ __launch_bounds__(64) void fwd_tab_kernel(FwdArgs a) {
     }
 }
 
+// ------------------------------------------------- chained table items
+// A chained item is G reads of one region (one per group of L lanes, rows
+// laid out as in process_item_tab) scored against the same K haplotypes of
+// that region in turn.  The read's priors and LDS table are built once per
+// item, and each lane runs from haplotype k's last column straight into
+// haplotype k+1's first, where its state is reset to the column-0 boundary
+// (the state an item starts from, with the new haplotype's K/H): the L-1
+// fill and drain steps and the read's set-up are paid once per item instead
+// of once per pair.  Lane l is at global column dd - l at step dd;
+// haplotype k occupies global columns (S_k, S_k + H_k].  The groups share
+// the haplotype codes, staged in two LDS buffers: haplotype k+1's while k is
+// swept.  Two hand-overs never overlap when H_k >= L + 2 for k >= 1, which
+// the plan guarantees.
+
+// One haplotype's codes (block 0..3 per column, column j at dst[j - 1]),
+// staged by the whole wave: 16 columns per lane and pass.
+__device__ __forceinline__ void stage_codes(const FwdArgs &a, uint64_t hoff, int H, uint8_t *dst) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + hoff);
+    const int nch = (H + 15) >> 4;
+    for (int c = threadIdx.x; c < nch; c += 64) {
+        const uint4 v = src[c];
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        uint32_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)  // one-hot 1,2,4,8 -> 0..3 per byte
+            o[k] = ((w[k] >> 1) & 0x7f7f7f7fu) - ((w[k] >> 3) & 0x1f1f1f1fu);
+        *reinterpret_cast<uint4 *>(dst + 16 * c) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+}
+
+// The FP32 sum of a pair and its rescue flag (pairhmm_api.cpp:17).
+__device__ __forceinline__ void put_raw32(const FwdArgs &a, int p, float Ssum) {
+    a.raw32[p] = Ssum;
+    uint8_t fl = 0;
+    if (Ssum < 1e-28f) {
+        fl = 0x01;
+        const uint32_t k = atomicAdd(a.rescue_count, 1u);
+        a.rescue_list[k] = (uint32_t)p;
+    }
+    a.flags[p] = fl;
+}
+
+template <int L, int MR, bool INTER>
+__device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
+    constexpr int G = 64 / L;
+    constexpr int MRT = TabLayout<MR>::MRT;
+    constexpr uint32_t kBlk = (uint32_t)L * MRT * 4;
+    constexpr int P0 = L < 16 ? 16 : L;  // buffer byte of column 1 (16-byte aligned)
+    const int lane = threadIdx.x;
+    const int g = lane / L;
+    const int l = lane % L;
+    const uint32_t c0 = __builtin_amdgcn_readfirstlane(a.chain_off[it]);
+    const uint32_t K = __builtin_amdgcn_readfirstlane(a.chain_off[it + 1]) - c0;
+    const ItemDesc d = a.desc[(uint64_t)it * G + g];
+    const bool valid = d.p >= 0;
+    const int R = (int)d.R;
+    const int off = valid ? (MR - R % MR) % MR : 0;
+    const int lR = valid ? (R - 1 + off) / MR : 0;
+    uint8_t *tab = lds + (uint32_t)g * 4u * kBlk;
+    uint8_t *buf = lds + (uint32_t)G * 4u * kBlk;  // two code buffers of lds_stride bytes
+
+    RowRaw raw[MR];
+    bool realv[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int i = l * MR + s - off + 1;
+        realv[s] = valid && i >= 1 && i <= R;
+        raw[s] = load_raw(a, d.ro + (uint64_t)(realv[s] ? i - 1 : 0));
+    }
+    const int H0 = __builtin_amdgcn_readfirstlane((int)a.chain_hap[c0].H);
+    const uint32_t hoff0_lo = __builtin_amdgcn_readfirstlane((uint32_t)a.chain_hap[c0].hoff);
+    const uint32_t hoff0_hi = __builtin_amdgcn_readfirstlane((uint32_t)(a.chain_hap[c0].hoff >> 32));
+    stage_codes(a, ((uint64_t)hoff0_hi << 32) | hoff0_lo, H0, buf + P0);
+
+    Priors<float, MR> P;
+    float X[MR], Y[MR];
+    uint32_t rc[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        row_priors<float, MR, INTER>(a, P, s, realv[s], raw[s]);
+        X[s] = P.X[s];
+        Y[s] = P.Y[s];
+        rc[s] = P.rpos[s];
+    }
+    float *tl = reinterpret_cast<float *>(tab + (uint32_t)l * 16);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int s = 0; s < MR; s++)
+            tl[c * (kBlk / 4) + (s >> 2) * (L * 4) + (s & 3)] = (rc[s] == (uint32_t)c) ? X[s] : Y[s];
+    }
+    wave_lds_sync();
+    uint32_t next = 0;
+    if (lane == 0) next = atomicAdd(a.counter, 1u);
+
+    State<float, MR> S;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        S.M[s] = 0.0f;
+        S.I[s] = 0.0f;
+        S.D[s] = realv[s] ? 0.0f : d.initd;
+    }
+    const LaneRot<L> rot;
+    Carry C;
+    C.dM = 0.0f;
+    C.dI = 0.0f;
+    C.dD = rot(S.D[MR - 1]);
+    C.uM = 0.0f;
+    C.uI = 0.0f;
+    C.yyA = rot(P.yy[MR - 1]);
+    C.myA = rot(P.my[MR - 1]);
+
+    const uint8_t *cl = buf + (P0 - 1) - l;  // code of step x: cl[x]
+    const uint8_t *tlane = tab + (uint32_t)l * 16;
+    const int p0 = d.p;
+    int p = p0;  // this group's pair for the current haplotype
+    float sm = 0.0f, si = 0.0f;
+    int dd = 1;
+    int Sk = 0;  // columns of the haplotypes before the current one
+    int H = H0;
+    float pa[MR];
+    load_priors<L, MR>(pa, tlane + ld_code(cl + dd) * kBlk);
+    uint32_t c1 = ld_code(cl + dd + 1);
+    for (uint32_t k = 0; k + 1 < K; k++) {
+        const ChainHap *hn = a.chain_hap + c0 + k + 1;
+        const int Hn = __builtin_amdgcn_readfirstlane((int)hn->H);
+        const float initdn = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(hn->initd)));
+        const int pdn = __builtin_amdgcn_readfirstlane(hn->pdelta);
+        const uint32_t hlo = __builtin_amdgcn_readfirstlane((uint32_t)hn->hoff);
+        const uint32_t hhi = __builtin_amdgcn_readfirstlane((uint32_t)(hn->hoff >> 32));
+        const int Sn = Sk + H;
+        // the other buffer was last read in the previous hand-over
+        uint8_t *nb = buf + ((k + 1) & 1) * a.lds_stride;
+        stage_codes(a, ((uint64_t)hhi << 32) | hlo, Hn, nb + P0);
+        const uint8_t *cln = nb + (P0 - 1) - l - Sn;
+        // every lane, and every code it prefetches, inside haplotype k
+        for (; dd + kU - 1 <= Sn - 2; dd += kU) {
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint32_t c2 = ld_code(cl + dd + u + 2);
+                float pb[MR];
+                step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+                sm += S.M[MR - 1];
+                si += S.I[MR - 1];
+#pragma unroll
+                for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+                c1 = c2;
+            }
+        }
+        for (; dd <= Sn - 2; dd++) {
+            const uint32_t c2 = ld_code(cl + dd + 2);
+            float pb[MR];
+            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+#pragma unroll
+            for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+            c1 = c2;
+        }
+        // hand-over: lane l reads the new codes from step Sn - 1 + l on and
+        // enters haplotype k+1 at step Sn + 1 + l
+        float dres[MR];
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int i = l * MR + s - off + 1;
+            dres[s] = (valid && i >= 1 && i <= R) ? 0.0f : initdn;
+        }
+        const float dDres = rot(dres[MR - 1]);
+        for (; dd <= Sn + L; dd++) {
+            if (dd + 1 - l == Sn) cl = cln;
+            const uint32_t c2 = ld_code(cl + dd + 2);
+            const bool cross = dd - l == Sn + 1;
+            if (cross && valid && l == lR) put_raw32(a, p, sm + si);  // pairhmm_impl.cpp:434-437
+            if (cross) {
+                sm = 0.0f;
+                si = 0.0f;
+#pragma unroll
+                for (int s = 0; s < MR; s++) {
+                    S.M[s] = 0.0f;
+                    S.I[s] = 0.0f;
+                    S.D[s] = dres[s];
+                }
+                C.dM = 0.0f;
+                C.dI = 0.0f;
+                C.dD = dDres;
+            }
+            float pb[MR];
+            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+#pragma unroll
+            for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+            c1 = c2;
+        }
+        Sk = Sn;
+        H = Hn;
+        p = p0 + pdn;
+    }
+    // the last haplotype ends like a single pair
+    const int key = valid ? Sk + H + lR : 0x7fffffff;
+    const int keymax = valid ? Sk + H + lR : -1;
+    const int Dmain = wave_min4(key, G);
+    const int Dend = wave_max4(keymax, G);
+    for (; dd + kU - 1 <= Dmain; dd += kU) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const uint32_t c2 = ld_code(cl + dd + u + 2);
+            float pb[MR];
+            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+#pragma unroll
+            for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+            c1 = c2;
+        }
+    }
+    for (; dd <= Dend; dd++) {
+        const uint32_t c2 = ld_code(cl + dd + 2);
+        float pb[MR];
+        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+        if (dd - l - Sk <= H) {
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+        c1 = c2;
+    }
+    if (valid && l == lR) put_raw32(a, p, sm + si);
+    // every lane has finished its LDS reads before the next item rewrites
+    // the table and the first code buffer
+    wave_lds_sync();
+    return __builtin_amdgcn_readfirstlane(next);
+}
+
+// Persistent, like fwd_tab_kernel, over chained items.  LDS: the G prior
+// tables, then the two code buffers, which start all zero (prefetches past
+// a haplotype read codes left there, always a valid block 0..3).
+template <int L, int MR, bool INTER>
+__global__ __launch_bounds__(64) void fwd_chain_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int G = 64 / L;
+    constexpr uint32_t tabs = (uint32_t)G * 4u * (uint32_t)L * TabLayout<MR>::MRT * 4u;
+    for (uint32_t k = threadIdx.x * 16u; k < 2u * a.lds_stride; k += 64u * 16u)
+        *reinterpret_cast<uint4 *>(lds + tabs + k) = make_uint4(0, 0, 0, 0);
+    wave_lds_sync();
+    const uint32_t n = a.n_items;
+    uint32_t it = 0;
+    if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+    it = __builtin_amdgcn_readfirstlane(it);
+    for (uint32_t iter = 0; iter < n && it < n; iter++) it = process_chain<L, MR, INTER>(a, lds, it);
+}
+
 template <typename T, int L, int MR, bool INTER, bool RESCUE>
 __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
@@ -798,6 +1051,39 @@ void *fwd_f32_kernel(int L, int MR, bool inter) {
 void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
 #define TK(l, m) \
     case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<l, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<l, m, false>);
+    if (L == 8) {
+        switch (MR) {
+            TK(8, 6) TK(8, 7) TK(8, 8) TK(8, 9) TK(8, 10) TK(8, 11) TK(8, 12)
+        default: return nullptr;
+        }
+    }
+    if (L == 16) {
+        switch (MR) {
+            TK(16, 1) TK(16, 2) TK(16, 3) TK(16, 4) TK(16, 5) TK(16, 6) TK(16, 7) TK(16, 8)
+            TK(16, 9) TK(16, 10) TK(16, 11) TK(16, 12) TK(16, 13) TK(16, 14) TK(16, 15) TK(16, 16)
+        default: return nullptr;
+        }
+    }
+    if (L == 32) {
+        switch (MR) {
+            TK(32, 2) TK(32, 3) TK(32, 4) TK(32, 5) TK(32, 6) TK(32, 7) TK(32, 8) TK(32, 9) TK(32, 10)
+        default: return nullptr;
+        }
+    }
+    if (L == 64) {
+        switch (MR) {
+            TK(64, 4) TK(64, 8) TK(64, 12) TK(64, 16)
+        default: return nullptr;
+        }
+    }
+#undef TK
+    return nullptr;
+}
+
+// Chain kernels exist for every table-kernel shape.
+void *fwd_chain_kernel_ptr(int L, int MR, bool inter) {
+#define TK(l, m) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<l, m, true>) : reinterpret_cast<void *>(&fwd_chain_kernel<l, m, false>);
     if (L == 8) {
         switch (MR) {
             TK(8, 6) TK(8, 7) TK(8, 8) TK(8, 9) TK(8, 10) TK(8, 11) TK(8, 12)

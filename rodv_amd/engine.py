@@ -62,7 +62,8 @@ class Config(C.Structure):
         ("max_idle_ratio_float", C.c_double), ("max_idle_ratio_double", C.c_double),
         ("simd_width_float", C.c_uint32), ("simd_width_double", C.c_uint32),
         ("use_graph", C.c_int32), ("waves_per_cu", C.c_int32), ("short_lanes", C.c_int32),
-        ("extended_range", C.c_int32), ("prior_table", C.c_int32), ("use_tuning", C.c_int32)]
+        ("extended_range", C.c_int32), ("prior_table", C.c_int32), ("use_tuning", C.c_int32),
+        ("haplotype_chains", C.c_int32)]
 
 
 TUNE_CLASSES = 18

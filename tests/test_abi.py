@@ -17,10 +17,10 @@ SIZES_C = r"""
 #include <stdio.h>
 #include "rodv_phmm.h"
 int main(void) {
-    printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rodv_phmm_config), sizeof(rodv_phmm_batch),
+    printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rodv_phmm_config), sizeof(rodv_phmm_batch),
            sizeof(rodv_phmm_stats), sizeof(rodv_phmm_tuning), sizeof(rodv_phmm_tune_entry),
            sizeof(rodv_phmm_params), offsetof(rodv_phmm_config, use_tuning),
-           offsetof(rodv_phmm_tuning, cls));
+           offsetof(rodv_phmm_config, haplotype_chains), offsetof(rodv_phmm_tuning, cls));
     return 0;
 }
 """
@@ -60,7 +60,8 @@ def test_struct_layouts_match_ctypes(tmp_path):
                                            text=True).stdout.split()]
     assert vals == [C.sizeof(E.Config), C.sizeof(E.CBatch), C.sizeof(E.Stats),
                     C.sizeof(E.Tuning), C.sizeof(E.TuneEntry), C.sizeof(E.Params),
-                    E.Config.use_tuning.offset, E.Tuning.cls.offset]
+                    E.Config.use_tuning.offset, E.Config.haplotype_chains.offset,
+                    E.Tuning.cls.offset]
 
 
 def test_schedule_shim_compiles_with_reference_signature(tmp_path):

@@ -1,0 +1,103 @@
+"""Chained items (reads of a region swept across up to 16 of their
+haplotypes in one item, fwd_chain_kernel): bit-identical to one pair per
+slot, and to the oracle, across the hand-over's edge cases.  The tests force
+chains (haplotype_chains=2); the default forms them only in buckets large
+enough to keep the device full, which the last test checks."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from rodv_amd import synth
+from rodv_amd.batch import Batch
+from rodv_amd.engine import TUNE_CLASSES, Tuning, default_config
+
+from test_forward_gpu import check_against_oracle
+from test_schedule_gpu import check_schedule
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(engine, batch, chains, **cfg):
+    plan = engine.plan(batch, default_config(haplotype_chains=chains, **cfg))
+    plan.execute()
+    got = plan.read(raw=True)
+    nb = plan.stats()["num_buckets"]
+    plan.close()
+    return got, nb
+
+
+def _same(engine, batch, expect_chains=True, mode=2, **cfg):
+    on, nb_on = _run(engine, batch, mode, **cfg)
+    off, nb_off = _run(engine, batch, 0, **cfg)
+    if expect_chains:
+        assert nb_on > nb_off, "no chained items formed"
+    np.testing.assert_array_equal(on[2].view(np.uint32), off[2].view(np.uint32))
+    np.testing.assert_array_equal(on[1], off[1])
+    np.testing.assert_array_equal(on[0], off[0])
+
+
+def _edge_batch(rng, L=16):
+    # haplotypes at the hand-over bound (L + 2) and just below it (L + 1,
+    # which leaves its reads unchained), reads whose length fills the lanes
+    # exactly (the last lane carries the row-0 boundary) or leaves an
+    # identity slot in lane 0, G reads and G + 1, unrelated reads (FP64
+    # rescue inside a chain), nine reads of one length (one chain of eight)
+    regs = [
+        synth.region(rng, [L + 2, L + 2, 300], [16 * 9, 16 * 9 - 1, 100, 100, 150]),
+        synth.region(rng, [L + 1, 200, 250], [120, 120, 120, 120]),
+        synth.region(rng, [400, 390, 380, 60, 50], [240, 241, 250, 200, 200, 233, 90, 91, 92],
+                     unrelated=[0, 4, 6]),
+        synth.region(rng, [35] * 7, [5, 6, 7, 8, 30, 31, 32, 33]),
+        synth.region(rng, [500, 450], list(rng.integers(50, 250, 23))),
+        synth.region(rng, [120, 110, 100], [70] * 9),
+    ]
+    return Batch.concat(regs)
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_chains_equal_single_pairs(engine, semantics):
+    rng = np.random.default_rng(41)
+    _same(engine, _edge_batch(rng), semantics=semantics)
+    _same(engine, synth.config(2, scale=0.003), semantics=semantics)
+    _same(engine, synth.config(1), semantics=semantics)
+
+
+def test_chains_against_oracle(engine, oracle_mod):
+    rng = np.random.default_rng(42)
+    check_against_oracle(oracle_mod, engine, _edge_batch(rng), haplotype_chains=2)
+    check_schedule(engine, oracle_mod, _edge_batch(np.random.default_rng(43)), haplotype_chains=2)
+
+
+@pytest.mark.parametrize("ppw", [8, 4, 2, 1])
+def test_chains_every_group_width(engine, oracle_mod, ppw):
+    # 8, 4, 2 and 1 pairs per wave (8-, 16-, 32- and 64-lane chains)
+    t = Tuning()
+    t.struct_size = ctypes.sizeof(Tuning)
+    t.num_classes = TUNE_CLASSES
+    for k in range(TUNE_CLASSES):
+        t.cls[k].pairs_per_wave = ppw
+    engine.set_tuning(t)
+    try:
+        rng = np.random.default_rng(50 + ppw)
+        L = 64 // ppw
+        b = _edge_batch(rng, L=L)
+        # per-pair semantics keep each region's pairs together, so the nine
+        # equal reads always chain; schedule routing may split them
+        _same(engine, b, semantics=0)
+        _same(engine, b, expect_chains=False, semantics=1)
+        check_against_oracle(oracle_mod, engine, b, haplotype_chains=2)
+        check_schedule(engine, oracle_mod, b, haplotype_chains=2)
+    finally:
+        engine.set_tuning(None)
+
+
+def test_default_chains_large_batch(engine):
+    # the uneven batched workload at 30 %: large enough for chained items in
+    # the default mode, which must then give the one-pair-per-slot bits
+    b = synth.config(2, scale=0.3)
+    _same(engine, b, mode=1, semantics=1)
+    # a small batch keeps one pair per slot by default
+    _, nb1 = _run(engine, synth.config(1), 1, semantics=1)
+    _, nb0 = _run(engine, synth.config(1), 0, semantics=1)
+    assert nb1 == nb0

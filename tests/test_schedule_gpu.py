@@ -17,10 +17,10 @@ ROUTE_INTER32, ROUTE_INTER64 = 1, 2
 
 
 def check_schedule(engine, oracle_mod, batch, wf=16, wd=8, idle_f=0.1, idle_d=0.1,
-                   use_double=False, mode="batched"):
+                   use_double=False, mode="batched", **extra):
     cfg = default_config(semantics=1, simd_width_float=wf, simd_width_double=wd,
                          max_idle_ratio_float=idle_f, max_idle_ratio_double=idle_d,
-                         use_double=int(use_double), use_graph=1)
+                         use_double=int(use_double), use_graph=1, **extra)
     plan = engine.plan(batch, cfg)
     if mode == "plain":
         plan.execute_plain()
@@ -120,8 +120,8 @@ def test_plain_and_graph_agree_under_routing(engine, oracle_mod):
 def test_random_shapes_tunings_and_widths(engine, oracle_mod, seed):
     # regions of random lengths (unrelated reads, N in reads and haplotypes),
     # random group widths and idle bounds, and a random forced tuning per
-    # length class (8/4/2/1 pairs per wave, strip caps, grids): every shape
-    # gives the oracle's routing and bits
+    # length class (8/4/2/1 pairs per wave, strip caps, grids), chained
+    # items forced or not: every shape gives the oracle's routing and bits
     import ctypes
     from rodv_amd.engine import TUNE_CLASSES, Tuning
     rng = np.random.default_rng(700 + seed)
@@ -148,6 +148,7 @@ def test_random_shapes_tunings_and_widths(engine, oracle_mod, seed):
     try:
         check_schedule(engine, oracle_mod, b, wf=wf, wd=wd,
                        idle_f=float(rng.choice([0.0, 0.1, 0.5, 1.0])),
-                       idle_d=float(rng.choice([0.1, 0.5, 1.0])))
+                       idle_d=float(rng.choice([0.1, 0.5, 1.0])),
+                       haplotype_chains=int(rng.choice([1, 2])))
     finally:
         engine.set_tuning(None)
