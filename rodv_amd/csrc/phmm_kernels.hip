@@ -789,7 +789,9 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
             dres[s] = (valid && i >= 1 && i <= R) ? 0.0f : initdn;
         }
         const float dDres = rot(dres[MR - 1]);
-        for (; dd <= Sn + L; dd++) {
+        // steps Sn - 1 .. Sn + L, two at a time with the prior registers
+        // swapping roles (no copies)
+        auto hand = [&](const float (&pin)[MR], float (&pout)[MR]) {
             if (dd + 1 - l == Sn) cl = cln;
             const uint32_t c2 = ld_code(cl + dd + 2);
             const bool cross = dd - l == Sn + 1;
@@ -807,13 +809,26 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
                 C.dI = 0.0f;
                 C.dD = dDres;
             }
-            float pb[MR];
-            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            step_tab<L, MR, INTER>(S, C, P, pin, rot, pout, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
-#pragma unroll
-            for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
             c1 = c2;
+            dd++;
+        };
+        static_assert((L + 2) % 2 == 0, "hand-over steps in pairs");
+        if constexpr (MR < 16) {
+            for (int u = 0; u < (L + 2) / 2; u++) {
+                float pb[MR];
+                hand(pa, pb);
+                hand(pb, pa);
+            }
+        } else {  // in pairs the 16-row shapes would need more than 256 VGPRs
+            for (int u = 0; u < L + 2; u++) {
+                float pb[MR];
+                hand(pa, pb);
+#pragma unroll
+                for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+            }
         }
         Sk = Sn;
         H = Hn;
