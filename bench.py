@@ -31,8 +31,8 @@ ADJ_FLOP_PER_CELL = 46
 # --pmc runs of this script on the same workload: FETCH_SIZE x 2.000 +
 # WRITE_SIZE x 1.000, the factors a 1 GiB device copy in the same run gives
 # (--copy-calibration-mib 1024).  Keyed by (config, scale, semantics,
-# autotuned).
-MEASURED_TRAFFIC = {(2, 1.0, "schedule", True): 2.239e10}
+# autotuned, haplotype chains).
+MEASURED_TRAFFIC = {(2, 1.0, "schedule", True, 1): 4.333e9, (2, 1.0, "schedule", True, 0): 2.239e10}
 
 
 def log(msg):
@@ -155,6 +155,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint", type=int, default=1)
     ap.add_argument("--prior-table", type=int, default=1)
+    ap.add_argument("--haplotype-chains", type=int, choices=(0, 1, 2), default=1,
+                    help="chained items: 1 where a length bucket is large enough (default), "
+                         "2 always, 0 one pair per slot")
     ap.add_argument("--autotune", type=int, default=1,
                     help="tune pairs per wavefront / strip width / waves per CU on a 4%% sample "
                          "of the workload's distribution before timing")
@@ -201,7 +204,8 @@ def main():
                 f"{c['gcups']} GCUPS (built-in {c['default_gcups']})")
     t0 = time.time()
     plan = eng.plan(batch, default_config(use_graph=1, prior_table=args.prior_table,
-                                         waves_per_cu=args.waves_per_cu, semantics=sem))
+                                         waves_per_cu=args.waves_per_cu, semantics=sem,
+                                         haplotype_chains=args.haplotype_chains))
     log(f"plan + upload {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -282,7 +286,8 @@ def main():
         ceiling["forward_cells_per_s"] = round(cps, 1)
         ceiling["frac_of_cell_mix_ceiling"] = round(cps / ceiling["cells_per_s"], 4)
         log(f"VALU ceiling: {ceiling}")
-    traffic = MEASURED_TRAFFIC.get((args.config, args.scale, args.semantics, bool(args.autotune))) \
+    traffic = MEASURED_TRAFFIC.get((args.config, args.scale, args.semantics, bool(args.autotune),
+                                    args.haplotype_chains)) \
         if (args.waves_per_cu == 0 and args.prior_table) else None
     res = {
         "metric": "GCUPS fwd/adjoint, batched vs plain, 1 MI355X, |dlog10L|<=1e-4",
@@ -312,7 +317,9 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "kernel": "fwd_tab_kernel<float> (all length buckets, one FP32 pass)"
+                     "kernel": ("fwd_chain_kernel + fwd_tab_kernel <float> (all length buckets, "
+                                "one FP32 pass)" if args.haplotype_chains else
+                                "fwd_tab_kernel<float> (all length buckets, one FP32 pass)")
                      if args.prior_table else "fwd_kernel<float> (all length buckets)",
                      "kernel_ms": round(fp32_ms, 3),
                      "valu_issue_ceiling": ceiling},
