@@ -98,7 +98,7 @@ def cpu_baseline(batch, seconds=12.0, seed=7, all_core_seconds=6.0):
                      f"{cells:.3e} cells in {dt:.1f} s; host {_cpu_model()}"}
     if ref is not None and all_core_seconds > 0:
         # the same per-pair kernel on every host core of this process (at most
-        # 16: the share of a one-GPU box), threads over pairs
+        # 16, a one-GPU host's share), threads over pairs
         nt = max(1, min(16, len(os.sched_getaffinity(0))))
         target = res["value"] * 1e9 * nt * all_core_seconds
         hs, rs = [], []

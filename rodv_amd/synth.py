@@ -1,8 +1,8 @@
 """Deterministic synthetic workloads for the PairHMM engine.
 
 Shapes follow the five benchmark configurations of this path (read length,
-haplotype length, mutation rates, quality distributions); see DESIGN.md for
-the exact interpretation.  Each region draws one uniform ACGT sequence; its
+haplotype length, mutation rates, quality distributions), interpreted as
+follows.  Each region draws one uniform ACGT sequence; its
 haplotypes are prefixes of it (of their own lengths) carrying independent
 0.5% substitutions, as candidate haplotypes of one locus are (the reference's
 own generator builds mutation variants of one base sequence,
