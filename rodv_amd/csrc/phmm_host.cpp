@@ -691,15 +691,32 @@ int validate(const rodv_phmm_batch *b) {
     return RODV_PHMM_OK;
 }
 
+// No exception leaves the C ABI: a failed host allocation becomes
+// RODV_PHMM_ERR_OOM, anything else RODV_PHMM_ERR_DEVICE.
+template <typename F>
+int guarded(F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return RODV_PHMM_ERR_OOM;
+    } catch (...) {
+        return RODV_PHMM_ERR_DEVICE;
+    }
+}
+
 }  // namespace
 
 // ================================================================== C ABI
 
 extern "C" {
 
-int rodv_phmm_init(void) {
+static int init_impl(void) {
     (void)tables();
     return RODV_PHMM_OK;
+}
+
+int rodv_phmm_init(void) {
+    return guarded([&] { return init_impl(); });
 }
 
 const char *rodv_phmm_strerror(int status) {
@@ -732,7 +749,7 @@ void rodv_phmm_config_default(rodv_phmm_config *cfg) {
     cfg->haplotype_chains = 1;
 }
 
-int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
+static int ctx_create_impl(int device, rodv_phmm_ctx **out) {
     if (!out) return RODV_PHMM_ERR_INVALID_ARG;
     *out = nullptr;
     int ndev = 0;
@@ -764,6 +781,10 @@ int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
     return RODV_PHMM_OK;
 }
 
+int rodv_phmm_ctx_create(int device, rodv_phmm_ctx **out) {
+    return guarded([&] { return ctx_create_impl(device, out); });
+}
+
 void rodv_phmm_ctx_destroy(rodv_phmm_ctx *ctx) {
     if (!ctx) return;
     for (int k = 0; k < kSideStreams; k++) {
@@ -775,7 +796,7 @@ void rodv_phmm_ctx_destroy(rodv_phmm_ctx *ctx) {
     delete ctx;
 }
 
-int rodv_phmm_tables_f32(float *ph2pr, float *mm, int mm_n) {
+static int tables_f32_impl(float *ph2pr, float *mm, int mm_n) {
     const HostTables &t = tables();
     if (mm_n < 0 || mm_n > kMMSize) return RODV_PHMM_ERR_INVALID_ARG;
     if (ph2pr) std::memcpy(ph2pr, t.ph2pr_f, sizeof(t.ph2pr_f));
@@ -783,7 +804,11 @@ int rodv_phmm_tables_f32(float *ph2pr, float *mm, int mm_n) {
     return RODV_PHMM_OK;
 }
 
-int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n) {
+int rodv_phmm_tables_f32(float *ph2pr, float *mm, int mm_n) {
+    return guarded([&] { return tables_f32_impl(ph2pr, mm, mm_n); });
+}
+
+static int tables_f64_impl(double *ph2pr, double *mm, int mm_n) {
     const HostTables &t = tables();
     if (mm_n < 0 || mm_n > kMMSize) return RODV_PHMM_ERR_INVALID_ARG;
     if (ph2pr) std::memcpy(ph2pr, t.ph2pr_d, sizeof(t.ph2pr_d));
@@ -791,7 +816,11 @@ int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n) {
     return RODV_PHMM_OK;
 }
 
-int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
+int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n) {
+    return guarded([&] { return tables_f64_impl(ph2pr, mm, mm_n); });
+}
+
+static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                           const rodv_phmm_config *cfg_in, rodv_phmm_plan **out) {
     if (!ctx || !out) return RODV_PHMM_ERR_INVALID_ARG;
     *out = nullptr;
@@ -880,6 +909,14 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
 
     rodv_phmm_plan *p = new (std::nothrow) rodv_phmm_plan();
     if (!p) return RODV_PHMM_ERR_OOM;
+    // frees the plan on every way out but success (which clears p), an
+    // exception included
+    struct Hold {
+        rodv_phmm_plan *&p;
+        ~Hold() {
+            if (p) free_plan(p);
+        }
+    } hold{p};
     p->ctx = ctx;
     p->cfg = cfg;
     p->n_pairs = n_pairs;
@@ -1010,7 +1047,6 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         bk.kern = bk.tab ? fwd_tab_kernel_ptr(bk.L, bk.MR, bk.inter)
                          : fwd_f32_kernel(bk.L, bk.MR, bk.inter);
         if (!bk.kern) {
-            free_plan(p);
             return RODV_PHMM_ERR_UNSUPPORTED;
         }
         // LDS for every pair the bucket's kernel may take, plain launches included
@@ -1105,7 +1141,6 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             p->d_plain_items.push_back(d);
             p->plain_items_n.push_back(items.size());
             if (rc) {
-                free_plan(p);
                 return rc;
             }
         }
@@ -1185,14 +1220,19 @@ int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         if (!rc && hipDeviceSynchronize() != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     }
     if (rc) {
-        free_plan(p);
         return rc;
     }
     p->stats.num_pairs = n_pairs;
     p->stats.cells = p->cells;
     p->stats.num_buckets = (uint32_t)p->buckets.size();
     *out = p;
+    p = nullptr;  // the caller owns it now
     return RODV_PHMM_OK;
+}
+
+int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
+                          const rodv_phmm_config *cfg_in, rodv_phmm_plan **out) {
+    return guarded([&] { return plan_create_impl(ctx, b, cfg_in, out); });
 }
 
 }  // extern "C"
@@ -1339,7 +1379,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
 
 extern "C" {
 
-int rodv_phmm_plan_execute(rodv_phmm_plan *p, void *stream) {
+static int plan_execute_impl(rodv_phmm_plan *p, void *stream) {
     if (!p) return RODV_PHMM_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(p->ctx->device));
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -1373,13 +1413,21 @@ int rodv_phmm_plan_execute(rodv_phmm_plan *p, void *stream) {
     return RODV_PHMM_OK;
 }
 
-int rodv_phmm_plan_execute_plain(rodv_phmm_plan *p, void *stream) {
+int rodv_phmm_plan_execute(rodv_phmm_plan *p, void *stream) {
+    return guarded([&] { return plan_execute_impl(p, stream); });
+}
+
+static int plan_execute_plain_impl(rodv_phmm_plan *p, void *stream) {
     if (!p) return RODV_PHMM_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(p->ctx->device));
     return enqueue(p, reinterpret_cast<hipStream_t>(stream), 1, nullptr);
 }
 
-int rodv_phmm_plan_profile(rodv_phmm_plan *p, void *stream) {
+int rodv_phmm_plan_execute_plain(rodv_phmm_plan *p, void *stream) {
+    return guarded([&] { return plan_execute_plain_impl(p, stream); });
+}
+
+static int plan_profile_impl(rodv_phmm_plan *p, void *stream) {
     if (!p) return RODV_PHMM_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(p->ctx->device));
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -1401,6 +1449,10 @@ int rodv_phmm_plan_profile(rodv_phmm_plan *p, void *stream) {
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+int rodv_phmm_plan_profile(rodv_phmm_plan *p, void *stream) {
+    return guarded([&] { return plan_profile_impl(p, stream); });
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -1433,7 +1485,7 @@ int time_candidate(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sub, const rodv_ph
 }
 }  // namespace
 
-int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
+static int autotune_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
                        const rodv_phmm_config *cfg_in, rodv_phmm_tuning *out) {
     if (!ctx || !sample || !out) return RODV_PHMM_ERR_INVALID_ARG;
     int rc = validate(sample);
@@ -1444,6 +1496,15 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
         if (cfg_in->struct_size != sizeof(rodv_phmm_config)) return RODV_PHMM_ERR_INVALID_ARG;
         cfg = *cfg_in;
     }
+    // the shape the tuner forces while timing is cleared on every way out,
+    // an exception included
+    struct Unforce {
+        rodv_phmm_ctx *c;
+        ~Unforce() {
+            c->force_class = -1;
+            c->force = TuneEntry{};
+        }
+    } unforce{ctx};
     cfg.semantics = RODV_PHMM_SEMANTICS_INTRA;  // shapes do not depend on routing
     cfg.use_graph = 0;
     cfg.use_tuning = 0;
@@ -1545,7 +1606,12 @@ int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
     return RODV_PHMM_OK;
 }
 
-int rodv_phmm_set_tuning(rodv_phmm_ctx *ctx, const rodv_phmm_tuning *t) {
+int rodv_phmm_autotune(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
+                       const rodv_phmm_config *cfg_in, rodv_phmm_tuning *out) {
+    return guarded([&] { return autotune_impl(ctx, sample, cfg_in, out); });
+}
+
+static int set_tuning_impl(rodv_phmm_ctx *ctx, const rodv_phmm_tuning *t) {
     if (!ctx) return RODV_PHMM_ERR_INVALID_ARG;
     if (!t) {
         ctx->tuned = false;
@@ -1568,7 +1634,11 @@ int rodv_phmm_set_tuning(rodv_phmm_ctx *ctx, const rodv_phmm_tuning *t) {
     return RODV_PHMM_OK;
 }
 
-int rodv_phmm_get_tuning(const rodv_phmm_ctx *ctx, rodv_phmm_tuning *out) {
+int rodv_phmm_set_tuning(rodv_phmm_ctx *ctx, const rodv_phmm_tuning *t) {
+    return guarded([&] { return set_tuning_impl(ctx, t); });
+}
+
+static int get_tuning_impl(const rodv_phmm_ctx *ctx, rodv_phmm_tuning *out) {
     if (!ctx || !out) return RODV_PHMM_ERR_INVALID_ARG;
     rodv_phmm_tuning r{};
     r.struct_size = sizeof(r);
@@ -1585,7 +1655,11 @@ int rodv_phmm_get_tuning(const rodv_phmm_ctx *ctx, rodv_phmm_tuning *out) {
     return RODV_PHMM_OK;
 }
 
-int rodv_phmm_plan_read(rodv_phmm_plan *p, double *out_log10, uint8_t *out_flags,
+int rodv_phmm_get_tuning(const rodv_phmm_ctx *ctx, rodv_phmm_tuning *out) {
+    return guarded([&] { return get_tuning_impl(ctx, out); });
+}
+
+static int plan_read_impl(rodv_phmm_plan *p, double *out_log10, uint8_t *out_flags,
                         float *out_raw32, double *out_raw64) {
     if (!p) return RODV_PHMM_ERR_INVALID_ARG;
     HIPCHK(hipSetDevice(p->ctx->device));
@@ -1599,6 +1673,11 @@ int rodv_phmm_plan_read(rodv_phmm_plan *p, double *out_log10, uint8_t *out_flags
     HIPCHK(hipMemcpy(&cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
     p->stats.rescued_pairs = cnt;
     return RODV_PHMM_OK;
+}
+
+int rodv_phmm_plan_read(rodv_phmm_plan *p, double *out_log10, uint8_t *out_flags,
+                        float *out_raw32, double *out_raw64) {
+    return guarded([&] { return plan_read_impl(p, out_log10, out_flags, out_raw32, out_raw64); });
 }
 
 int rodv_phmm_plan_stats(const rodv_phmm_plan *p, rodv_phmm_stats *out) {
@@ -1616,7 +1695,7 @@ void rodv_phmm_plan_destroy(rodv_phmm_plan *p) {
     free_plan(p);
 }
 
-int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+static int forward_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                       const rodv_phmm_config *cfg, double *out_log10, uint8_t *out_flags) {
     if (!ctx || !out_log10) return RODV_PHMM_ERR_INVALID_ARG;
     rodv_phmm_plan *p = nullptr;
@@ -1632,7 +1711,12 @@ int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     return rc;
 }
 
-int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                      const rodv_phmm_config *cfg, double *out_log10, uint8_t *out_flags) {
+    return guarded([&] { return forward_impl(ctx, batch, cfg, out_log10, out_flags); });
+}
+
+static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                       const rodv_phmm_params *params, const double *weights, int ckpt_stride,
                       double *out_log10, double *grad, double *kernel_ms) {
     if (!ctx || !batch || !params || !grad) return RODV_PHMM_ERR_INVALID_ARG;
@@ -1813,6 +1897,12 @@ int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         if (b) (void)hipFree(b);
     rodv_phmm_plan_destroy(p);
     return rc;
+}
+
+int rodv_phmm_adjoint(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                      const rodv_phmm_params *params, const double *weights, int ckpt_stride,
+                      double *out_log10, double *grad, double *kernel_ms) {
+    return guarded([&] { return adjoint_impl(ctx, batch, params, weights, ckpt_stride, out_log10, grad, kernel_ms); });
 }
 
 }  // extern "C"

@@ -183,3 +183,21 @@ def test_invalid_inputs_are_rejected(engine):
     with pytest.raises(PhmmError) as e:
         engine.plan(no_reads)
     assert e.value.code == ERR_EMPTY
+
+
+def test_allocation_failure_is_an_error_and_leaves_the_context_usable(engine, oracle_mod):
+    # RODV_PHMM_FAIL_ALLOC=1 makes every device allocation of the library
+    # fail: plan creation reports ERR_OOM (the partly built plan is freed),
+    # and the same context plans and scores normally afterwards
+    from rodv_amd.engine import ERR_OOM
+    rng = np.random.default_rng(13)
+    b = synth.random_batch(rng, 3, 9)
+    os.environ["RODV_PHMM_FAIL_ALLOC"] = "1"
+    try:
+        for _ in range(3):
+            with pytest.raises(PhmmError) as e:
+                engine.plan(b)
+            assert e.value.code == ERR_OOM
+    finally:
+        del os.environ["RODV_PHMM_FAIL_ALLOC"]
+    check_against_oracle(oracle_mod, engine, b)
