@@ -909,6 +909,37 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
     }
 }
 
+// One rescued pair in FP64 with as few rows per lane as its read needs: one
+// strip of 63*MR rows for MR = 1..3, MR = 4 (and strips of 252 rows) beyond.
+// Fewer rows per lane leave fewer identity rows to compute; the arithmetic
+// of every read row is the same.
+template <bool INTER>
+__device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int p) {
+    const int R = __builtin_amdgcn_readfirstlane((int)a.read_len[a.pair_read[p]]);
+    if (R <= 63)
+        process_item<double, 64, 1, INTER, true>(a, lds, p);
+    else if (R <= 126)
+        process_item<double, 64, 2, INTER, true>(a, lds, p);
+    else if (R <= 189)
+        process_item<double, 64, 3, INTER, true>(a, lds, p);
+    else
+        process_item<double, 64, 4, INTER, true>(a, lds, p);
+}
+
+// FP64 rescue of every pair as intra::computeLikelihoods would
+// (pairhmm_api.cpp:20-27): the device-side list of FP32-rejected pairs.
+__global__ __launch_bounds__(64) void fwd_rescue_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t n = *a.n_items_dev;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        rescue_pair<false>(a, lds, (int)a.rescue_list[it]);
+    }
+}
+
 // FP64 rescue under schedule_pairhmm routing: pairs the FP64 regroup put in
 // an inter group run the inter arithmetic, every other rescued pair the
 // intra one (pairhmm_schedule.cpp:273-304, 307-335).  One pair per wave, so
@@ -925,9 +956,9 @@ __global__ __launch_bounds__(64) void fwd_rescue_route_kernel(FwdArgs a) {
         const uint32_t inter =
             __builtin_amdgcn_readfirstlane((a.flags[p] & kFlagGrouped64) ? 1u : 0u);
         if (inter)
-            process_item<double, 64, 4, true, true>(a, lds, p);
+            rescue_pair<true>(a, lds, p);
         else
-            process_item<double, 64, 4, false, true>(a, lds, p);
+            rescue_pair<false>(a, lds, p);
     }
 }
 
@@ -1133,7 +1164,7 @@ uint32_t fwd_tab_bytes(int L, int MR) { return 4u * (uint32_t)L * (uint32_t)((MR
 // route: per pair intra or inter arithmetic as the FP64 regroup decided.
 void *fwd_f64_rescue_kernel(bool route) {
     return route ? reinterpret_cast<void *>(&fwd_rescue_route_kernel)
-                 : kernel_ptr<double, 64, 4, false, true>();
+                 : reinterpret_cast<void *>(&fwd_rescue_kernel);
 }
 
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st) {
