@@ -312,6 +312,7 @@ def main():
         "semantics": args.semantics,
         "autotune": tuning,
         "rescued_pairs": rescued,
+        "rescued_fraction": round(rescued / max(1, batch.num_pairs()), 6),
         "fp64_rescue_ms": round(fp64_ms, 3),
         "median_replay_ms": round(median_ms, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
