@@ -101,3 +101,12 @@ def test_default_chains_large_batch(engine):
     _, nb1 = _run(engine, synth.config(1), 1, semantics=1)
     _, nb0 = _run(engine, synth.config(1), 0, semantics=1)
     assert nb1 == nb0
+
+
+def test_chain_mode_out_of_range_is_rejected(engine):
+    from rodv_amd.engine import ERR_INVALID_ARG, PhmmError
+    b = synth.random_batch(np.random.default_rng(3), 2, 5)
+    for bad in (-1, 3):
+        with pytest.raises(PhmmError) as e:
+            engine.plan(b, default_config(haplotype_chains=bad))
+        assert e.value.code == ERR_INVALID_ARG
