@@ -110,3 +110,23 @@ def test_chain_mode_out_of_range_is_rejected(engine):
         with pytest.raises(PhmmError) as e:
             engine.plan(b, default_config(haplotype_chains=bad))
         assert e.value.code == ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_chains_random_multi_region_batches(engine, seed):
+    # 300 regions of random shapes (equal or mixed read lengths per region,
+    # haplotypes of 1..999 columns, unrelated reads, N bases), about 260,000
+    # pairs: forced chains give the one-pair-per-slot bits
+    rng = np.random.default_rng(4000 + seed)
+    regs = []
+    for _ in range(300):
+        nh, nr = int(rng.integers(1, 40)), int(rng.integers(1, 90))
+        R = int(rng.integers(1, 300))
+        rl = [R] * nr if rng.random() < 0.7 else list(rng.integers(1, 300, nr))
+        hl = list(rng.integers(1, 1000, nh))
+        regs.append(synth.region(rng, hl, rl,
+                                 unrelated=[i for i in range(nr) if rng.random() < 0.05],
+                                 n_read=[i for i in range(nr) if rng.random() < 0.02],
+                                 n_hap=[i for i in range(nh) if rng.random() < 0.02],
+                                 qual=int(rng.integers(5, 41))))
+    _same(engine, Batch.concat(regs), semantics=seed % 2)
