@@ -25,6 +25,9 @@ void *fwd_tab_kernel_ptr(int L, int MR, bool inter);
 void *fwd_chain_kernel_ptr(int L, int MR, bool inter);
 uint32_t fwd_tab_bytes(int L, int MR);
 int fwd_f64_rescue_mr();
+int rescue_long_waves();
+int rescue_long_rows();
+hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
@@ -285,9 +288,11 @@ struct rodv_phmm_plan {
     uint32_t *d_rescue = nullptr;
     uint32_t *d_counters = nullptr;  // [0] rescue count, [1] rescue work, [2..] per launch
     uint32_t n_counters = 0;
-    double *d_bnd64 = nullptr;
+    double *d_bnd64 = nullptr;  // boundary rows of the long-read FP64 rescue
     uint64_t bnd64_elems = 0;
     int rescue_grid = 0;
+    int long_grid = 0;          // workgroups of fwd_rescue_long_kernel (0: no long reads)
+    uint32_t long_counter = 0;  // its work counter
     uint32_t rescue_lds = 0;
     // host copies needed at launch
     std::vector<Bucket> buckets;
@@ -971,10 +976,14 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     std::vector<int> bucket_of_key(4 * 4 * 17, -1);
     std::vector<std::vector<uint32_t>> members;
     uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
+    uint64_t n_long = 0;      // such pairs
     for (uint64_t k = 0; k < n_pairs; k++) {
         const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
         p->cells += (uint64_t)R * H;
-        if (R > 63u * 4u) max_long_h = std::max(max_long_h, H);
+        if (R > (uint32_t)rescue_long_rows()) {
+            max_long_h = std::max(max_long_h, H);
+            n_long++;
+        }
         const int cls = tune_class(R);
         const TuneEntry *te = nullptr;
         if (ctx->force_class == cls)
@@ -1185,7 +1194,8 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                 rc = RODV_PHMM_ERR_DEVICE;
         }
     }
-    p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size();
+    p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size() + 1;
+    p->long_counter = p->n_counters - 1;
     if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
     // FP64 rescue: one pair per wave, two waves per SIMD (its ~180 VGPRs
     // allow no more); boundary rows only if some read > 252
@@ -1195,9 +1205,14 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     for (uint32_t h = 0; h < M; h++) max_h_all = std::max(max_h_all, b->hap_lengths[h]);
     p->rescue_lds = lds_stride_for(max_h_all, 64);
     if (max_long_h) {
+        // long reads: workgroups of rescue_long_waves() (8) waves, one per CU
+        // (2 waves per SIMD; the kernel's VGPRs allow 3, so a second would
+        // not fit), each with rescue_long_waves() + 1 boundary-row buffers
+        p->long_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n_long, (uint64_t)ctx->num_cus));
+        const size_t nbuf = (size_t)p->long_grid * (size_t)(rescue_long_waves() + 1);
         p->bnd64_elems = 3ull * (max_long_h + 3ull * 64 + 2);
-        if (!rc) rc = dev_alloc(&p->d_bnd64, (size_t)p->rescue_grid * 2 * p->bnd64_elems);
-        if (!rc && hipMemset(p->d_bnd64, 0, (size_t)p->rescue_grid * 2 * p->bnd64_elems * sizeof(double)) != hipSuccess)
+        if (!rc) rc = dev_alloc(&p->d_bnd64, nbuf * p->bnd64_elems);
+        if (!rc && hipMemset(p->d_bnd64, 0, nbuf * p->bnd64_elems * sizeof(double)) != hipSuccess)
             rc = RODV_PHMM_ERR_DEVICE;
     }
     // descriptors of the table kernels' item slots (batched and plain order)
@@ -1358,6 +1373,11 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
         a.bnd_elems = p->bnd64_elems;
         HIPCHK(launch_fwd(fwd_f64_rescue_kernel(p->any_inter), p->rescue_grid, p->rescue_lds, st, a));
         launches++;
+        if (p->long_grid) {  // the rescued reads longer than one strip
+            a.counter = p->d_counters + p->long_counter;
+            HIPCHK(launch_rescue_long(p->any_inter, p->long_grid, p->rescue_lds, st, a));
+            launches++;
+        }
     }
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     FinalizeArgs f;

@@ -910,9 +910,10 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
 }
 
 // One rescued pair in FP64 with as few rows per lane as its read needs: one
-// strip of 63*MR rows for MR = 1..3, MR = 4 (and strips of 252 rows) beyond.
-// Fewer rows per lane leave fewer identity rows to compute; the arithmetic
-// of every read row is the same.
+// strip of 63*MR rows for MR = 1..4.  Fewer rows per lane leave fewer
+// identity rows to compute; the arithmetic of every read row is the same.
+// Longer reads are fwd_rescue_long_kernel's.
+constexpr int kRescueStrip = 63 * 4;
 template <bool INTER>
 __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int p) {
     const int R = __builtin_amdgcn_readfirstlane((int)a.read_len[a.pair_read[p]]);
@@ -922,8 +923,180 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
         process_item<double, 64, 2, INTER, true>(a, lds, p);
     else if (R <= 189)
         process_item<double, 64, 3, INTER, true>(a, lds, p);
-    else
+    else if (R <= kRescueStrip)
         process_item<double, 64, 4, INTER, true>(a, lds, p);
+}
+
+// ------------------------------------------------- long reads, FP64
+// A rescued pair longer than one strip of 252 rows, its strips pipelined
+// over the kLW waves of a workgroup: wave w takes strips w, w + kLW, ...
+// (64 lanes x 4 rows, lane 63 the identity lane / feeder, as in
+// process_item).  The waves advance in blocks of kLB steps with one
+// workgroup barrier per block.  Strip s starts at block
+// (s / kLW) * P + 2 * (s % kLW), P = max(blocks per strip, 2 * kLW): a
+// wave's strips do not overlap, and every strip starts at least two blocks
+// after the strip above it.  The boundary row strip s+1 reads at its step d
+// (column d + 1 of strip s's last row) is written by strip s at its step
+// d + L - 1 <= d + kLB - 1, so in an earlier block, before the barrier that
+// precedes its use.  Every wave runs the same number of blocks, so the
+// barrier count is uniform.  Boundary rows go through kLW + 1 HBM buffers
+// per workgroup (strip s writes buffer s % (kLW + 1)); a buffer is written
+// again only by strip s + kLW + 1, which starts once strip s + 1, its
+// reader, has finished.
+constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
+constexpr int kLB = 64;  // steps per block (>= L - 1)
+
+template <bool INTER>
+__device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
+    using T = double;
+    constexpr int L = 64, MR = 4;
+    constexpr int C = (L - 1) * MR;
+    static_assert(kLB >= L - 1, "a block covers the boundary row's lag");
+    const int l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const uint32_t h = __builtin_amdgcn_readfirstlane(a.pair_hap[p]);
+    const uint32_t r = __builtin_amdgcn_readfirstlane(a.pair_read[p]);
+    const int R = __builtin_amdgcn_readfirstlane((int)a.read_len[r]);
+    const int H = __builtin_amdgcn_readfirstlane((int)a.hap_len[h]);
+    const uint64_t ro64 = a.read_off[r];
+    const uint64_t ro = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ro64 >> 32)) << 32) |
+                        (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ro64);
+    const int off = (MR - R % MR) % MR;
+    const int vR = R - 1 + off;
+    const int nstr = vR / C + 1;
+    const int lR = (vR % C) / MR;
+    const T initd = Tab<T>::initd(a, h);
+
+    // one-hot codes of the haplotype, shared by the waves: column j at
+    // lds[L - 1 + j]
+    for (uint32_t k = threadIdx.x * 16u; k < a.lds_stride; k += 64u * kLW * 16u)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
+        const int nch = (H + 15) >> 4;
+        for (int c = threadIdx.x; c < nch; c += 64 * kLW)
+            *reinterpret_cast<uint4 *>(lds + L + 16 * c) = src[c];
+    }
+    __syncthreads();
+    const uint8_t *cp = lds + (L - 1) - l;
+
+    const int nb = (H + 2 * L - 1 + kLB - 1) / kLB;  // blocks of a strip (steps 1 .. H + 2L - 1)
+    const int P0 = nb > 2 * kLW ? nb : 2 * kLW;
+    const int nblk = ((nstr - 1) / kLW) * P0 + 2 * ((nstr - 1) % kLW) + nb;
+    T *bbase = reinterpret_cast<T *>(a.bnd) + (uint64_t)blockIdx.x * (kLW + 1) * a.bnd_elems;
+
+    const LaneRot<L> rot;
+    const bool feeder = l == L - 1;
+    Priors<T, MR> P;
+    State<T, MR> S;
+    T yyA = T(0), myA = T(0), sm = T(0), si = T(0);
+    T *bnd_in = nullptr, *bnd_out = nullptr;
+    bool inject = false, store = false, last = false;
+    int Dend = 0, d = 1;
+    int cs = w;  // this wave's current strip
+    for (int t = 0; t < nblk; t++) {
+        const int st0 = (cs / kLW) * P0 + 2 * (cs % kLW);
+        if (cs < nstr && t >= st0 && t < st0 + nb) {
+            if (t == st0) {  // the strip's set-up, as process_item's
+#pragma unroll
+                for (int s = 0; s < MR; s++) {
+                    const int i = cs * C + l * MR + s - off + 1;  // 1-based read row
+                    const bool real = l < L - 1 && i >= 1 && i <= R;
+                    load_row<T, MR, INTER>(a, P, s, real, ro + (uint64_t)(real ? i - 1 : 0));
+                    S.M[s] = T(0);
+                    S.I[s] = T(0);
+                    S.D[s] = real ? T(0) : initd;
+                }
+                S.Mp = T(0);
+                S.Ip = T(0);
+                S.Dp = rot(S.D[MR - 1]);
+                yyA = rot(P.yy[MR - 1]);
+                myA = rot(P.my[MR - 1]);
+                if (cs > 0 && l == 0) {
+                    const uint64_t ip = ro + (uint64_t)(cs * C - off - 1);  // 0-based row above
+                    yyA = Tab<T>::ph2pr(a.tab)[a.gcp[ip] & 127u];
+                    myA = Tab<T>::ph2pr(a.tab)[a.del[ip] & 127u];
+                    S.Dp = T(0);
+                }
+                last = cs == nstr - 1;
+                inject = cs > 0;
+                store = !last;
+                bnd_out = bbase + (uint64_t)(cs % (kLW + 1)) * a.bnd_elems;
+                bnd_in = bbase + (uint64_t)((cs + kLW) % (kLW + 1)) * a.bnd_elems;  // strip cs - 1's
+                if (inject) {
+                    T bm = ld_agent(bnd_in + 3 * (1 + L)), bi = ld_agent(bnd_in + 3 * (1 + L) + 1),
+                      bd = ld_agent(bnd_in + 3 * (1 + L) + 2);
+                    if (feeder) {
+                        S.M[MR - 1] = bm;
+                        S.I[MR - 1] = bi;
+                        S.D[MR - 1] = bd;
+                    }
+                }
+                Dend = last ? H + lR : H + 2 * L - 1;
+                sm = T(0);
+                si = T(0);
+                d = 1;
+            }
+            const int dlim = d + kLB - 1 < Dend ? d + kLB - 1 : Dend;
+            for (; d <= dlim; d++) {
+                step<T, L, MR, INTER>(S, P, yyA, myA, cp[d], rot);
+                if (last && d - l <= H) {
+                    sm += S.M[MR - 1];
+                    si += S.I[MR - 1];
+                }
+                if (store && l == L - 2) {  // column d - (L-2) at index d + 2
+                    T *o = bnd_out + 3 * (d + 2);
+                    st_agent(o, S.M[MR - 1]);
+                    st_agent(o + 1, S.I[MR - 1]);
+                    st_agent(o + 2, S.D[MR - 1]);
+                }
+                if (inject) {  // column d + 1 of the strip above
+                    const T *in = bnd_in + 3 * (d + 1 + L);
+                    T bm = ld_agent(in), bi = ld_agent(in + 1), bd = ld_agent(in + 2);
+                    if (feeder) {
+                        S.M[MR - 1] = bm;
+                        S.I[MR - 1] = bi;
+                        S.D[MR - 1] = bd;
+                    }
+                }
+            }
+            if (t == st0 + nb - 1) {
+                if (last && l == lR) a.raw64[p] = sm + si;  // pairhmm_impl.cpp:434-437
+                cs += kLW;
+            }
+        }
+        __threadfence();  // this block's boundary stores before the barrier
+        __syncthreads();
+    }
+}
+
+// Persistent over the rescue list; takes the pairs longer than one strip
+// (rescue_pair leaves them), one pair per workgroup at a time.  ROUTE: per
+// pair inter or intra FP64 as the regroup decided.
+template <bool ROUTE>
+__global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t s_it;
+    const uint32_t n = *a.n_items_dev;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        if (threadIdx.x == 0) {
+            uint32_t it = atomicAdd(a.counter, 1u);
+            while (it < n && (int)a.read_len[a.pair_read[a.rescue_list[it]]] <= kRescueStrip)
+                it = atomicAdd(a.counter, 1u);
+            s_it = it;
+        }
+        __syncthreads();
+        const uint32_t it = s_it;
+        __syncthreads();  // every wave has read s_it before thread 0 writes it again
+        if (it >= n) break;
+        const int p = (int)a.rescue_list[it];
+        const bool inter = ROUTE && (a.flags[p] & kFlagGrouped64);
+        if (__builtin_amdgcn_readfirstlane(inter ? 1 : 0))
+            long_pair<true>(a, lds, p);
+        else
+            long_pair<false>(a, lds, p);
+    }
 }
 
 // FP64 rescue of every pair as intra::computeLikelihoods would
@@ -1165,6 +1338,16 @@ uint32_t fwd_tab_bytes(int L, int MR) { return 4u * (uint32_t)L * (uint32_t)((MR
 void *fwd_f64_rescue_kernel(bool route) {
     return route ? reinterpret_cast<void *>(&fwd_rescue_route_kernel)
                  : reinterpret_cast<void *>(&fwd_rescue_kernel);
+}
+
+// FP64 rescue of the reads longer than one strip (workgroups of kLW waves).
+int rescue_long_waves() { return kLW; }
+int rescue_long_rows() { return kRescueStrip; }
+hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
+    void *kern = route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true>)
+                       : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false>);
+    void *args[] = {const_cast<FwdArgs *>(&a)};
+    return hipLaunchKernel(kern, dim3(grid), dim3(64 * kLW), args, lds_bytes, st);
 }
 
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st) {
