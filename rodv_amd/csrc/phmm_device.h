@@ -24,7 +24,7 @@ struct LaneXfer {
     static_assert(DIR == 1 || DIR == -1, "direction");
     int addr;
     __device__ __forceinline__ LaneXfer() {
-        const int lane = threadIdx.x;
+        const int lane = threadIdx.x & 63;  // the lane in its wave
         const int src = (lane & ~(L - 1)) | ((lane + (DIR > 0 ? L - 1 : 1)) & (L - 1));
         addr = src << 2;
     }

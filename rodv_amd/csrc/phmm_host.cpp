@@ -28,6 +28,8 @@ int fwd_f64_rescue_mr();
 int rescue_long_waves();
 int rescue_long_rows();
 hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
+void *fwd_long_kernel_ptr(int MR, bool inter);
+hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
@@ -224,6 +226,8 @@ struct Bucket {
     uint64_t bnd_elems = 0;
     int grid = 0;
     int grid_cap = 1;  // persistent grid of a full launch (plain launches are capped by it)
+    int block = 64;    // threads per workgroup (fwd_long_kernel: several waves)
+    uint32_t nbnd = 2; // boundary-row buffers per workgroup
     uint64_t cells = 0;
 };
 
@@ -1089,6 +1093,16 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         if (bk.grid < 1) bk.grid = 1;
         if (bk.max_strips > 1) {
             bk.bnd_elems = 3ull * (bk.max_h + 3ull * bk.L + 2);
+            // reads longer than one strip: their strips pipelined over the
+            // waves of a workgroup, one workgroup per CU
+            void *lk = bk.L == 64 ? fwd_long_kernel_ptr(bk.MR, bk.inter) : nullptr;
+            if (lk) {
+                bk.kern = lk;
+                bk.block = 64 * rescue_long_waves();
+                bk.nbnd = (uint32_t)rescue_long_waves() + 1;
+                bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus));
+                bk.grid_cap = bk.grid;
+            }
         }
     }
     // buckets by descending work, so long launches start first
@@ -1189,8 +1203,8 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             if (!rc) rc = upload(&bk.d_chain_hap, bk.chain_hap.data(), bk.chain_hap.size());
         }
         if (!rc && bk.bnd_elems) {
-            rc = dev_alloc(&bk.d_bnd, (size_t)bk.grid * 2 * bk.bnd_elems);
-            if (!rc && hipMemset(bk.d_bnd, 0, (size_t)bk.grid * 2 * bk.bnd_elems * sizeof(float)) != hipSuccess)
+            rc = dev_alloc(&bk.d_bnd, (size_t)bk.grid * bk.nbnd * bk.bnd_elems);
+            if (!rc && hipMemset(bk.d_bnd, 0, (size_t)bk.grid * bk.nbnd * bk.bnd_elems * sizeof(float)) != hipSuccess)
                 rc = RODV_PHMM_ERR_DEVICE;
         }
     }
@@ -1319,7 +1333,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 // chained items: G prior tables and two code buffers per wave
                 const size_t lds = bk.chain ? (size_t)(64 / bk.L) * bk.tab_bytes + 2u * bk.lds_stride
                                             : (size_t)(64 / bk.L) * (bk.lds_stride + bk.tab_bytes);
-                HIPCHK(launch_fwd(bk.kern, bk.grid, lds, bs, a));
+                HIPCHK(launch_fwd_block(bk.kern, bk.grid, bk.block, lds, bs, a));
                 launches++;
             }
             for (int k = 0; k + 1 < ns; k++) {
@@ -1341,7 +1355,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
                 const size_t lds = (size_t)G * (bk.lds_stride + bk.tab_bytes);
-                HIPCHK(launch_fwd(bk.kern, std::min(pl.grid, bk.grid_cap), lds, st, a));
+                HIPCHK(launch_fwd_block(bk.kern, std::min(pl.grid, bk.grid_cap), bk.block, lds, st, a));
                 launches++;
             }
         }

@@ -946,10 +946,9 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
 constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
 constexpr int kLB = 64;  // steps per block (>= L - 1)
 
-template <bool INTER>
+template <typename T, int MR, bool INTER, bool RESCUE>
 __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
-    using T = double;
-    constexpr int L = 64, MR = 4;
+    constexpr int L = 64;
     constexpr int C = (L - 1) * MR;
     static_assert(kLB >= L - 1, "a block covers the boundary row's lag");
     const int l = threadIdx.x & 63;
@@ -1062,7 +1061,21 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
                 }
             }
             if (t == st0 + nb - 1) {
-                if (last && l == lR) a.raw64[p] = sm + si;  // pairhmm_impl.cpp:434-437
+                if (last && l == lR) {
+                    const T Ssum = sm + si;  // pairhmm_impl.cpp:434-437
+                    if constexpr (RESCUE) {
+                        a.raw64[p] = (double)Ssum;
+                    } else {
+                        a.raw32[p] = (float)Ssum;
+                        uint8_t fl = 0;
+                        if ((float)Ssum < 1e-28f) {  // MIN_ACCEPTED, pairhmm_api.cpp:4,17
+                            fl = 0x01;
+                            const uint32_t k = atomicAdd(a.rescue_count, 1u);
+                            a.rescue_list[k] = (uint32_t)p;
+                        }
+                        a.flags[p] = fl;
+                    }
+                }
                 cs += kLW;
             }
         }
@@ -1093,9 +1106,28 @@ __global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
         const int p = (int)a.rescue_list[it];
         const bool inter = ROUTE && (a.flags[p] & kFlagGrouped64);
         if (__builtin_amdgcn_readfirstlane(inter ? 1 : 0))
-            long_pair<true>(a, lds, p);
+            long_pair<double, 4, true, true>(a, lds, p);
         else
-            long_pair<false>(a, lds, p);
+            long_pair<double, 4, false, true>(a, lds, p);
+    }
+}
+
+// The FP32 pass of a 64-lane select-prior bucket whose reads need more than
+// one strip: one pair per item, its strips pipelined over the workgroup's
+// waves as above.
+template <int MR, bool INTER>
+__global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t s_it;
+    const uint32_t n = a.n_items;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        if (threadIdx.x == 0) s_it = atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint32_t it = s_it;
+        __syncthreads();  // every wave has read s_it before thread 0 writes it again
+        if (it >= n) break;
+        const int p = a.items[it];
+        if (p >= 0) long_pair<float, MR, INTER, false>(a, lds, p);
     }
 }
 
@@ -1342,6 +1374,19 @@ void *fwd_f64_rescue_kernel(bool route) {
 
 // FP64 rescue of the reads longer than one strip (workgroups of kLW waves).
 int rescue_long_waves() { return kLW; }
+void *fwd_long_kernel_ptr(int MR, bool inter) {
+    switch (MR) {
+    case 4: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<4, true>) : reinterpret_cast<void *>(&fwd_long_kernel<4, false>);
+    case 8: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<8, true>) : reinterpret_cast<void *>(&fwd_long_kernel<8, false>);
+    case 12: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<12, true>) : reinterpret_cast<void *>(&fwd_long_kernel<12, false>);
+    case 16: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<16, true>) : reinterpret_cast<void *>(&fwd_long_kernel<16, false>);
+    default: return nullptr;
+    }
+}
+hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
+    void *args[] = {const_cast<FwdArgs *>(&a)};
+    return hipLaunchKernel(kern, dim3(grid), dim3(block), args, lds_bytes, st);
+}
 int rescue_long_rows() { return kRescueStrip; }
 hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
     void *kern = route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true>)
