@@ -946,6 +946,15 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
 constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
 constexpr int kLB = 64;  // steps per block (>= L - 1)
 
+// Lane k's value, for every lane (k uniform).
+__device__ __forceinline__ float lane_value(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+__device__ __forceinline__ double lane_value(double v, int k) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), k),
+                            __builtin_amdgcn_readlane(__double2loint(v), k));
+}
+
 template <typename T, int MR, bool INTER, bool RESCUE>
 __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
     constexpr int L = 64;
@@ -1038,6 +1047,18 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
                 d = 1;
             }
             const int dlim = d + kLB - 1 < Dend ? d + kLB - 1 : Dend;
+            // the feeder's inputs of this block's steps, all written before
+            // the last barrier: lane k loads column d0 + k + 1 of the strip
+            // above (index d0 + k + 1 + L), one coalesced load per block;
+            // step d0 + k then takes lane k's values with readlane
+            const int d0 = d;
+            T fm = T(0), fi = T(0), fd = T(0);
+            if (inject && d0 + l <= dlim) {
+                const T *in = bnd_in + 3 * (d0 + l + 1 + L);
+                fm = ld_agent(in);
+                fi = ld_agent(in + 1);
+                fd = ld_agent(in + 2);
+            }
             for (; d <= dlim; d++) {
                 step<T, L, MR, INTER>(S, P, yyA, myA, cp[d], rot);
                 if (last && d - l <= H) {
@@ -1051,8 +1072,8 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
                     st_agent(o + 2, S.D[MR - 1]);
                 }
                 if (inject) {  // column d + 1 of the strip above
-                    const T *in = bnd_in + 3 * (d + 1 + L);
-                    T bm = ld_agent(in), bi = ld_agent(in + 1), bd = ld_agent(in + 2);
+                    const int k = d - d0;
+                    const T bm = lane_value(fm, k), bi = lane_value(fi, k), bd = lane_value(fd, k);
                     if (feeder) {
                         S.M[MR - 1] = bm;
                         S.I[MR - 1] = bi;
