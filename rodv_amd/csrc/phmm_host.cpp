@@ -229,6 +229,13 @@ struct Bucket {
     int block = 64;    // threads per workgroup (fwd_long_kernel: several waves)
     uint32_t nbnd = 2; // boundary-row buffers per workgroup
     uint64_t cells = 0;
+    // fwd_long_kernel, batched: the items' pairs as units of strips, whose
+    // slots follow the grid's boundary rows in d_bnd and take counters
+    // prog_base .. prog_base + n_xslots - 1
+    std::vector<LongUnit> units;
+    LongUnit *d_units = nullptr;
+    uint32_t n_xslots = 0;
+    uint32_t prog_base = 0;
 };
 
 // one launch of the plain (per-region) schedule
@@ -297,6 +304,12 @@ struct rodv_phmm_plan {
     int rescue_grid = 0;
     int long_grid = 0;          // workgroups of fwd_rescue_long_kernel (0: no long reads)
     uint32_t long_counter = 0;  // its work counter
+    // its units: every pair whose read is longer than one FP64 strip, the
+    // longest pipelines first; slots after the grid's rows in d_bnd64
+    std::vector<LongUnit> long_units;
+    LongUnit *d_long_units = nullptr;
+    uint32_t n_x64 = 0;
+    uint32_t prog64_base = 0;
     uint32_t rescue_lds = 0;
     // host copies needed at launch
     std::vector<Bucket> buckets;
@@ -340,7 +353,9 @@ void free_plan(rodv_phmm_plan *p) {
         if (b.d_chain_off) (void)hipFree(b.d_chain_off);
         if (b.d_chain_hap) (void)hipFree(b.d_chain_hap);
         if (b.d_bnd) (void)hipFree(b.d_bnd);
+        if (b.d_units) (void)hipFree(b.d_units);
     }
+    if (p->d_long_units) (void)hipFree(p->d_long_units);
     for (auto *d : p->d_plain_items)
         if (d) (void)hipFree(d);
     for (auto *d : p->d_plain_desc)
@@ -596,6 +611,27 @@ int strips_for(uint32_t R, int L, int MR) {
     const int C = (L - 1) * MR;
     const int off = (MR - (int)R % MR) % MR;
     return ((int)R - 1 + off) / C + 1;
+}
+
+// A pair's units for the long-read kernels: runs of at most
+// rescue_long_waves() strips (one per wave), each but the last handing its
+// boundary row to the next through a new slot.  False if the strip count
+// does not fit the units' 16-bit fields.
+bool append_units(std::vector<LongUnit> &units, uint32_t &nslots, int32_t pair, int nstr) {
+    if (nstr < 1 || nstr > 0xffff) return false;
+    const int W = rescue_long_waves();
+    uint32_t xin = kNoSlot;
+    for (int s0 = 0; s0 < nstr; s0 += W) {
+        const int s1 = std::min(nstr, s0 + W);
+        LongUnit u;
+        u.pair = pair;
+        u.strips = (uint32_t)s0 | (uint32_t)s1 << 16;
+        u.xin = xin;
+        u.xout = s1 < nstr ? nslots++ : kNoSlot;
+        xin = u.xout;
+        units.push_back(u);
+    }
+    return true;
 }
 
 // Group widths schedule_pairhmm would use on this host (pairhmm_schedule.cpp:
@@ -1102,6 +1138,13 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                 bk.nbnd = (uint32_t)rescue_long_waves() + 1;
                 bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus));
                 bk.grid_cap = bk.grid;
+                // batched: the pairs split into units of strips, so that a
+                // long pair's units run on several CUs at once
+                for (int32_t q : bk.items)
+                    if (q >= 0 && !append_units(bk.units, bk.n_xslots, q,
+                                                strips_for(b->read_lengths[pr[q]], 64, bk.MR)))
+                        return RODV_PHMM_ERR_INVALID_ARG;
+                bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.units.size(), (uint64_t)ctx->num_cus));
             }
         }
     }
@@ -1203,14 +1246,25 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             if (!rc) rc = upload(&bk.d_chain_hap, bk.chain_hap.data(), bk.chain_hap.size());
         }
         if (!rc && bk.bnd_elems) {
-            rc = dev_alloc(&bk.d_bnd, (size_t)bk.grid * bk.nbnd * bk.bnd_elems);
-            if (!rc && hipMemset(bk.d_bnd, 0, (size_t)bk.grid * bk.nbnd * bk.bnd_elems * sizeof(float)) != hipSuccess)
+            // the units' slots start as NaN: a row read before its unit
+            // wrote it would make the pair's result NaN, not a plausible
+            // wrong value (the columns read unwritten lie beyond the
+            // haplotype and reach no counted cell)
+            const size_t nring = (size_t)bk.grid * bk.nbnd;
+            rc = dev_alloc(&bk.d_bnd, (nring + bk.n_xslots) * bk.bnd_elems);
+            if (!rc && (hipMemset(bk.d_bnd, 0, nring * bk.bnd_elems * sizeof(float)) != hipSuccess ||
+                        hipMemset(bk.d_bnd + nring * bk.bnd_elems, 0xff,
+                                  (size_t)bk.n_xslots * bk.bnd_elems * sizeof(float)) != hipSuccess))
                 rc = RODV_PHMM_ERR_DEVICE;
         }
+        if (!rc && !bk.units.empty()) rc = upload(&bk.d_units, bk.units.data(), bk.units.size());
     }
     p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size() + 1;
     p->long_counter = p->n_counters - 1;
-    if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
+    for (auto &bk : p->buckets) {  // progress counters of the long-read slots
+        bk.prog_base = p->n_counters;
+        p->n_counters += bk.n_xslots;
+    }
     // FP64 rescue: one pair per wave, two waves per SIMD (its ~180 VGPRs
     // allow no more); boundary rows only if some read > 252
     const int wpc64 = 8;
@@ -1221,14 +1275,38 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (max_long_h) {
         // long reads: workgroups of rescue_long_waves() (8) waves, one per CU
         // (2 waves per SIMD; the kernel's VGPRs allow 3, so a second would
-        // not fit), each with rescue_long_waves() + 1 boundary-row buffers
-        p->long_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n_long, (uint64_t)ctx->num_cus));
-        const size_t nbuf = (size_t)p->long_grid * (size_t)(rescue_long_waves() + 1);
+        // not fit), each with rescue_long_waves() + 1 boundary-row buffers.
+        // Units of every pair that may need them, longest pipeline first:
+        // blocks of a strip plus the two-block lag of each further strip
+        std::vector<std::pair<uint64_t, uint32_t>> lp;
+        lp.reserve(n_long);
+        for (uint64_t k = 0; k < n_pairs; k++) {
+            const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
+            if (R <= (uint32_t)rescue_long_rows()) continue;
+            const uint64_t ns = (uint64_t)strips_for(R, 64, 4);
+            lp.push_back({((uint64_t)H + 127) / 64 + 2 * (ns - 1), (uint32_t)k});
+        }
+        std::stable_sort(lp.begin(), lp.end(), [](const std::pair<uint64_t, uint32_t> &x,
+                                                  const std::pair<uint64_t, uint32_t> &y) {
+            return x.first > y.first;
+        });
+        for (const auto &e : lp)
+            if (!rc && !append_units(p->long_units, p->n_x64, (int32_t)e.second,
+                                     strips_for(b->read_lengths[pr[e.second]], 64, 4)))
+                rc = RODV_PHMM_ERR_INVALID_ARG;
+        p->prog64_base = p->n_counters;
+        p->n_counters += p->n_x64;
+        p->long_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(p->long_units.size(), (uint64_t)ctx->num_cus));
+        const size_t nring = (size_t)p->long_grid * (size_t)(rescue_long_waves() + 1);
         p->bnd64_elems = 3ull * (max_long_h + 3ull * 64 + 2);
-        if (!rc) rc = dev_alloc(&p->d_bnd64, nbuf * p->bnd64_elems);
-        if (!rc && hipMemset(p->d_bnd64, 0, nbuf * p->bnd64_elems * sizeof(double)) != hipSuccess)
+        if (!rc) rc = dev_alloc(&p->d_bnd64, (nring + p->n_x64) * p->bnd64_elems);
+        if (!rc && (hipMemset(p->d_bnd64, 0, nring * p->bnd64_elems * sizeof(double)) != hipSuccess ||
+                    hipMemset(p->d_bnd64 + nring * p->bnd64_elems, 0xff,
+                              (size_t)p->n_x64 * p->bnd64_elems * sizeof(double)) != hipSuccess))
             rc = RODV_PHMM_ERR_DEVICE;
+        if (!rc) rc = upload(&p->d_long_units, p->long_units.data(), p->long_units.size());
     }
+    if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
     // descriptors of the table kernels' item slots (batched and plain order)
     if (!rc) {
         const FwdArgs a = base_args(p);
@@ -1330,6 +1408,12 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 a.bnd = bk.d_bnd;
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
+                if (bk.d_units) {  // fwd_long_kernel over units of strips
+                    a.units = bk.d_units;
+                    a.n_items = (uint32_t)bk.units.size();
+                    a.prog = p->d_counters + bk.prog_base;
+                    a.xbnd = bk.d_bnd + (size_t)bk.grid * bk.nbnd * bk.bnd_elems;
+                }
                 // chained items: G prior tables and two code buffers per wave
                 const size_t lds = bk.chain ? (size_t)(64 / bk.L) * bk.tab_bytes + 2u * bk.lds_stride
                                             : (size_t)(64 / bk.L) * (bk.lds_stride + bk.tab_bytes);
@@ -1389,6 +1473,10 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
         launches++;
         if (p->long_grid) {  // the rescued reads longer than one strip
             a.counter = p->d_counters + p->long_counter;
+            a.units = p->d_long_units;
+            a.n_items = (uint32_t)p->long_units.size();
+            a.prog = p->d_counters + p->prog64_base;
+            a.xbnd = p->d_bnd64 + (size_t)p->long_grid * (rescue_long_waves() + 1) * p->bnd64_elems;
             HIPCHK(launch_rescue_long(p->any_inter, p->long_grid, p->rescue_lds, st, a));
             launches++;
         }

@@ -51,6 +51,19 @@ struct alignas(16) ChainHap {
     uint32_t pad[3];
 };
 
+// A unit of a read longer than one strip (the long-read kernels): strips
+// [s0, s1) of pair `pair`, one strip per wave of a workgroup.  The units of
+// a pair are consecutive in their list, in strip order.  A unit that does
+// not end at the pair's last strip hands its last strip's boundary row to
+// the next unit through slot xout (a boundary-row buffer and a progress
+// counter); xin is the slot of the unit before (kNoSlot: none).
+constexpr uint32_t kNoSlot = 0xffffffffu;
+struct alignas(16) LongUnit {
+    int32_t pair;
+    uint32_t strips;  // s0 | s1 << 16
+    uint32_t xin, xout;
+};
+
 // Arguments of every forward launch (passed by value).
 struct FwdArgs {
     const DevTables *tab;
@@ -92,6 +105,12 @@ struct FwdArgs {
     uint64_t bnd_elems;  // elements of T per buffer; 2 buffers per group
     uint32_t lds_stride; // LDS bytes per group
     uint32_t max_strips;
+    // long-read kernels: units (null: one whole pair per item of `items`),
+    // per slot a progress counter (blocks of the slot's strip written) and
+    // a boundary row of bnd_elems elements
+    const LongUnit *units;
+    uint32_t *prog;
+    void *xbnd;
 };
 
 // Flag bit used between the passes of a plan: the FP64 regroup of

@@ -927,24 +927,51 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
         process_item<double, 64, 4, INTER, true>(a, lds, p);
 }
 
-// ------------------------------------------------- long reads, FP64
-// A rescued pair longer than one strip of 252 rows, its strips pipelined
-// over the kLW waves of a workgroup: wave w takes strips w, w + kLW, ...
-// (64 lanes x 4 rows, lane 63 the identity lane / feeder, as in
-// process_item).  The waves advance in blocks of kLB steps with one
-// workgroup barrier per block.  Strip s starts at block
-// (s / kLW) * P + 2 * (s % kLW), P = max(blocks per strip, 2 * kLW): a
-// wave's strips do not overlap, and every strip starts at least two blocks
-// after the strip above it.  The boundary row strip s+1 reads at its step d
-// (column d + 1 of strip s's last row) is written by strip s at its step
-// d + L - 1 <= d + kLB - 1, so in an earlier block, before the barrier that
-// precedes its use.  Every wave runs the same number of blocks, so the
-// barrier count is uniform.  Boundary rows go through kLW + 1 HBM buffers
-// per workgroup (strip s writes buffer s % (kLW + 1)); a buffer is written
-// again only by strip s + kLW + 1, which starts once strip s + 1, its
-// reader, has finished.
+// ------------------------------------------------- long reads
+// Strips [s0, s1) of a pair longer than one strip (64 lanes x MR rows, lane
+// 63 the identity lane / feeder, as in process_item), pipelined over the
+// kLW waves of a workgroup: wave w takes strips s0 + w, s0 + w + kLW, ...
+// The waves advance in blocks of kLB steps with one workgroup barrier per
+// block.  Strip s0 + c starts at block (c / kLW) * P + 2 * (c % kLW),
+// P = max(blocks per strip, 2 * kLW): a wave's strips do not overlap, and
+// every strip starts at least two blocks after the strip above it.  The
+// boundary row strip s+1 reads at its step d (column d + 1 of strip s's
+// last row) is written by strip s at its step d + L - 1 <= d + kLB - 1, so
+// in an earlier block, before the barrier that precedes its use.  Every
+// wave runs the same number of blocks, so the barrier count is uniform.
+// Boundary rows go through kLW + 1 HBM buffers per workgroup (strip s0 + c
+// writes buffer c % (kLW + 1)); a buffer is written again only by strip
+// c + kLW + 1, which starts once strip c + 1, its reader, has finished.
+//
+// A pair split into units of at most kLW strips runs its units on several
+// workgroups at once.  The last strip of a unit hands its boundary row to
+// the next unit's first strip through a slot: an HBM row and a counter of
+// the blocks written, published with release order after each block.  The
+// next unit's first strip waits, at the start of its block b, until
+// min(b + 2, blocks per strip) blocks are published -- the two-block lag of
+// the strips inside a workgroup.  Units are taken from a work counter in
+// list order, so the unit a workgroup waits for was taken earlier by a
+// running workgroup, and the wait ends.  It is bounded all the same
+// (kSpinLimit polls): if the bound runs out, NaN goes into the strip and
+// from there into the pair's result, and the wait is not repeated.
 constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
 constexpr int kLB = 64;  // steps per block (>= L - 1)
+constexpr uint32_t kSpinLimit = 1u << 20;  // polls of ~0.5-1 us
+
+// Waits until *c >= need (the slot's blocks published); false if the poll
+// bound ran out.  Wave-uniform.
+__device__ __forceinline__ bool wait_published(const uint32_t *c, uint32_t need) {
+    for (uint32_t k = 0; k < kSpinLimit; k++) {
+        const uint32_t v = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (v >= need) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the row's stores before the count
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(8);
+    }
+    return false;
+}
 
 // Lane k's value, for every lane (k uniform).
 __device__ __forceinline__ float lane_value(float v, int k) {
@@ -956,7 +983,8 @@ __device__ __forceinline__ double lane_value(double v, int k) {
 }
 
 template <typename T, int MR, bool INTER, bool RESCUE>
-__device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
+__device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1, uint32_t xin,
+                          uint32_t xout) {
     constexpr int L = 64;
     constexpr int C = (L - 1) * MR;
     static_assert(kLB >= L - 1, "a block covers the boundary row's lag");
@@ -989,10 +1017,15 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
     __syncthreads();
     const uint8_t *cp = lds + (L - 1) - l;
 
+    if (s1 > nstr) s1 = nstr;
+    const int nu = s1 - s0;  // strips of this unit
     const int nb = (H + 2 * L - 1 + kLB - 1) / kLB;  // blocks of a strip (steps 1 .. H + 2L - 1)
     const int P0 = nb > 2 * kLW ? nb : 2 * kLW;
-    const int nblk = ((nstr - 1) / kLW) * P0 + 2 * ((nstr - 1) % kLW) + nb;
+    const int nblk = ((nu - 1) / kLW) * P0 + 2 * ((nu - 1) % kLW) + nb;
     T *bbase = reinterpret_cast<T *>(a.bnd) + (uint64_t)blockIdx.x * (kLW + 1) * a.bnd_elems;
+    T *xb_in = xin != kNoSlot ? reinterpret_cast<T *>(a.xbnd) + (uint64_t)xin * a.bnd_elems : nullptr;
+    T *xb_out = xout != kNoSlot ? reinterpret_cast<T *>(a.xbnd) + (uint64_t)xout * a.bnd_elems : nullptr;
+    bool xlive = true;  // the waits for the unit above have not run out
 
     const LaneRot<L> rot;
     const bool feeder = l == L - 1;
@@ -1002,10 +1035,16 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
     T *bnd_in = nullptr, *bnd_out = nullptr;
     bool inject = false, store = false, last = false;
     int Dend = 0, d = 1;
-    int cs = w;  // this wave's current strip
+    int cr = w;  // this wave's current strip, s0 + cr
     for (int t = 0; t < nblk; t++) {
-        const int st0 = (cs / kLW) * P0 + 2 * (cs % kLW);
-        if (cs < nstr && t >= st0 && t < st0 + nb) {
+        const int st0 = (cr / kLW) * P0 + 2 * (cr % kLW);
+        bool pub = false;  // this block of the unit's last strip goes to slot xout
+        if (cr < nu && t >= st0 && t < st0 + nb) {
+            const int cs = s0 + cr;
+            if (cr == 0 && xb_in && xlive) {  // the unit above's blocks up to this one's lag
+                const int need = t + 2 < nb ? t + 2 : nb;
+                xlive = wait_published(a.prog + xin, (uint32_t)need);
+            }
             if (t == st0) {  // the strip's set-up, as process_item's
 #pragma unroll
                 for (int s = 0; s < MR; s++) {
@@ -1030,8 +1069,9 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
                 last = cs == nstr - 1;
                 inject = cs > 0;
                 store = !last;
-                bnd_out = bbase + (uint64_t)(cs % (kLW + 1)) * a.bnd_elems;
-                bnd_in = bbase + (uint64_t)((cs + kLW) % (kLW + 1)) * a.bnd_elems;  // strip cs - 1's
+                bnd_out = cr == nu - 1 ? xb_out : bbase + (uint64_t)(cr % (kLW + 1)) * a.bnd_elems;
+                bnd_in = cr == 0 ? xb_in  // strip cs - 1's
+                                 : bbase + (uint64_t)((cr + kLW) % (kLW + 1)) * a.bnd_elems;
                 if (inject) {
                     T bm = ld_agent(bnd_in + 3 * (1 + L)), bi = ld_agent(bnd_in + 3 * (1 + L) + 1),
                       bd = ld_agent(bnd_in + 3 * (1 + L) + 2);
@@ -1059,6 +1099,7 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
                 fi = ld_agent(in + 1);
                 fd = ld_agent(in + 2);
             }
+            if (!xlive && cr == 0) fm = T(__builtin_nan(""));
             for (; d <= dlim; d++) {
                 step<T, L, MR, INTER>(S, P, yyA, myA, cp[d], rot);
                 if (last && d - l <= H) {
@@ -1097,45 +1138,47 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p) {
                         a.flags[p] = fl;
                     }
                 }
-                cs += kLW;
+                cr += kLW;
             }
+            pub = bnd_out == xb_out && store && xb_out;
         }
         __threadfence();  // this block's boundary stores before the barrier
+        if (pub && l == L - 2)  // the storing lane: the blocks of the strip written so far
+            __hip_atomic_store(a.prog + xout, (uint32_t)(t - st0 + 1), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
     }
 }
 
-// Persistent over the rescue list; takes the pairs longer than one strip
-// (rescue_pair leaves them), one pair per workgroup at a time.  ROUTE: per
-// pair inter or intra FP64 as the regroup decided.
+// Persistent over the units of the pairs longer than one strip (rescue_pair
+// leaves them), one unit per workgroup at a time; a unit whose pair the
+// FP32 pass did not reject is skipped, like every other unit of its pair.
+// ROUTE: per pair inter or intra FP64 as the regroup decided.
 template <bool ROUTE>
 __global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t s_it;
-    const uint32_t n = *a.n_items_dev;
+    const uint32_t n = a.n_items;
     for (uint32_t iter = 0; iter <= n; iter++) {
-        if (threadIdx.x == 0) {
-            uint32_t it = atomicAdd(a.counter, 1u);
-            while (it < n && (int)a.read_len[a.pair_read[a.rescue_list[it]]] <= kRescueStrip)
-                it = atomicAdd(a.counter, 1u);
-            s_it = it;
-        }
+        if (threadIdx.x == 0) s_it = atomicAdd(a.counter, 1u);
         __syncthreads();
         const uint32_t it = s_it;
         __syncthreads();  // every wave has read s_it before thread 0 writes it again
         if (it >= n) break;
-        const int p = (int)a.rescue_list[it];
-        const bool inter = ROUTE && (a.flags[p] & kFlagGrouped64);
-        if (__builtin_amdgcn_readfirstlane(inter ? 1 : 0))
-            long_pair<double, 4, true, true>(a, lds, p);
+        const LongUnit u = a.units[it];
+        const uint8_t fl = a.flags[u.pair];
+        if (!(fl & 0x01u)) continue;  // RESCUED
+        const int s0 = (int)(u.strips & 0xffffu), s1 = (int)(u.strips >> 16);
+        if (__builtin_amdgcn_readfirstlane((ROUTE && (fl & kFlagGrouped64)) ? 1 : 0))
+            long_pair<double, 4, true, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
         else
-            long_pair<double, 4, false, true>(a, lds, p);
+            long_pair<double, 4, false, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
     }
 }
 
 // The FP32 pass of a 64-lane select-prior bucket whose reads need more than
-// one strip: one pair per item, its strips pipelined over the workgroup's
-// waves as above.
+// one strip, the pairs' strips pipelined over the workgroup's waves as
+// above: over a.units when given, else one whole pair per item.
 template <int MR, bool INTER>
 __global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
@@ -1147,8 +1190,19 @@ __global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
         const uint32_t it = s_it;
         __syncthreads();  // every wave has read s_it before thread 0 writes it again
         if (it >= n) break;
-        const int p = a.items[it];
-        if (p >= 0) long_pair<float, MR, INTER, false>(a, lds, p);
+        int p, s0 = 0, s1 = 1 << 16;
+        uint32_t xin = kNoSlot, xout = kNoSlot;
+        if (a.units) {
+            const LongUnit u = a.units[it];
+            p = u.pair;
+            s0 = (int)(u.strips & 0xffffu);
+            s1 = (int)(u.strips >> 16);
+            xin = u.xin;
+            xout = u.xout;
+        } else {
+            p = a.items[it];
+        }
+        if (p >= 0) long_pair<float, MR, INTER, false>(a, lds, p, s0, s1, xin, xout);
     }
 }
 

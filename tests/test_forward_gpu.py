@@ -201,3 +201,48 @@ def test_allocation_failure_is_an_error_and_leaves_the_context_usable(engine, or
     finally:
         del os.environ["RODV_PHMM_FAIL_ALLOC"]
     check_against_oracle(oracle_mod, engine, b)
+
+
+def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
+    # Reads longer than one strip: batched launches split each pair into
+    # units of at most 8 strips that run on several workgroups at once and
+    # hand boundary rows over through HBM slots (NaN until written); plain
+    # launches run every pair whole in one workgroup.  Read lengths sit at
+    # the unit boundaries: FP64 strips of 252 rows (2,016 / 2,017 / 4,032 /
+    # 4,033 rows: 8 / 9 / 16 / 17 strips), FP32 strips of 1,008 rows at 16
+    # rows per lane (8,064 / 8,065).  Reads are taken from haplotype 0 with
+    # rare substitutions (FP32 results), and are unrelated to haplotype 1
+    # (rescued in FP64).
+    rng = np.random.default_rng(23)
+    lens = [253, 1100, 2016, 2017, 4032, 4033, 8064, 8065]
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    h0 = acgt[rng.integers(0, 4, 8400)]
+    h1 = acgt[rng.integers(0, 4, 3000)]
+    reads = []
+    for n in lens:
+        s = int(rng.integers(0, 8400 - n + 1))
+        r = h0[s:s + n].copy()
+        sub = rng.random(n) < 5e-4
+        r[sub] = acgt[(np.searchsorted(acgt, r[sub]) + 1) % 4]
+        reads.append(r.tobytes())
+    q = [rng.integers(20, 41, n) for n in lens]
+    ins = [rng.integers(35, 46, n) for n in lens]
+    dels = [rng.integers(35, 46, n) for n in lens]
+    gcp = [np.full(n, 10) for n in lens]
+    b = Batch.from_lists([h0.tobytes(), h1.tobytes()], reads, q, ins, dels, gcp)
+    out, fl, _ = check_against_oracle(oracle_mod, engine, b)  # a fresh plan, batched
+    n = len(lens)
+    assert not (fl[:n] & FLAG_RESCUED).any()
+    assert (fl[n:][np.array(lens) > 252] & FLAG_RESCUED).all()
+    plan = engine.plan(b, default_config(semantics=0))
+    plan.execute_plain()
+    ref = plan.read(raw=True)
+    resc = (ref[1] & FLAG_RESCUED) != 0
+    for _ in range(2):
+        plan.execute()
+        got = plan.read(raw=True)
+        np.testing.assert_array_equal(got[2].view(np.uint32), ref[2].view(np.uint32))
+        np.testing.assert_array_equal(got[3][resc].view(np.uint64), ref[3][resc].view(np.uint64))
+        np.testing.assert_array_equal(got[1], ref[1])
+        np.testing.assert_array_equal(got[0], ref[0])
+    plan.close()
