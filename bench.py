@@ -318,10 +318,14 @@ def main():
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "kernel": ("fwd_chain_kernel + fwd_tab_kernel <float> (all length buckets, "
-                                "one FP32 pass)" if args.haplotype_chains else
-                                "fwd_tab_kernel<float> (all length buckets, one FP32 pass)")
-                     if args.prior_table else "fwd_kernel<float> (all length buckets)",
+                     "kernel": (("fwd_chain_kernel + fwd_tab_kernel <float> (all length buckets, "
+                                 "one FP32 pass)" if args.haplotype_chains else
+                                 "fwd_tab_kernel<float> (all length buckets, one FP32 pass)")
+                                if args.prior_table else "fwd_kernel<float> (all length buckets)")
+                     # reads beyond one 64-lane table strip (1,023 rows) take
+                     # the strip-pipelined long-read kernel
+                     + (" + fwd_long_kernel<float> (reads > 1,023 rows)"
+                        if int(batch.read_len.max()) > 1023 else ""),
                      "kernel_ms": round(fp32_ms, 3),
                      "valu_issue_ceiling": ceiling},
     }
