@@ -155,7 +155,10 @@ int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
  * want inputs resident in HBM: plan_create validates, buckets and uploads;
  * plan_execute runs the batched launch on `stream` (a hipStream_t, NULL for
  * the default stream) without host synchronisation; plan_execute_plain runs
- * one launch per region instead; plan_read synchronises and copies results.
+ * one launch per region instead; plan_read synchronises and copies results
+ * (RODV_PHMM_ERR_DEVICE, after the copies, if the strips of a read longer
+ * than one strip, run on several workgroups, waited past their bound for
+ * one another; the results of such pairs are NaN).
  */
 int rodv_phmm_plan_create(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                           const rodv_phmm_config *cfg, rodv_phmm_plan **out);

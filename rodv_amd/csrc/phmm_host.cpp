@@ -132,6 +132,14 @@ bool fail_alloc_hook() {
     return e && e[0] == '1';
 }
 
+// Polls of one hand-over wait of the long-read kernels: 2^20 (about a
+// second; the waits take microseconds to milliseconds).  Tests force the
+// bound to run out with RODV_PHMM_SPIN_LIMIT=0, read when a plan is created.
+uint32_t spin_limit_hook() {
+    const char *e = std::getenv("RODV_PHMM_SPIN_LIMIT");
+    return e && e[0] ? (uint32_t)std::strtoul(e, nullptr, 10) : (1u << 20);
+}
+
 template <typename T>
 int dev_alloc(T **p, size_t n) {
     *p = nullptr;
@@ -310,6 +318,8 @@ struct rodv_phmm_plan {
     LongUnit *d_long_units = nullptr;
     uint32_t n_x64 = 0;
     uint32_t prog64_base = 0;
+    uint32_t fault_counter = 0;  // hand-over waits of the long-read kernels that ran out
+    uint32_t spin_limit = 1u << 20;
     uint32_t rescue_lds = 0;
     // host copies needed at launch
     std::vector<Bucket> buckets;
@@ -1259,8 +1269,10 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         }
         if (!rc && !bk.units.empty()) rc = upload(&bk.d_units, bk.units.data(), bk.units.size());
     }
-    p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size() + 1;
-    p->long_counter = p->n_counters - 1;
+    p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size() + 2;
+    p->long_counter = p->n_counters - 2;
+    p->fault_counter = p->n_counters - 1;
+    p->spin_limit = spin_limit_hook();
     for (auto &bk : p->buckets) {  // progress counters of the long-read slots
         bk.prog_base = p->n_counters;
         p->n_counters += bk.n_xslots;
@@ -1412,6 +1424,8 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                     a.units = bk.d_units;
                     a.n_items = (uint32_t)bk.units.size();
                     a.prog = p->d_counters + bk.prog_base;
+                    a.fault = p->d_counters + p->fault_counter;
+                    a.spin_limit = p->spin_limit;
                     a.xbnd = bk.d_bnd + (size_t)bk.grid * bk.nbnd * bk.bnd_elems;
                 }
                 // chained items: G prior tables and two code buffers per wave
@@ -1476,6 +1490,8 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
             a.units = p->d_long_units;
             a.n_items = (uint32_t)p->long_units.size();
             a.prog = p->d_counters + p->prog64_base;
+            a.fault = p->d_counters + p->fault_counter;
+            a.spin_limit = p->spin_limit;
             a.xbnd = p->d_bnd64 + (size_t)p->long_grid * (rescue_long_waves() + 1) * p->bnd64_elems;
             HIPCHK(launch_rescue_long(p->any_inter, p->long_grid, p->rescue_lds, st, a));
             launches++;
@@ -1791,9 +1807,13 @@ static int plan_read_impl(rodv_phmm_plan *p, double *out_log10, uint8_t *out_fla
     if (out_flags) HIPCHK(hipMemcpy(out_flags, p->d_flags, n, hipMemcpyDeviceToHost));
     if (out_raw32) HIPCHK(hipMemcpy(out_raw32, p->d_raw32, n * sizeof(float), hipMemcpyDeviceToHost));
     if (out_raw64) HIPCHK(hipMemcpy(out_raw64, p->d_raw64, n * sizeof(double), hipMemcpyDeviceToHost));
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, faults = 0;
     HIPCHK(hipMemcpy(&cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&faults, p->d_counters + p->fault_counter, sizeof(faults), hipMemcpyDeviceToHost));
     p->stats.rescued_pairs = cnt;
+    // a long read's units waited past their bound for one another: those
+    // pairs' results are NaN
+    if (faults) return RODV_PHMM_ERR_DEVICE;
     return RODV_PHMM_OK;
 }
 

@@ -111,6 +111,8 @@ struct FwdArgs {
     const LongUnit *units;
     uint32_t *prog;
     void *xbnd;
+    uint32_t *fault;  // counts the hand-over waits that ran out their bound
+    uint32_t spin_limit;  // polls of one hand-over wait
 };
 
 // Flag bit used between the passes of a plan: the FP64 regroup of

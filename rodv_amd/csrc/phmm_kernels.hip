@@ -952,16 +952,17 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
 // the strips inside a workgroup.  Units are taken from a work counter in
 // list order, so the unit a workgroup waits for was taken earlier by a
 // running workgroup, and the wait ends.  It is bounded all the same
-// (kSpinLimit polls): if the bound runs out, NaN goes into the strip and
-// from there into the pair's result, and the wait is not repeated.
+// (a.spin_limit polls): if the bound runs out, the unit counts it in
+// *a.fault (the plan's read then reports a device error), NaN goes into the
+// strip and from there into the pair's result, and the wait is not
+// repeated.
 constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
 constexpr int kLB = 64;  // steps per block (>= L - 1)
-constexpr uint32_t kSpinLimit = 1u << 20;  // polls of ~0.5-1 us
 
-// Waits until *c >= need (the slot's blocks published); false if the poll
-// bound ran out.  Wave-uniform.
-__device__ __forceinline__ bool wait_published(const uint32_t *c, uint32_t need) {
-    for (uint32_t k = 0; k < kSpinLimit; k++) {
+// Waits until *c >= need (the slot's blocks published); false if `limit`
+// polls (of ~0.5-1 us each) did not see it.  Wave-uniform.
+__device__ __forceinline__ bool wait_published(const uint32_t *c, uint32_t need, uint32_t limit) {
+    for (uint32_t k = 0; k < limit; k++) {
         const uint32_t v = __builtin_amdgcn_readfirstlane(
             __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (v >= need) {
@@ -1043,7 +1044,8 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
             const int cs = s0 + cr;
             if (cr == 0 && xb_in && xlive) {  // the unit above's blocks up to this one's lag
                 const int need = t + 2 < nb ? t + 2 : nb;
-                xlive = wait_published(a.prog + xin, (uint32_t)need);
+                xlive = wait_published(a.prog + xin, (uint32_t)need, a.spin_limit);
+                if (!xlive && l == 0) atomicAdd(a.fault, 1u);
             }
             if (t == st0) {  // the strip's set-up, as process_item's
 #pragma unroll

@@ -203,6 +203,38 @@ def test_allocation_failure_is_an_error_and_leaves_the_context_usable(engine, or
     check_against_oracle(oracle_mod, engine, b)
 
 
+def test_long_read_wait_bound_is_an_error(engine, oracle_mod):
+    # The waits between the units of a long read are bounded.  Forced to run
+    # out from the host (RODV_PHMM_SPIN_LIMIT=0: no poll, so every unit that
+    # follows another proceeds at once, with the NaN-filled hand-over rows of
+    # a fresh plan), the plan's read reports a device error after copying the
+    # results, the pairs whose FP64 rescue runs in several units (reads of
+    # more than 2,016 rows) are NaN and the others are not, and the same
+    # context scores normally afterwards.
+    from rodv_amd.engine import ERR_DEVICE, _ptr
+    rng = np.random.default_rng(31)
+    lens = [100, 300, 2017, 4033]
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    hap = acgt[rng.integers(0, 4, 3000)].tobytes()
+    reads = [acgt[rng.integers(0, 4, n)].tobytes() for n in lens]  # unrelated: all rescued
+    quals = [rng.integers(20, 41, n) for n in lens]
+    b = Batch.from_lists([hap], reads, quals, quals, quals, [np.full(n, 10) for n in lens])
+    multi = np.array(lens) > 2016
+    os.environ["RODV_PHMM_SPIN_LIMIT"] = "0"
+    try:
+        plan = engine.plan(b, default_config(semantics=0))
+    finally:
+        del os.environ["RODV_PHMM_SPIN_LIMIT"]
+    plan.execute()
+    out = np.zeros(plan.num_pairs)
+    fl = np.zeros(plan.num_pairs, np.uint8)
+    rc = plan._lib.rodv_phmm_plan_read(plan._h, _ptr(out), _ptr(fl), None, None)
+    plan.close()
+    assert rc == ERR_DEVICE
+    assert np.isnan(out[multi]).all() and np.isfinite(out[~multi]).all()
+    check_against_oracle(oracle_mod, engine, b)
+
+
 def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
     # Reads longer than one strip: batched launches split each pair into
     # units of at most 8 strips that run on several workgroups at once and
