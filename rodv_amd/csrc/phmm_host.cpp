@@ -27,6 +27,7 @@ uint32_t fwd_tab_bytes(int L, int MR);
 int fwd_f64_rescue_mr();
 int rescue_long_waves();
 int rescue_long_rows();
+int rescue_long_mr();
 hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 void *fwd_long_kernel_ptr(int MR, bool inter);
 hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
@@ -1277,8 +1278,8 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         bk.prog_base = p->n_counters;
         p->n_counters += bk.n_xslots;
     }
-    // FP64 rescue: one pair per wave, two waves per SIMD (its ~180 VGPRs
-    // allow no more); boundary rows only if some read > 252
+    // FP64 rescue: one pair per wave, two waves per SIMD (its ~240 VGPRs
+    // allow no more); boundary rows only if some read > 378 (one strip)
     const int wpc64 = 8;
     p->rescue_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n_pairs, (uint64_t)ctx->num_cus * wpc64));
     uint32_t max_h_all = 0;
@@ -1295,7 +1296,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         for (uint64_t k = 0; k < n_pairs; k++) {
             const uint32_t R = b->read_lengths[pr[k]], H = b->hap_lengths[ph[k]];
             if (R <= (uint32_t)rescue_long_rows()) continue;
-            const uint64_t ns = (uint64_t)strips_for(R, 64, 4);
+            const uint64_t ns = (uint64_t)strips_for(R, 64, rescue_long_mr());
             lp.push_back({((uint64_t)H + 127) / 64 + 2 * (ns - 1), (uint32_t)k});
         }
         std::stable_sort(lp.begin(), lp.end(), [](const std::pair<uint64_t, uint32_t> &x,
@@ -1304,7 +1305,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         });
         for (const auto &e : lp)
             if (!rc && !append_units(p->long_units, p->n_x64, (int32_t)e.second,
-                                     strips_for(b->read_lengths[pr[e.second]], 64, 4)))
+                                     strips_for(b->read_lengths[pr[e.second]], 64, rescue_long_mr())))
                 rc = RODV_PHMM_ERR_INVALID_ARG;
         p->prog64_base = p->n_counters;
         p->n_counters += p->n_x64;

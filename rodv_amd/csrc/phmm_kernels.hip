@@ -910,10 +910,10 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
 }
 
 // One rescued pair in FP64 with as few rows per lane as its read needs: one
-// strip of 63*MR rows for MR = 1..4.  Fewer rows per lane leave fewer
+// strip of 63*MR rows for MR = 1..6.  Fewer rows per lane leave fewer
 // identity rows to compute; the arithmetic of every read row is the same.
 // Longer reads are fwd_rescue_long_kernel's.
-constexpr int kRescueStrip = 63 * 4;
+constexpr int kRescueStrip = 63 * 6;
 template <bool INTER>
 __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int p) {
     const int R = __builtin_amdgcn_readfirstlane((int)a.read_len[a.pair_read[p]]);
@@ -923,8 +923,12 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
         process_item<double, 64, 2, INTER, true>(a, lds, p);
     else if (R <= 189)
         process_item<double, 64, 3, INTER, true>(a, lds, p);
-    else if (R <= kRescueStrip)
+    else if (R <= 252)
         process_item<double, 64, 4, INTER, true>(a, lds, p);
+    else if (R <= 315)
+        process_item<double, 64, 5, INTER, true>(a, lds, p);
+    else if (R <= kRescueStrip)
+        process_item<double, 64, 6, INTER, true>(a, lds, p);
 }
 
 // ------------------------------------------------- long reads
@@ -957,6 +961,8 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
 // strip and from there into the pair's result, and the wait is not
 // repeated.
 constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
+// rows per lane of the FP64 long-read rescue (strips of 63 * 6 = 378 rows)
+constexpr int kLongRescueMR = 6;
 constexpr int kLB = 64;  // steps per block (>= L - 1)
 
 // Waits until *c >= need (the slot's blocks published); false if `limit`
@@ -1172,9 +1178,9 @@ __global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
         if (!(fl & 0x01u)) continue;  // RESCUED
         const int s0 = (int)(u.strips & 0xffffu), s1 = (int)(u.strips >> 16);
         if (__builtin_amdgcn_readfirstlane((ROUTE && (fl & kFlagGrouped64)) ? 1 : 0))
-            long_pair<double, 4, true, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
+            long_pair<double, kLongRescueMR, true, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
         else
-            long_pair<double, 4, false, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
+            long_pair<double, kLongRescueMR, false, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
     }
 }
 
@@ -1465,6 +1471,7 @@ hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, h
     return hipLaunchKernel(kern, dim3(grid), dim3(block), args, lds_bytes, st);
 }
 int rescue_long_rows() { return kRescueStrip; }
+int rescue_long_mr() { return kLongRescueMR; }
 hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
     void *kern = route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true>)
                        : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false>);

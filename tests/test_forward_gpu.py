@@ -209,17 +209,17 @@ def test_long_read_wait_bound_is_an_error(engine, oracle_mod):
     # follows another proceeds at once, with the NaN-filled hand-over rows of
     # a fresh plan), the plan's read reports a device error after copying the
     # results, the pairs whose FP64 rescue runs in several units (reads of
-    # more than 2,016 rows) are NaN and the others are not, and the same
-    # context scores normally afterwards.
+    # more than 3,024 rows: 8 strips of 378) are NaN and the others are not,
+    # and the same context scores normally afterwards.
     from rodv_amd.engine import ERR_DEVICE, _ptr
     rng = np.random.default_rng(31)
-    lens = [100, 300, 2017, 4033]
+    lens = [100, 500, 3025, 4033]
     acgt = np.frombuffer(b"ACGT", np.uint8)
     hap = acgt[rng.integers(0, 4, 3000)].tobytes()
     reads = [acgt[rng.integers(0, 4, n)].tobytes() for n in lens]  # unrelated: all rescued
     quals = [rng.integers(20, 41, n) for n in lens]
     b = Batch.from_lists([hap], reads, quals, quals, quals, [np.full(n, 10) for n in lens])
-    multi = np.array(lens) > 2016
+    multi = np.array(lens) > 3024
     os.environ["RODV_PHMM_SPIN_LIMIT"] = "0"
     try:
         plan = engine.plan(b, default_config(semantics=0))
@@ -240,13 +240,13 @@ def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
     # units of at most 8 strips that run on several workgroups at once and
     # hand boundary rows over through HBM slots (NaN until written); plain
     # launches run every pair whole in one workgroup.  Read lengths sit at
-    # the unit boundaries: FP64 strips of 252 rows (2,016 / 2,017 / 4,032 /
-    # 4,033 rows: 8 / 9 / 16 / 17 strips), FP32 strips of 1,008 rows at 16
+    # the unit boundaries: FP64 strips of 378 rows (3,024 / 3,025 / 6,048 /
+    # 6,049 rows: 8 / 9 / 16 / 17 strips), FP32 strips of 1,008 rows at 16
     # rows per lane (8,064 / 8,065).  Reads are taken from haplotype 0 with
     # rare substitutions (FP32 results), and are unrelated to haplotype 1
     # (rescued in FP64).
     rng = np.random.default_rng(23)
-    lens = [253, 1100, 2016, 2017, 4032, 4033, 8064, 8065]
+    lens = [379, 1100, 3024, 3025, 6048, 6049, 8064, 8065]
     acgt = np.frombuffer(b"ACGT", np.uint8)
     h0 = acgt[rng.integers(0, 4, 8400)]
     h1 = acgt[rng.integers(0, 4, 3000)]
@@ -265,7 +265,7 @@ def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
     out, fl, _ = check_against_oracle(oracle_mod, engine, b)  # a fresh plan, batched
     n = len(lens)
     assert not (fl[:n] & FLAG_RESCUED).any()
-    assert (fl[n:][np.array(lens) > 252] & FLAG_RESCUED).all()
+    assert (fl[n:][np.array(lens) > 378] & FLAG_RESCUED).all()
     plan = engine.plan(b, default_config(semantics=0))
     plan.execute_plain()
     ref = plan.read(raw=True)

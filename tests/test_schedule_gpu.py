@@ -155,14 +155,14 @@ def test_random_shapes_tunings_and_widths(engine, oracle_mod, seed):
 
 
 def test_long_reads_in_inter_fp64_groups(engine, oracle_mod):
-    # Reads longer than one FP64 strip (252 rows), unrelated to their
+    # Reads longer than one FP64 strip (378 rows), unrelated to their
     # haplotypes: the FP32 pass rejects them, the regroup puts windows of
     # them in inter FP64 groups, and the long-read rescue runs them with the
-    # inter arithmetic as units of at most 8 strips (reads over 2,016 rows
+    # inter arithmetic as units of at most 8 strips (reads over 3,024 rows
     # take more than one unit); batched and plain launches alike.
     rng = np.random.default_rng(29)
     regions = [
-        synth.region(rng, [1800, 2300], [2017, 2040, 2100, 1990, 700, 720, 760, 300],
+        synth.region(rng, [1800, 2300], [3030, 3060, 3100, 2990, 700, 720, 760, 300],
                      unrelated=range(8)),
         synth.region(rng, [900], [600, 610, 640, 650, 4100], unrelated=range(5)),
     ]
@@ -172,6 +172,6 @@ def test_long_reads_in_inter_fp64_groups(engine, oracle_mod):
     for mode in ("batched", "plain"):
         fl, route = check_schedule(engine, oracle_mod, b, wf=4, wd=4, idle_f=0.5, idle_d=0.5,
                                    mode=mode)
-        assert ((route == ROUTE_INTER64) & (R > 2016)).any()
-        assert ((route == ROUTE_INTER64) & (R > 252) & (R <= 2016)).any()
-        assert (fl[R > 252] & FLAG_RESCUED).all()
+        assert ((route == ROUTE_INTER64) & (R > 3024)).any()
+        assert ((route == ROUTE_INTER64) & (R > 378) & (R <= 3024)).any()
+        assert (fl[R > 378] & FLAG_RESCUED).all()
