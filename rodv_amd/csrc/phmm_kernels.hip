@@ -961,8 +961,8 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
 // strip and from there into the pair's result, and the wait is not
 // repeated.
 constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
-// rows per lane of the FP64 long-read rescue (strips of 63 * 6 = 378 rows)
-constexpr int kLongRescueMR = 6;
+// rows per lane of the FP64 long-read rescue (strips of 63 * 7 = 441 rows)
+constexpr int kLongRescueMR = 7;
 constexpr int kLB = 64;  // steps per block (>= L - 1)
 
 // Waits until *c >= need (the slot's blocks published); false if `limit`

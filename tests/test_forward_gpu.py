@@ -15,6 +15,8 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-4  # |d log10 L| bound of this path
+FP64_STRIP = 63 * 7  # rows of an FP64 strip of the long-read rescue (7 rows per lane)
+UNIT = 8 * FP64_STRIP  # rows of a unit of 8 strips
 
 
 def run(engine, batch, **cfg):
@@ -209,17 +211,17 @@ def test_long_read_wait_bound_is_an_error(engine, oracle_mod):
     # follows another proceeds at once, with the NaN-filled hand-over rows of
     # a fresh plan), the plan's read reports a device error after copying the
     # results, the pairs whose FP64 rescue runs in several units (reads of
-    # more than 3,024 rows: 8 strips of 378) are NaN and the others are not,
+    # more than 3,528 rows: 8 strips of 441) are NaN and the others are not,
     # and the same context scores normally afterwards.
     from rodv_amd.engine import ERR_DEVICE, _ptr
     rng = np.random.default_rng(31)
-    lens = [100, 500, 3025, 4033]
+    lens = [100, 500, UNIT + 1, UNIT + 505]
     acgt = np.frombuffer(b"ACGT", np.uint8)
     hap = acgt[rng.integers(0, 4, 3000)].tobytes()
     reads = [acgt[rng.integers(0, 4, n)].tobytes() for n in lens]  # unrelated: all rescued
     quals = [rng.integers(20, 41, n) for n in lens]
     b = Batch.from_lists([hap], reads, quals, quals, quals, [np.full(n, 10) for n in lens])
-    multi = np.array(lens) > 3024
+    multi = np.array(lens) > UNIT
     os.environ["RODV_PHMM_SPIN_LIMIT"] = "0"
     try:
         plan = engine.plan(b, default_config(semantics=0))
@@ -240,13 +242,13 @@ def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
     # units of at most 8 strips that run on several workgroups at once and
     # hand boundary rows over through HBM slots (NaN until written); plain
     # launches run every pair whole in one workgroup.  Read lengths sit at
-    # the unit boundaries: FP64 strips of 378 rows (3,024 / 3,025 / 6,048 /
-    # 6,049 rows: 8 / 9 / 16 / 17 strips), FP32 strips of 1,008 rows at 16
+    # the unit boundaries: FP64 strips of 441 rows (3,528 / 3,529 / 7,056 /
+    # 7,057 rows: 8 / 9 / 16 / 17 strips), FP32 strips of 1,008 rows at 16
     # rows per lane (8,064 / 8,065).  Reads are taken from haplotype 0 with
     # rare substitutions (FP32 results), and are unrelated to haplotype 1
     # (rescued in FP64).
     rng = np.random.default_rng(23)
-    lens = [379, 1100, 3024, 3025, 6048, 6049, 8064, 8065]
+    lens = [379, 1100, UNIT, UNIT + 1, 2 * UNIT, 2 * UNIT + 1, 8064, 8065]
     acgt = np.frombuffer(b"ACGT", np.uint8)
     h0 = acgt[rng.integers(0, 4, 8400)]
     h1 = acgt[rng.integers(0, 4, 3000)]

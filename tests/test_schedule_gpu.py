@@ -14,6 +14,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
 ROUTE_INTER32, ROUTE_INTER64 = 1, 2
+FP64_STRIP = 63 * 7  # rows of an FP64 strip of the long-read rescue (7 rows per lane)
+UNIT = 8 * FP64_STRIP  # rows of a unit of 8 strips
 
 
 def check_schedule(engine, oracle_mod, batch, wf=16, wd=8, idle_f=0.1, idle_d=0.1,
@@ -155,14 +157,14 @@ def test_random_shapes_tunings_and_widths(engine, oracle_mod, seed):
 
 
 def test_long_reads_in_inter_fp64_groups(engine, oracle_mod):
-    # Reads longer than one FP64 strip (378 rows), unrelated to their
+    # Reads longer than the one-wave FP64 rescue (378 rows), unrelated to their
     # haplotypes: the FP32 pass rejects them, the regroup puts windows of
     # them in inter FP64 groups, and the long-read rescue runs them with the
-    # inter arithmetic as units of at most 8 strips (reads over 3,024 rows
+    # inter arithmetic as units of at most 8 strips (reads over 3,528 rows
     # take more than one unit); batched and plain launches alike.
     rng = np.random.default_rng(29)
     regions = [
-        synth.region(rng, [1800, 2300], [3030, 3060, 3100, 2990, 700, 720, 760, 300],
+        synth.region(rng, [1800, 2300], [3540, 3560, 3600, 3490, 700, 720, 760, 300],
                      unrelated=range(8)),
         synth.region(rng, [900], [600, 610, 640, 650, 4100], unrelated=range(5)),
     ]
@@ -172,6 +174,6 @@ def test_long_reads_in_inter_fp64_groups(engine, oracle_mod):
     for mode in ("batched", "plain"):
         fl, route = check_schedule(engine, oracle_mod, b, wf=4, wd=4, idle_f=0.5, idle_d=0.5,
                                    mode=mode)
-        assert ((route == ROUTE_INTER64) & (R > 3024)).any()
-        assert ((route == ROUTE_INTER64) & (R > 378) & (R <= 3024)).any()
+        assert ((route == ROUTE_INTER64) & (R > UNIT)).any()
+        assert ((route == ROUTE_INTER64) & (R > 378) & (R <= UNIT)).any()
         assert (fl[R > 378] & FLAG_RESCUED).all()
