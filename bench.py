@@ -21,6 +21,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (the rescue pass)
 FLOP_PER_CELL = 12        # 8 mul + 4 add per forward cell (pairhmm_impl.cpp:266-289)
 FP_INSTR_PER_CELL = 8     # non-packed FP32 instructions of the bit-exact FMA split
 # adjoint, algorithmic work per cell: 12 forward + 24 reverse + 10 parameter
@@ -117,6 +118,58 @@ def cpu_baseline(batch, seconds=12.0, seed=7, all_core_seconds=6.0):
     return res
 
 
+def oracle_check(batch, out, fl, r32, cfg, semantics, n_regions=100, max_cells=2.5e10, seed=11):
+    """Outside the timed region: up to `n_regions` seeded random regions of
+    the timed run (at most `max_cells` cells) against the oracle's
+    restatement of the same semantics -- schedule_pairhmm routing with the
+    run's group widths and idle bounds, or intra::computeLikelihoods per
+    pair.  Reports the largest |d log10 L| over finite results and whether
+    the deciding FP32 sums, the rescue set and the -inf set are identical."""
+    import dataclasses
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from rodv_amd.engine import FLAG_INTER, FLAG_RESCUED, FLAG_UNDERFLOW
+    regs = batch.regions
+    npairs = regs[:, 1].astype(np.int64) * regs[:, 3].astype(np.int64)
+    start = np.concatenate(([0], np.cumsum(npairs)[:-1]))
+    sel, cells = [], 0
+    for k in np.random.default_rng(seed).permutation(len(regs)):
+        h0, m, r0, n = (int(x) for x in regs[k])
+        c = int(batch.hap_len[h0:h0 + m].sum()) * int(batch.read_len[r0:r0 + n].sum())
+        if sel and cells + c > max_cells:
+            continue
+        sel.append(int(k))
+        cells += c
+        if len(sel) >= n_regions:
+            break
+    sel = np.sort(np.array(sel))
+    sub = dataclasses.replace(batch, regions=regs[sel])
+    idx = np.concatenate([np.arange(start[k], start[k] + npairs[k]) for k in sel])
+    nt = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    if semantics == "schedule":
+        o_out, o_fl, o_r32, _ = O.schedule_batch(
+            sub, use_double=bool(cfg.use_double), idle_f=cfg.max_idle_ratio_float,
+            idle_d=cfg.max_idle_ratio_double, wf=cfg.simd_width_float, wd=cfg.simd_width_double,
+            nthreads=nt)
+        mask = FLAG_RESCUED | FLAG_UNDERFLOW | FLAG_INTER
+    else:
+        o_out, o_r32, _, o_fl = O.run_batch(sub, nthreads=nt)
+        mask = FLAG_RESCUED | FLAG_UNDERFLOW
+    dt = time.perf_counter() - t0
+    g_out, g_fl, g_r32 = out[idx], fl[idx], r32[idx]
+    fin = np.isfinite(o_out)
+    err = float(np.max(np.abs(g_out[fin] - o_out[fin]))) if fin.any() else 0.0
+    return {"regions": int(len(sel)), "pairs": int(len(idx)), "cells": int(cells),
+            "max_abs_dlog10": err,
+            "raw32_bit_exact": bool(np.array_equal(g_r32.view(np.uint32), o_r32.view(np.uint32))),
+            "rescue_set_equal": bool(np.array_equal(g_fl & FLAG_RESCUED, o_fl & FLAG_RESCUED)),
+            "flags_equal": bool(np.array_equal(g_fl & mask, o_fl & mask)),
+            "neg_inf_equal": bool(np.array_equal(np.isneginf(g_out), np.isneginf(o_out))),
+            "rescued": int(((o_fl & FLAG_RESCUED) != 0).sum()),
+            "semantics": semantics, "oracle_threads": nt, "oracle_seconds": round(dt, 1)}
+
+
 def valu_ceiling():
     """FP32 issue ceiling of this device (rodv_amd/libvalu_ceiling.so): lane
     operations per second of independent v_fma_f32, v_mul_f32 and of the
@@ -167,6 +220,9 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (reference for PMC byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--oracle-regions", type=int, default=100,
+                    help="random regions of the timed run checked against the oracle "
+                         "afterwards (0: no check)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this "
@@ -203,9 +259,10 @@ def main():
                 f"rows/lane cap {c['max_rows_per_lane']}, waves/CU {c['waves_per_cu']}: "
                 f"{c['gcups']} GCUPS (built-in {c['default_gcups']})")
     t0 = time.time()
-    plan = eng.plan(batch, default_config(use_graph=1, prior_table=args.prior_table,
-                                         waves_per_cu=args.waves_per_cu, semantics=sem,
-                                         haplotype_chains=args.haplotype_chains))
+    run_cfg = default_config(use_graph=1, prior_table=args.prior_table,
+                             waves_per_cu=args.waves_per_cu, semantics=sem,
+                             haplotype_chains=args.haplotype_chains)
+    plan = eng.plan(batch, run_cfg)
     log(f"plan + upload {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
@@ -244,9 +301,16 @@ def main():
     fp32_ms = min(p["fp32_kernel_ms"] for p in profs)
     fp64_ms = min(p["fp64_kernel_ms"] for p in profs)
     st = profs[-1]
-    out, fl = plan.read()
+    out, fl, r32, _ = plan.read(raw=True)
     rescued = int(st["rescued_pairs"])
-    log(f"fp32 kernels {fp32_ms:.2f} ms, fp64 rescue {fp64_ms:.2f} ms, rescued {rescued}")
+    rescued_cells = int(st["rescued_cells"])
+    log(f"fp32 kernels {fp32_ms:.2f} ms, fp64 rescue {fp64_ms:.2f} ms, rescued {rescued} "
+        f"({rescued_cells:.3e} cells)")
+    parity = None
+    if args.oracle_regions > 0:
+        t0 = time.time()
+        parity = oracle_check(batch, out, fl, r32, run_cfg, args.semantics, args.oracle_regions)
+        log(f"oracle check ({time.time() - t0:.1f}s): {parity}")
 
     # plain: one launch per region (and length bucket)
     plain_ms = None
@@ -280,6 +344,8 @@ def main():
         log(f"adjoint: {adj_ms:.2f} ms, {adj['gcups']} GCUPS")
 
     achieved = FLOP_PER_CELL * cells / (fp32_ms / 1e3) / 1e12
+    achieved_step = FLOP_PER_CELL * cells / (ms / 1e3) / 1e12
+    fp64_achieved = FLOP_PER_CELL * rescued_cells / (fp64_ms / 1e3) / 1e12 if fp64_ms > 0 else 0.0
     ceiling = valu_ceiling()
     if ceiling:
         cps = cells / (fp32_ms / 1e3)
@@ -311,13 +377,29 @@ def main():
         "adjoint": adj,
         "semantics": args.semantics,
         "autotune": tuning,
+        "parity": parity,
         "rescued_pairs": rescued,
+        "rescued_cells": rescued_cells,
         "rescued_fraction": round(rescued / max(1, batch.num_pairs()), 6),
         "fp64_rescue_ms": round(fp64_ms, 3),
         "median_replay_ms": round(median_ms, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                     "traffic_source": None if traffic is None else
+                     "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this configuration, scaled by a "
+                     "1 GiB device copy (a committed measurement, not this run's)",
+                     # the whole step (FP32 pass, FP64 regroup and rescue, log10)
+                     # over the driver-timed ms_per_step
+                     "achieved_step": round(achieved_step, 3),
+                     "frac_step": round(achieved_step / FP32_PEAK_TFLOPS, 4),
+                     "fp64": {"bound": "valu", "achieved": round(fp64_achieved, 3),
+                              "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                              "frac": round(fp64_achieved / FP64_PEAK_TFLOPS, 4),
+                              "cells": rescued_cells, "pairs": rescued,
+                              "kernel_ms": round(fp64_ms, 3),
+                              "kernel": "regroup_kernel + fwd_rescue(_route)_kernel + "
+                                        "fwd_rescue_long_kernel <double>"},
                      "kernel": (("fwd_chain_kernel + fwd_tab_kernel <float> (all length buckets, "
                                  "one FP32 pass)" if args.haplotype_chains else
                                  "fwd_tab_kernel<float> (all length buckets, one FP32 pass)")

@@ -120,7 +120,7 @@ typedef struct rodv_phmm_stats {
     uint64_t num_pairs;
     uint64_t cells;          /* sum over pairs of R*H                      */
     uint64_t rescued_pairs;  /* pairs that went through the FP64 pass      */
-    uint64_t rescued_cells;
+    uint64_t rescued_cells;  /* sum of R*H over them (rodv_phmm_plan_profile) */
     uint64_t launches;       /* kernel launches of the last execution       */
     uint32_t num_buckets;    /* FP32 kernel shapes used                     */
     double fp32_kernel_ms;   /* summed FP32 forward kernel time (profiled)  */

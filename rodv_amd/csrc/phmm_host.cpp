@@ -34,6 +34,7 @@ hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, h
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
 hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st);
+hipError_t launch_rescued_cells(const FwdArgs &a, unsigned long long *out, hipStream_t st);
 hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, uint64_t n,
                        hipStream_t st);
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st);
@@ -307,6 +308,7 @@ struct rodv_phmm_plan {
     uint8_t *d_flags = nullptr;
     uint32_t *d_rescue = nullptr;
     uint32_t *d_counters = nullptr;  // [0] rescue count, [1] rescue work, [2..] per launch
+    unsigned long long *d_rescued_cells = nullptr;  // plan_profile's sum of R*H over the rescue list
     uint32_t n_counters = 0;
     double *d_bnd64 = nullptr;  // boundary rows of the long-read FP64 rescue
     uint64_t bnd64_elems = 0;
@@ -355,7 +357,8 @@ void free_plan(rodv_phmm_plan *p) {
                     p->d_rcode, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
                     p->d_read_len, p->d_pair_hap, p->d_pair_read, p->d_pair_inter,
                     p->d_raw32, p->d_raw64, p->d_out, p->d_flags, p->d_rescue,
-                    p->d_counters, p->d_bnd64, p->d_sched_list, p->d_sched_off, p->d_sched_scratch};
+                    p->d_counters, p->d_bnd64, p->d_sched_list, p->d_sched_off, p->d_sched_scratch,
+                    p->d_rescued_cells};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &b : p->buckets) {
@@ -1320,6 +1323,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         if (!rc) rc = upload(&p->d_long_units, p->long_units.data(), p->long_units.size());
     }
     if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
+    if (!rc) rc = dev_alloc(&p->d_rescued_cells, 1);
     // descriptors of the table kernels' item slots (batched and plain order)
     if (!rc) {
         const FwdArgs a = base_args(p);
@@ -1585,6 +1589,14 @@ static int plan_profile_impl(rodv_phmm_plan *p, void *stream) {
         uint32_t cnt = 0;
         if (hipMemcpy(&cnt, p->d_counters, sizeof(cnt), hipMemcpyDeviceToHost) == hipSuccess)
             p->stats.rescued_pairs = cnt;
+        // the FP64 pass's cells: R * H summed over the rescue list
+        unsigned long long rc_cells = 0;
+        if (launch_reset(reinterpret_cast<uint32_t *>(p->d_rescued_cells), 2, st) != hipSuccess ||
+            launch_rescued_cells(base_args(p), p->d_rescued_cells, st) != hipSuccess ||
+            hipMemcpyAsync(&rc_cells, p->d_rescued_cells, sizeof(rc_cells), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            rc = RODV_PHMM_ERR_DEVICE;
+        p->stats.rescued_cells = rc_cells;
     }
     for (auto &e : ev) (void)hipEventDestroy(e);
     return rc;
@@ -1594,8 +1606,11 @@ int rodv_phmm_plan_profile(rodv_phmm_plan *p, void *stream) {
     return guarded([&] { return plan_profile_impl(p, stream); });
 }
 
+}  // extern "C"
+
 // ---------------------------------------------------------------- autotuner
 
+// (outside the extern "C" block: helpers of C++ linkage, not exported)
 namespace {
 constexpr uint32_t kMaxTunePairs = 131072;
 
@@ -1623,6 +1638,8 @@ int time_candidate(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sub, const rodv_ph
     return RODV_PHMM_OK;
 }
 }  // namespace
+
+extern "C" {
 
 static int autotune_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *sample,
                        const rodv_phmm_config *cfg_in, rodv_phmm_tuning *out) {

@@ -1344,6 +1344,22 @@ __global__ void reset_kernel(uint32_t *counters, uint32_t n) {
     if (k < n) counters[k] = 0;
 }
 
+// Sum of R * H over the rescue list (the FP64 pass's cells, for the plan's
+// statistics): grid-stride over the list, a wave sum, one atomic per wave.
+__global__ void rescued_cells_kernel(const uint32_t *list, const uint32_t *count,
+                                     const uint32_t *pair_hap, const uint32_t *pair_read,
+                                     const uint32_t *hap_len, const uint32_t *read_len,
+                                     unsigned long long *out) {
+    const uint32_t n = *count;
+    unsigned long long acc = 0;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t p = list[k];
+        acc += (unsigned long long)read_len[pair_read[p]] * hap_len[pair_hap[p]];
+    }
+    for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, acc);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ launchers
@@ -1533,6 +1549,12 @@ hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, ui
 hipError_t launch_reset(uint32_t *counters, uint32_t n, hipStream_t st) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(reset_kernel, dim3((n + 255) / 256), dim3(256), 0, st, counters, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rescued_cells(const FwdArgs &a, unsigned long long *out, hipStream_t st) {
+    hipLaunchKernelGGL(rescued_cells_kernel, dim3(256), dim3(256), 0, st, a.rescue_list, a.rescue_count,
+                       a.pair_hap, a.pair_read, a.hap_len, a.read_len, out);
     return hipGetLastError();
 }
 

@@ -3,12 +3,17 @@ the oracle's restatement of it (oracle/phmm_oracle_sched.cpp, pinned by
 test_schedule_oracle.py): the same pairs take the inter arithmetic, the FP32
 sums that decide the rescue are bit-identical, the same pairs are regrouped
 for inter FP64, and every log10 L agrees within 1e-4."""
+import dataclasses
+import os
+
 import numpy as np
 import pytest
 
 from rodv_amd import synth
-from rodv_amd.batch import Batch
+from rodv_amd.batch import Batch, parse_gkl_testdata
 from rodv_amd.engine import FLAG_INTER, FLAG_RESCUED, FLAG_UNDERFLOW, default_config
+
+from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +24,7 @@ UNIT = 8 * FP64_STRIP  # rows of a unit of 8 strips
 
 
 def check_schedule(engine, oracle_mod, batch, wf=16, wd=8, idle_f=0.1, idle_d=0.1,
-                   use_double=False, mode="batched", **extra):
+                   use_double=False, mode="batched", return_out=False, **extra):
     cfg = default_config(semantics=1, simd_width_float=wf, simd_width_double=wd,
                          max_idle_ratio_float=idle_f, max_idle_ratio_double=idle_d,
                          use_double=int(use_double), use_graph=1, **extra)
@@ -42,6 +47,8 @@ def check_schedule(engine, oracle_mod, batch, wf=16, wd=8, idle_f=0.1, idle_d=0.
     if fin.any():
         err = np.abs(out[fin] - o_out[fin])
         assert err.max() <= TOL, f"max |dlog10L| {err.max()}"
+    if return_out:
+        return fl, route, out
     return fl, route
 
 
@@ -92,10 +99,49 @@ def test_no_simd_widths_is_all_intra(engine, oracle_mod):
     assert not (fl & FLAG_INTER).any()
 
 
+@pytest.mark.parametrize("wf,wd", [(16, 8), (8, 4)])
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_configs_full_size_schedule(engine, oracle_mod, cfg, wf, wd):
+    # the single-region configurations at full size under the default
+    # semantics: equal read lengths, so every window of wf pairs sorted by
+    # haplotype length has idle ratio 0 and every pair takes the inter
+    # arithmetic (config 0: 512 pairs, config 1: 4,096)
+    b = synth.config(cfg)
+    fl, route = check_schedule(engine, oracle_mod, b, wf=wf, wd=wd)
+    assert (route == ROUTE_INTER32).all()
+    assert (fl & FLAG_INTER).all()
+
+
 def test_config2_sample_schedule(engine, oracle_mod):
-    b = synth.config(2, scale=0.002)
+    # 120 regions (1.2 % of the batched workload's 10,000), 315,193 pairs
+    b = synth.config(2, scale=0.012)
+    assert len(b.regions) >= 100
     fl, route = check_schedule(engine, oracle_mod, b)
     assert (route == ROUTE_INTER32).sum() > 0.5 * len(route)
+    assert (fl & FLAG_RESCUED).any()
+
+
+@pytest.mark.parametrize("wf,wd", [(16, 8), (8, 4)])
+def test_gkl_vectors_through_inter_groups(engine, oracle_mod, wf, wd):
+    # The 104 GKL vectors through the inter arithmetic, as the reference's
+    # ALLInterMatch tests run them (pairhmm_unittest.cpp:414-495): one region
+    # of all 104 haplotypes x 104 reads with idle bound 1.0, so every window
+    # of wf sorted pairs is accepted (10,816 pairs, a multiple of 16 and 8);
+    # the diagonal pairs are the vectors, within 1e-5 of their expected value.
+    # Most off-diagonal pairs are unrelated sequences whose FP32 sums the
+    # inter groups reject: they go through the FP64 regroup (inter FP64, or
+    # intra FP64 for a short last window).
+    b, exp = parse_gkl_testdata(os.path.join(GOLDEN, "pairhmm-testdata.txt"))
+    n = len(exp)
+    one = dataclasses.replace(b, regions=np.array([[0, n, 0, n]], np.uint32))
+    fl, route, out = check_schedule(engine, oracle_mod, one, wf=wf, wd=wd, idle_f=1.0, idle_d=1.0,
+                                    return_out=True)
+    assert (route != 0).all()  # every pair took an inter FP32 group
+    assert (route == ROUTE_INTER64).sum() > 1000
+    k = np.arange(n)
+    assert (route.reshape(n, n)[k, k] == ROUTE_INTER32).all()
+    diag = out.reshape(n, n)[k, k]
+    assert np.max(np.abs(diag - exp)) < 1e-5
 
 
 def test_plain_and_graph_agree_under_routing(engine, oracle_mod):
