@@ -12,6 +12,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -170,6 +171,45 @@ def oracle_check(batch, out, fl, r32, cfg, semantics, n_regions=100, max_cells=2
             "semantics": semantics, "oracle_threads": nt, "oracle_seconds": round(dt, 1)}
 
 
+def dropin_latency(reps=20):
+    """Per-call wall time of the drop-in as a reference caller sees it: the
+    C++ caller tests/dropin/schedule_driver (pairhmm::schedule::schedule_pairhmm
+    declared as the reference's header does, linked against the engine) on
+    the single-region configurations, `reps` calls in a row in one child
+    process (the first creates the engine's context; later calls reuse its
+    device buffers); beside it the reference's own intra kernel
+    (oracle/_ref, one host thread) on the same pairs."""
+    from rodv_amd import synth
+    sys.path.insert(0, os.path.join(ROOT, "tests", "dropin"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import driver as D
+    import oracle as O
+    ref = None
+    for isa in ("avx512", "avx2"):
+        try:
+            ref = O.RefLib(isa)
+            break
+        except (FileNotFoundError, RuntimeError, OSError):
+            continue
+    res = {}
+    for cfg in (0, 1):
+        b = synth.config(cfg)
+        ok, _, us = D.run(*D.batch_lists(b), reps=reps)
+        med = float(np.median(us[1:])) if len(us) > 1 else float(us[0])
+        e = {"ok": ok, "pairs": b.num_pairs(), "cells": b.cells(), "calls": len(us),
+             "call_us_median": round(med, 1), "call_us_first": round(float(us[0]), 1),
+             "gcups_per_call": round(b.cells() / (med / 1e6) / 1e9, 2)}
+        if ref is not None:
+            t0 = time.perf_counter()
+            ref.log10_batch(b, b.cross_pairs(), 1)
+            dt = time.perf_counter() - t0
+            e["reference_cpu_us"] = round(dt * 1e6, 1)
+            e["reference_cpu"] = (f"intra::computeLikelihoods{ref.isa.upper()} per pair, 1 thread, "
+                                  f"host {_cpu_model()}")
+        res[f"config{cfg}"] = e
+    return res
+
+
 def valu_ceiling():
     """FP32 issue ceiling of this device (rodv_amd/libvalu_ceiling.so): lane
     operations per second of independent v_fma_f32, v_mul_f32 and of the
@@ -220,6 +260,9 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (reference for PMC byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--dropin", type=int, default=1,
+                    help="time schedule_pairhmm per call through the C++ reference-style caller "
+                         "on the single-region configurations")
     ap.add_argument("--oracle-regions", type=int, default=100,
                     help="random regions of the timed run checked against the oracle "
                          "afterwards (0: no check)")
@@ -343,6 +386,15 @@ def main():
                "tflops": round(adj_tflops, 2), "frac_of_fp32_peak": round(adj_tflops / FP32_PEAK_TFLOPS, 4)}
         log(f"adjoint: {adj_ms:.2f} ms, {adj['gcups']} GCUPS")
 
+    dropin = None
+    if args.dropin:
+        t0 = time.time()
+        try:
+            dropin = dropin_latency()
+        except (OSError, RuntimeError, subprocess.SubprocessError) as e:
+            dropin = {"error": str(e)[-500:]}
+        log(f"drop-in schedule_pairhmm per call ({time.time() - t0:.1f}s): {dropin}")
+
     achieved = FLOP_PER_CELL * cells / (fp32_ms / 1e3) / 1e12
     achieved_step = FLOP_PER_CELL * cells / (ms / 1e3) / 1e12
     fp64_achieved = FLOP_PER_CELL * rescued_cells / (fp64_ms / 1e3) / 1e12 if fp64_ms > 0 else 0.0
@@ -375,6 +427,7 @@ def main():
         "plain_gcups": None if plain_ms is None else round(cells / (plain_ms / 1e3) / 1e9, 2),
         "batched_over_plain": None if plain_ms is None else round(plain_ms / ms, 3),
         "adjoint": adj,
+        "dropin_latency": dropin,
         "semantics": args.semantics,
         "autotune": tuning,
         "parity": parity,

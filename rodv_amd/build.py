@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librodv_phmm.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("phmm_kernels.hip", "phmm_adjoint.hip", "phmm_host.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("phmm_kernels.hip", "phmm_adjoint.hip", "phmm_host.cpp",
+                                                   "phmm_schedule.cpp")]
 # FP32 VALU issue-ceiling probe used by bench.py's roofline (not part of the engine)
 CEILING_LIB = os.path.join(HERE, "libvalu_ceiling.so")
 CEILING_SOURCES = [os.path.join(HERE, "csrc", "valu_ceiling.hip")]
@@ -24,12 +25,14 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # one-lane atomicAdd into a form that waits for the result right away, where
 # the table kernel fetches its next work item early and reads it late.
 # Unaligned access mode is off so that the table kernel's per-step byte loads
-# of haplotype codes are not merged into unaligned wider reads.
+# of haplotype codes are not merged into unaligned wider reads.  Hidden
+# visibility: the library exports the C ABI and schedule_pairhmm only.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero", "-fno-fast-math",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Xclang", "-target-feature", "-Xclang", "-unaligned-access-mode",
          "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
+         "-fvisibility=hidden", "-fvisibility-inlines-hidden",
          "-Wall", "-Wno-unused-function", f"-I{os.path.join(ROOT, 'include')}"]
 
 

@@ -14,7 +14,12 @@
 #include <thread>
 #include <vector>
 
+// the library is built with hidden visibility: the C ABI's declarations
+// are the only symbols it exports (with the schedule_pairhmm definition)
+#pragma GCC visibility push(default)
 #include "../../include/rodv_phmm.h"
+#pragma GCC visibility pop
+#include "phmm_arena.h"
 #include "phmm_internal.h"
 
 namespace rodv {
@@ -277,6 +282,9 @@ struct rodv_phmm_ctx {
     // that fork them from and join them to the launch stream
     hipStream_t side[kSideStreams] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
+    // rodv_phmm_forward's plans: device block and pinned mirror kept from
+    // call to call (grown on demand, freed with the context)
+    Arena fwd_arena;
 };
 
 struct rodv_phmm_plan {
@@ -337,6 +345,10 @@ struct rodv_phmm_plan {
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t last_stream = nullptr;
     rodv_phmm_stats stats{};
+    // buffers carved from this arena (nullptr: each one allocated)
+    Arena *arena = nullptr;
+    // no per-region launch list (plans of rodv_phmm_forward, run batched once)
+    bool no_plain = false;
 };
 
 namespace {
@@ -353,6 +365,10 @@ void free_plan(rodv_phmm_plan *p) {
     if (!p) return;
     if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->arena) {  // every buffer is arena memory, which its owner keeps
+        delete p;
+        return;
+    }
     void *bufs[] = {p->d_hapc, p->d_hapc_off, p->d_hap_len, p->d_initd_f, p->d_initd_d,
                     p->d_rcode, p->d_q, p->d_ins, p->d_del, p->d_gcp, p->d_read_off,
                     p->d_read_len, p->d_pair_hap, p->d_pair_read, p->d_pair_inter,
@@ -375,6 +391,49 @@ void free_plan(rodv_phmm_plan *p) {
     for (auto *d : p->d_plain_desc)
         if (d) (void)hipFree(d);
     delete p;
+}
+
+// A plan's device buffer of n elements: from its arena when it has one
+// (256-byte aligned; kArenaFull when the arena is too small), else its own
+// allocation.
+template <typename T>
+int plan_alloc(rodv_phmm_plan *p, T **out, size_t n) {
+    if (!p->arena) return dev_alloc(out, n);
+    *out = nullptr;
+    Arena &A = *p->arena;
+    const size_t bytes = ((n ? n : 1) * sizeof(T) + 255) & ~(size_t)255;
+    A.need += bytes;
+    if (A.used + bytes > A.cap) return kArenaFull;
+    *out = reinterpret_cast<T *>(A.dev + A.used);
+    A.used += bytes;
+    return RODV_PHMM_OK;
+}
+
+template <typename T>
+int plan_upload(rodv_phmm_plan *p, T **out, const T *src, size_t n) {
+    if (!p->arena) return upload(out, src, n);
+    int rc = plan_alloc(p, out, n);
+    if (rc) return rc;
+    if (n) std::memcpy(p->arena->host + (reinterpret_cast<uint8_t *>(*out) - p->arena->dev), src, n * sizeof(T));
+    return RODV_PHMM_OK;
+}
+
+// bytes of a plan buffer set to `v` (in the arena's mirror, else on the device)
+int plan_fill(rodv_phmm_plan *p, void *dst, int v, size_t bytes) {
+    if (!bytes) return RODV_PHMM_OK;
+    if (p->arena) {
+        std::memset(p->arena->host + (reinterpret_cast<uint8_t *>(dst) - p->arena->dev), v, bytes);
+        return RODV_PHMM_OK;
+    }
+    HIPCHK(hipMemset(dst, v, bytes));
+    return RODV_PHMM_OK;
+}
+
+// the arena mirror's used prefix to the device, in one copy
+int plan_flush(rodv_phmm_plan *p) {
+    if (!p->arena || !p->arena->used) return RODV_PHMM_OK;
+    HIPCHK(hipMemcpy(p->arena->dev, p->arena->host, p->arena->used, hipMemcpyHostToDevice));
+    return RODV_PHMM_OK;
 }
 
 // (L, MR) of the select-prior FP32 kernel for a read of R rows.  A strip
@@ -852,6 +911,8 @@ void rodv_phmm_ctx_destroy(rodv_phmm_ctx *ctx) {
     }
     if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->d_tab) (void)hipFree(ctx->d_tab);
+    if (ctx->fwd_arena.dev) (void)hipFree(ctx->fwd_arena.dev);
+    if (ctx->fwd_arena.host) (void)hipHostFree(ctx->fwd_arena.host);
     delete ctx;
 }
 
@@ -880,7 +941,8 @@ int rodv_phmm_tables_f64(double *ph2pr, double *mm, int mm_n) {
 }
 
 static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
-                          const rodv_phmm_config *cfg_in, rodv_phmm_plan **out) {
+                          const rodv_phmm_config *cfg_in, rodv_phmm_plan **out,
+                          Arena *arena = nullptr, bool no_plain = false) {
     if (!ctx || !out) return RODV_PHMM_ERR_INVALID_ARG;
     *out = nullptr;
     int rc = validate(b);
@@ -979,6 +1041,9 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     p->ctx = ctx;
     p->cfg = cfg;
     p->n_pairs = n_pairs;
+    p->arena = arena;
+    p->no_plain = no_plain;
+    if (arena) arena->used = arena->need = 0;
 
     // ---- pairs in output order, and their buckets
     std::vector<uint32_t> ph(n_pairs), pr(n_pairs);
@@ -1182,7 +1247,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     }
 
     // ---- plain schedule: per region, per bucket, region-ordered items
-    {
+    if (!no_plain) {
         for (size_t bi = 0; bi < p->buckets.size(); bi++) {
             Bucket &bk = p->buckets[bi];
             const int G = 64 / bk.L;
@@ -1217,7 +1282,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                 (void)rg;
             }
             int32_t *d = nullptr;
-            rc = upload(&d, items.data(), items.size());
+            rc = plan_upload(p, &d, items.data(), items.size());
             p->d_plain_items.push_back(d);
             p->plain_items_n.push_back(items.size());
             if (rc) {
@@ -1228,36 +1293,36 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     }
 
     // ---- device upload
-    rc = upload(&p->d_hapc, hapc.data(), hapc.size());
-    if (!rc) rc = upload(&p->d_hapc_off, hapc_off.data(), hapc_off.size());
-    if (!rc) rc = upload(&p->d_hap_len, b->hap_lengths, M);
-    if (!rc) rc = upload(&p->d_initd_f, initd_f.data(), M);
-    if (!rc) rc = upload(&p->d_initd_d, initd_d.data(), M);
-    if (!rc) rc = upload(&p->d_rcode, rcode.data(), rcode.size());
-    if (!rc) rc = upload(&p->d_q, q.data(), q.size());
-    if (!rc) rc = upload(&p->d_ins, qi.data(), qi.size());
-    if (!rc) rc = upload(&p->d_del, qd.data(), qd.size());
-    if (!rc) rc = upload(&p->d_gcp, qc.data(), qc.size());
-    if (!rc) rc = upload(&p->d_read_off, read_off.data(), N);
-    if (!rc) rc = upload(&p->d_read_len, b->read_lengths, N);
-    if (!rc) rc = upload(&p->d_pair_hap, ph.data(), n_pairs);
-    if (!rc) rc = upload(&p->d_pair_read, pr.data(), n_pairs);
+    rc = plan_upload(p, &p->d_hapc, hapc.data(), hapc.size());
+    if (!rc) rc = plan_upload(p, &p->d_hapc_off, hapc_off.data(), hapc_off.size());
+    if (!rc) rc = plan_upload(p, &p->d_hap_len, b->hap_lengths, M);
+    if (!rc) rc = plan_upload(p, &p->d_initd_f, initd_f.data(), M);
+    if (!rc) rc = plan_upload(p, &p->d_initd_d, initd_d.data(), M);
+    if (!rc) rc = plan_upload(p, &p->d_rcode, rcode.data(), rcode.size());
+    if (!rc) rc = plan_upload(p, &p->d_q, q.data(), q.size());
+    if (!rc) rc = plan_upload(p, &p->d_ins, qi.data(), qi.size());
+    if (!rc) rc = plan_upload(p, &p->d_del, qd.data(), qd.size());
+    if (!rc) rc = plan_upload(p, &p->d_gcp, qc.data(), qc.size());
+    if (!rc) rc = plan_upload(p, &p->d_read_off, read_off.data(), N);
+    if (!rc) rc = plan_upload(p, &p->d_read_len, b->read_lengths, N);
+    if (!rc) rc = plan_upload(p, &p->d_pair_hap, ph.data(), n_pairs);
+    if (!rc) rc = plan_upload(p, &p->d_pair_read, pr.data(), n_pairs);
     if (!rc && p->any_inter) {
-        rc = upload(&p->d_pair_inter, inter32.data(), n_pairs);
-        if (!rc) rc = upload(&p->d_sched_list, sched_list.data(), sched_list.size());
-        if (!rc) rc = upload(&p->d_sched_off, sched_off.data(), sched_off.size());
-        if (!rc) rc = dev_alloc(&p->d_sched_scratch, sched_list.size());
+        rc = plan_upload(p, &p->d_pair_inter, inter32.data(), n_pairs);
+        if (!rc) rc = plan_upload(p, &p->d_sched_list, sched_list.data(), sched_list.size());
+        if (!rc) rc = plan_upload(p, &p->d_sched_off, sched_off.data(), sched_off.size());
+        if (!rc) rc = plan_alloc(p, &p->d_sched_scratch, sched_list.size());
     }
-    if (!rc) rc = dev_alloc(&p->d_raw32, n_pairs);
-    if (!rc) rc = dev_alloc(&p->d_raw64, n_pairs);
-    if (!rc) rc = dev_alloc(&p->d_out, n_pairs);
-    if (!rc) rc = dev_alloc(&p->d_flags, n_pairs);
-    if (!rc) rc = dev_alloc(&p->d_rescue, n_pairs);
+    if (!rc) rc = plan_alloc(p, &p->d_raw32, n_pairs);
+    if (!rc) rc = plan_alloc(p, &p->d_raw64, n_pairs);
+    if (!rc) rc = plan_alloc(p, &p->d_out, n_pairs);
+    if (!rc) rc = plan_alloc(p, &p->d_flags, n_pairs);
+    if (!rc) rc = plan_alloc(p, &p->d_rescue, n_pairs);
     for (auto &bk : p->buckets) {
-        if (!rc) rc = upload(&bk.d_items, bk.items.data(), bk.items.size());
+        if (!rc) rc = plan_upload(p, &bk.d_items, bk.items.data(), bk.items.size());
         if (!rc && bk.chain) {
-            rc = upload(&bk.d_chain_off, bk.chain_off.data(), bk.chain_off.size());
-            if (!rc) rc = upload(&bk.d_chain_hap, bk.chain_hap.data(), bk.chain_hap.size());
+            rc = plan_upload(p, &bk.d_chain_off, bk.chain_off.data(), bk.chain_off.size());
+            if (!rc) rc = plan_upload(p, &bk.d_chain_hap, bk.chain_hap.data(), bk.chain_hap.size());
         }
         if (!rc && bk.bnd_elems) {
             // the units' slots start as NaN: a row read before its unit
@@ -1265,13 +1330,13 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             // wrong value (the columns read unwritten lie beyond the
             // haplotype and reach no counted cell)
             const size_t nring = (size_t)bk.grid * bk.nbnd;
-            rc = dev_alloc(&bk.d_bnd, (nring + bk.n_xslots) * bk.bnd_elems);
-            if (!rc && (hipMemset(bk.d_bnd, 0, nring * bk.bnd_elems * sizeof(float)) != hipSuccess ||
-                        hipMemset(bk.d_bnd + nring * bk.bnd_elems, 0xff,
-                                  (size_t)bk.n_xslots * bk.bnd_elems * sizeof(float)) != hipSuccess))
-                rc = RODV_PHMM_ERR_DEVICE;
+            rc = plan_alloc(p, &bk.d_bnd, (nring + bk.n_xslots) * bk.bnd_elems);
+            if (!rc) rc = plan_fill(p, bk.d_bnd, 0, nring * bk.bnd_elems * sizeof(float));
+            if (!rc)
+                rc = plan_fill(p, bk.d_bnd + nring * bk.bnd_elems, 0xff,
+                               (size_t)bk.n_xslots * bk.bnd_elems * sizeof(float));
         }
-        if (!rc && !bk.units.empty()) rc = upload(&bk.d_units, bk.units.data(), bk.units.size());
+        if (!rc && !bk.units.empty()) rc = plan_upload(p, &bk.d_units, bk.units.data(), bk.units.size());
     }
     p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size() + 2;
     p->long_counter = p->n_counters - 2;
@@ -1315,15 +1380,17 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         p->long_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(p->long_units.size(), (uint64_t)ctx->num_cus));
         const size_t nring = (size_t)p->long_grid * (size_t)(rescue_long_waves() + 1);
         p->bnd64_elems = 3ull * (max_long_h + 3ull * 64 + 2);
-        if (!rc) rc = dev_alloc(&p->d_bnd64, (nring + p->n_x64) * p->bnd64_elems);
-        if (!rc && (hipMemset(p->d_bnd64, 0, nring * p->bnd64_elems * sizeof(double)) != hipSuccess ||
-                    hipMemset(p->d_bnd64 + nring * p->bnd64_elems, 0xff,
-                              (size_t)p->n_x64 * p->bnd64_elems * sizeof(double)) != hipSuccess))
-            rc = RODV_PHMM_ERR_DEVICE;
-        if (!rc) rc = upload(&p->d_long_units, p->long_units.data(), p->long_units.size());
+        if (!rc) rc = plan_alloc(p, &p->d_bnd64, (nring + p->n_x64) * p->bnd64_elems);
+        if (!rc) rc = plan_fill(p, p->d_bnd64, 0, nring * p->bnd64_elems * sizeof(double));
+        if (!rc)
+            rc = plan_fill(p, p->d_bnd64 + nring * p->bnd64_elems, 0xff,
+                           (size_t)p->n_x64 * p->bnd64_elems * sizeof(double));
+        if (!rc) rc = plan_upload(p, &p->d_long_units, p->long_units.data(), p->long_units.size());
     }
-    if (!rc) rc = dev_alloc(&p->d_counters, p->n_counters);
-    if (!rc) rc = dev_alloc(&p->d_rescued_cells, 1);
+    if (!rc) rc = plan_alloc(p, &p->d_counters, p->n_counters);
+    if (!rc) rc = plan_alloc(p, &p->d_rescued_cells, 1);
+    // an arena's uploads and fills reach the device here, in one copy
+    if (!rc) rc = plan_flush(p);
     // descriptors of the table kernels' item slots (batched and plain order)
     if (!rc) {
         const FwdArgs a = base_args(p);
@@ -1331,11 +1398,11 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         for (size_t bi = 0; bi < p->buckets.size() && !rc; bi++) {
             Bucket &bk = p->buckets[bi];
             if (!bk.tab) continue;
-            rc = dev_alloc(&bk.d_desc, bk.items.size());
+            rc = plan_alloc(p, &bk.d_desc, bk.items.size());
             if (!rc && launch_desc(a, bk.d_items, bk.d_desc, bk.items.size(), nullptr) != hipSuccess)
                 rc = RODV_PHMM_ERR_DEVICE;
             if (!rc && bi < p->d_plain_items.size()) {
-                rc = dev_alloc(&p->d_plain_desc[bi], p->plain_items_n[bi]);
+                rc = plan_alloc(p, &p->d_plain_desc[bi], p->plain_items_n[bi]);
                 if (!rc && launch_desc(a, p->d_plain_items[bi], p->d_plain_desc[bi], p->plain_items_n[bi],
                                        nullptr) != hipSuccess)
                     rc = RODV_PHMM_ERR_DEVICE;
@@ -1562,6 +1629,7 @@ int rodv_phmm_plan_execute(rodv_phmm_plan *p, void *stream) {
 
 static int plan_execute_plain_impl(rodv_phmm_plan *p, void *stream) {
     if (!p) return RODV_PHMM_ERR_INVALID_ARG;
+    if (p->no_plain) return RODV_PHMM_ERR_UNSUPPORTED;
     HIPCHK(hipSetDevice(p->ctx->device));
     return enqueue(p, reinterpret_cast<hipStream_t>(stream), 1, nullptr);
 }
@@ -1855,20 +1923,77 @@ void rodv_phmm_plan_destroy(rodv_phmm_plan *p) {
     free_plan(p);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Largest arena rodv_phmm_forward keeps in its context; a batch whose plan
+// needs more allocates (and frees) its own buffers.
+constexpr size_t kFwdArenaMax = size_t(256) << 20;
+
+// (Re)allocates the context's forward arena to at least `bytes`.
+int grow_arena(Arena &A, size_t bytes) {
+    bytes = std::max<size_t>(bytes, size_t(4) << 20);
+    bytes = (bytes + ((size_t(1) << 20) - 1)) & ~((size_t(1) << 20) - 1);
+    if (A.dev) (void)hipFree(A.dev);
+    if (A.host) (void)hipHostFree(A.host);
+    A = Arena{};
+    if (fail_alloc_hook()) return RODV_PHMM_ERR_OOM;
+    if (hipMalloc(reinterpret_cast<void **>(&A.dev), bytes) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&A.host), bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        if (A.dev) (void)hipFree(A.dev);
+        A = Arena{};
+        return RODV_PHMM_ERR_OOM;
+    }
+    A.cap = bytes;
+    return RODV_PHMM_OK;
+}
+
+}  // namespace
+
+namespace rodv {
+int forward_in(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch, const rodv_phmm_config &c,
+               Arena *arena, double *out_log10, uint8_t *out_flags) {
+    rodv_phmm_plan *p = nullptr;
+    int rc = plan_create_impl(ctx, batch, &c, &p, arena, true);
+    if (rc) return rc;
+    rc = plan_execute_impl(p, nullptr);
+    if (!rc) rc = plan_read_impl(p, out_log10, out_flags, nullptr, nullptr);
+    rodv_phmm_plan_destroy(p);
+    return rc;
+}
+}  // namespace rodv
+
+extern "C" {
+
 static int forward_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                       const rodv_phmm_config *cfg, double *out_log10, uint8_t *out_flags) {
     if (!ctx || !out_log10) return RODV_PHMM_ERR_INVALID_ARG;
-    rodv_phmm_plan *p = nullptr;
     rodv_phmm_config c;
     rodv_phmm_config_default(&c);
-    if (cfg) c = *cfg;
+    if (cfg) {
+        if (cfg->struct_size != sizeof(rodv_phmm_config)) return RODV_PHMM_ERR_INVALID_ARG;
+        c = *cfg;
+    }
     c.use_graph = 0;  // one-shot: capture would not pay off
-    int rc = rodv_phmm_plan_create(ctx, batch, &c, &p);
-    if (rc) return rc;
-    rc = rodv_phmm_plan_execute(p, nullptr);
-    if (!rc) rc = rodv_phmm_plan_read(p, out_log10, out_flags, nullptr, nullptr);
-    rodv_phmm_plan_destroy(p);
-    return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    // the context's arena, grown until the plan fits (it then stays that
+    // size for the next calls); plans larger than kFwdArenaMax allocate
+    Arena &A = ctx->fwd_arena;
+    for (int attempt = 0; attempt < 8; attempt++) {
+        if (!A.dev) {
+            const int g = grow_arena(A, 0);
+            if (g) return g;
+        }
+        const int rc = forward_in(ctx, batch, c, &A, out_log10, out_flags);
+        if (rc != kArenaFull) return rc;
+        const size_t want = std::max(2 * A.cap, 2 * A.need);
+        if (want > kFwdArenaMax) break;
+        const int g = grow_arena(A, want);
+        if (g) return g;
+    }
+    return forward_in(ctx, batch, c, nullptr, out_log10, out_flags);
 }
 
 int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,

@@ -1515,6 +1515,7 @@ hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st) {
     return hipGetLastError();
 }
 
+namespace {
 // Item descriptors: one thread per item slot.
 __global__ void desc_kernel(FwdArgs a, const int32_t *items, ItemDesc *out, uint64_t n) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1537,6 +1538,7 @@ __global__ void desc_kernel(FwdArgs a, const int32_t *items, ItemDesc *out, uint
     }
     out[k] = d;
 }
+}  // namespace
 
 hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, uint64_t n,
                        hipStream_t st) {

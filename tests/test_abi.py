@@ -1,13 +1,19 @@
 """The C ABI as a C or C++ caller sees it: the header compiles on its own,
-the ctypes mirror in rodv_amd/engine.py has the same struct layouts, and the
-header-only schedule_pairhmm shim compiles with the reference's signature."""
+the ctypes mirror in rodv_amd/engine.py has the same struct layouts, the
+library exports pairhmm::schedule::schedule_pairhmm with the reference's
+signature (a caller declaring it as the reference's header does links and
+runs), and nothing but its API."""
 import ctypes as C
 import os
 import subprocess
+import sys
 
 import pytest
 
 from rodv_amd import engine as E
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin"))
+import driver as D  # noqa: E402
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include")
@@ -28,9 +34,10 @@ int main(void) {
 SHIM_CPP = r"""
 #include <vector>
 #include <cstdint>
+#include <cstdio>
 #include "rodv_phmm_schedule.hpp"
-// the reference's declaration (pairhmm_schedule.h:14-25), restated: a caller
-// compiled against it binds to the shim
+// the reference's declaration (pairhmm_schedule.h:14-25), restated: the
+// library's function has exactly its type
 namespace pairhmm { namespace schedule {
 using Fn = bool (*)(const std::vector<std::vector<uint8_t>> &, const std::vector<std::vector<uint8_t>> &,
                     std::vector<std::vector<double>> &, const std::vector<std::vector<uint8_t>> &,
@@ -38,7 +45,14 @@ using Fn = bool (*)(const std::vector<std::vector<uint8_t>> &, const std::vector
                     const std::vector<std::vector<uint8_t>> &, bool, double, double, bool);
 static Fn f = &schedule_pairhmm;
 }}
-int main() { return pairhmm::schedule::f == nullptr; }
+int main() {
+    // no haplotypes: false before any device is touched (pairhmm_schedule.cpp:130-132)
+    std::vector<std::vector<uint8_t>> none, reads{{'A'}}, q{{30}};
+    std::vector<std::vector<double>> res;
+    const bool r = pairhmm::schedule::f(none, reads, res, q, q, q, q, false, 0.1, 0.1, false);
+    std::printf("%d\n", (int)r);
+    return 0;
+}
 """
 
 
@@ -48,8 +62,9 @@ def _compile(tmp_path, src, name, cxx=False):
     exe = tmp_path / (name + ".out")
     cmd = (["g++", "-std=c++17"] if cxx else ["gcc", "-std=c11"]) + ["-Wall", "-Werror", f"-I{INC}",
                                                                        str(p), "-o", str(exe)]
-    if cxx:
-        cmd = ["g++", "-std=c++17", "-Wall", "-Werror", f"-I{INC}", "-fsyntax-only", str(p)]
+    if cxx:  # linked against the engine, as a reference caller would be
+        cmd += [f"-L{D.LIBDIR}", "-lrodv_phmm", f"-Wl,-rpath,{D.LIBDIR}",
+                "-Wl,-rpath-link,/opt/rocm/lib"]
     subprocess.run(cmd, check=True, capture_output=True, text=True)
     return exe
 
@@ -64,5 +79,32 @@ def test_struct_layouts_match_ctypes(tmp_path):
                     E.Tuning.cls.offset]
 
 
-def test_schedule_shim_compiles_with_reference_signature(tmp_path):
-    _compile(tmp_path, SHIM_CPP, "shim.cpp", cxx=True)
+def test_schedule_pairhmm_links_with_reference_signature(tmp_path):
+    exe = _compile(tmp_path, SHIM_CPP, "shim.cpp", cxx=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=60).stdout
+    assert out.strip() == "0"
+
+
+def _dynsyms(path, *flags):
+    out = subprocess.run(["nm", "-D", *flags, path], check=True, capture_output=True, text=True).stdout
+    return [l.split()[-1] for l in out.splitlines() if l.strip()]
+
+
+def test_reference_caller_links_unchanged(tmp_path):
+    # the driver declares schedule_pairhmm exactly as the reference's header
+    # does; the symbol g++ asks for is the one the library defines
+    exe = D.build(str(tmp_path / "schedule_driver"))
+    want = [n for n in _dynsyms(exe, "--undefined-only") if "schedule_pairhmm" in n]
+    assert len(want) == 1
+    assert want[0] in _dynsyms(E.LIB_PATH, "--defined-only")
+
+
+def test_library_exports_only_its_api():
+    # C names: the C ABI (rodv_phmm_*) and the HIP toolchain's compilation-
+    # unit ids; C++ names of this library's own namespaces: schedule_pairhmm
+    # alone (the rest are standard-library template instances)
+    names = _dynsyms(E.LIB_PATH, "--defined-only")
+    plain = [n for n in names if not n.startswith("_Z")]
+    assert plain and all(n.startswith(("rodv_phmm_", "__hip_cuid_")) for n in plain), plain
+    own = [n for n in names if n.startswith(("_ZN4rodv", "_ZN7pairhmm"))]
+    assert len(own) == 1 and "schedule_pairhmm" in own[0], own
