@@ -35,7 +35,8 @@ def check(engine, oracle_mod, b, reps=1, **kw):
     ok, res, us = D.run(*args, reps=reps, **kw)
     assert ok
     wf, wd = host_widths()
-    okr, o_res, _ = oracle_mod.schedule(*args, use_double=kw.get("use_double", False),
+    haps, reads = ([np.frombuffer(x, np.uint8) for x in seqs] for seqs in args[:2])
+    okr, o_res, _ = oracle_mod.schedule(haps, reads, *args[2:], use_double=kw.get("use_double", False),
                                         idle_f=kw.get("idle_f", 0.1), idle_d=kw.get("idle_d", 0.1),
                                         wf=wf, wd=wd)
     assert okr and res.shape == o_res.shape
