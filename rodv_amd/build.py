@@ -25,21 +25,23 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # one-lane atomicAdd into a form that waits for the result right away, where
 # the table kernel fetches its next work item early and reads it late.
 # Unaligned access mode is off so that the table kernel's per-step byte loads
-# of haplotype codes are not merged into unaligned wider reads.  Hidden
-# visibility: the library exports the C ABI and schedule_pairhmm only.
+# of haplotype codes are not merged into unaligned wider reads.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
          "-ffp-contract=off", "-fno-gpu-flush-denormals-to-zero", "-fno-fast-math",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Xclang", "-target-feature", "-Xclang", "-unaligned-access-mode",
          "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
-         "-fvisibility=hidden", "-fvisibility-inlines-hidden",
          "-Wall", "-Wno-unused-function", f"-I{os.path.join(ROOT, 'include')}"]
+
+
+# the engine exports its C ABI and schedule_pairhmm only
+ENGINE_FLAGS = ["-fvisibility=hidden", "-fvisibility-inlines-hidden"]
 
 
 def build(verbose: bool = False) -> str:
     jobs = []
-    for out, srcs in ((LIB, SOURCES), (CEILING_LIB, CEILING_SOURCES)):
-        cmd = [HIPCC, *FLAGS, *srcs, "-o", out + ".tmp"]
+    for out, srcs, extra in ((LIB, SOURCES, ENGINE_FLAGS), (CEILING_LIB, CEILING_SOURCES, [])):
+        cmd = [HIPCC, *FLAGS, *extra, *srcs, "-o", out + ".tmp"]
         jobs.append((out, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
                                            text=True)))
     failed = []
