@@ -33,7 +33,9 @@ enum {
     RODV_PHMM_ERR_OOM = 2,         /* host or device allocation failed        */
     RODV_PHMM_ERR_DEVICE = 3,      /* HIP runtime error / no device           */
     RODV_PHMM_ERR_UNSUPPORTED = 4, /* request outside what the engine supports */
-    RODV_PHMM_ERR_EMPTY = 5        /* M == 0 or N == 0 (the reference's false) */
+    RODV_PHMM_ERR_EMPTY = 5,       /* M == 0 or N == 0 (the reference's false) */
+    RODV_PHMM_ERR_TOO_LARGE = 6    /* batch exceeds rodv_phmm_forward_noalloc's
+                                      fixed device scratch                     */
 };
 
 /* Per-pair output flags. */
@@ -149,6 +151,24 @@ void rodv_phmm_ctx_destroy(rodv_phmm_ctx *ctx);
  */
 int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                       const rodv_phmm_config *cfg, double *out_log10, uint8_t *out_flags);
+
+/*
+ * rodv_phmm_forward without device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (serving
+ * loops and harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors as rodv_phmm_forward (the same plan and kernels), but every device
+ * buffer of the call comes from a fixed scratch array of
+ * RODV_PHMM_NOALLOC_SCRATCH_BYTES in device memory, which each device the
+ * library runs on holds until the process ends: no hipMalloc or hipFree.  A
+ * batch whose buffers do not fit returns RODV_PHMM_ERR_TOO_LARGE (about
+ * 400,000 pairs of 150-base reads; reads longer than 378 bases need far more
+ * for their boundary rows).  Synchronous (host inputs and outputs), so not
+ * for use under graph capture.  Calls take turns on the one scratch (a
+ * process-wide lock): overlapping calls wait rather than race.
+ */
+#define RODV_PHMM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int rodv_phmm_forward_noalloc(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
+                              const rodv_phmm_config *cfg, double *out_log10, uint8_t *out_flags);
 
 /*
  * Two-phase form for callers that score the same batch shape repeatedly or

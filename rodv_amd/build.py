@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librodv_phmm.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("phmm_kernels.hip", "phmm_adjoint.hip", "phmm_host.cpp",
-                                                   "phmm_schedule.cpp")]
+                                                   "phmm_schedule.cpp", "phmm_noalloc.hip")]
 # FP32 VALU issue-ceiling probe used by bench.py's roofline (not part of the engine)
 CEILING_LIB = os.path.join(HERE, "libvalu_ceiling.so")
 CEILING_SOURCES = [os.path.join(HERE, "csrc", "valu_ceiling.hip")]

@@ -26,6 +26,9 @@ struct Arena {
 // ABI: the callers grow the arena or report RODV_PHMM_ERR_TOO_LARGE).
 constexpr int kArenaFull = 100;
 
+// The device of a context.
+int ctx_device(const rodv_phmm_ctx *ctx);
+
 // One plan of `batch` in `arena` (nullptr: buffers of its own), executed
 // once without a graph on the null stream, read back and destroyed.
 int forward_in(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch, const rodv_phmm_config &c,

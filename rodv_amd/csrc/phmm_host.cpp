@@ -845,6 +845,9 @@ const char *rodv_phmm_strerror(int status) {
     case RODV_PHMM_ERR_DEVICE: return "HIP device error";
     case RODV_PHMM_ERR_UNSUPPORTED: return "unsupported request";
     case RODV_PHMM_ERR_EMPTY: return "empty batch (no haplotypes or no reads)";
+    case RODV_PHMM_ERR_TOO_LARGE:
+        return "batch needs more device memory than rodv_phmm_forward_noalloc's fixed scratch "
+               "(RODV_PHMM_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown status";
     }
 }
@@ -1953,6 +1956,8 @@ int grow_arena(Arena &A, size_t bytes) {
 }  // namespace
 
 namespace rodv {
+int ctx_device(const rodv_phmm_ctx *ctx) { return ctx->device; }
+
 int forward_in(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch, const rodv_phmm_config &c,
                Arena *arena, double *out_log10, uint8_t *out_flags) {
     rodv_phmm_plan *p = nullptr;

@@ -20,7 +20,8 @@ from .batch import Batch
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librodv_phmm.so")
 
-OK, ERR_INVALID_ARG, ERR_OOM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY = range(6)
+OK, ERR_INVALID_ARG, ERR_OOM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY, ERR_TOO_LARGE = range(7)
+NOALLOC_SCRATCH_BYTES = 32 << 20  # RODV_PHMM_NOALLOC_SCRATCH_BYTES
 FLAG_RESCUED, FLAG_UNDERFLOW, FLAG_AMBIGUOUS, FLAG_INTER = 1, 2, 4, 8
 SEMANTICS_INTRA, SEMANTICS_SCHEDULE = 0, 1
 
@@ -28,6 +29,7 @@ SEMANTICS_INTRA, SEMANTICS_SCHEDULE = 0, 1
 EXPORTS = (
     "rodv_phmm_init", "rodv_phmm_strerror", "rodv_phmm_config_default",
     "rodv_phmm_ctx_create", "rodv_phmm_ctx_destroy", "rodv_phmm_forward",
+    "rodv_phmm_forward_noalloc",
     "rodv_phmm_plan_create", "rodv_phmm_plan_execute", "rodv_phmm_plan_execute_plain",
     "rodv_phmm_plan_profile", "rodv_phmm_plan_read", "rodv_phmm_plan_stats",
     "rodv_phmm_plan_num_pairs", "rodv_phmm_plan_destroy", "rodv_phmm_tables_f32",
@@ -118,6 +120,7 @@ def load_library(path: str = LIB_PATH):
     L.rodv_phmm_ctx_create.argtypes = [i32, C.POINTER(vp)]
     L.rodv_phmm_ctx_destroy.argtypes = [vp]
     L.rodv_phmm_forward.argtypes = [vp, C.POINTER(CBatch), C.POINTER(Config), vp, vp]
+    L.rodv_phmm_forward_noalloc.argtypes = [vp, C.POINTER(CBatch), C.POINTER(Config), vp, vp]
     L.rodv_phmm_plan_create.argtypes = [vp, C.POINTER(CBatch), C.POINTER(Config), C.POINTER(vp)]
     L.rodv_phmm_plan_execute.argtypes = [vp, vp]
     L.rodv_phmm_plan_execute_plain.argtypes = [vp, vp]
@@ -227,6 +230,19 @@ class Engine:
         cfg = config or default_config()
         _check(self._lib.rodv_phmm_forward(self._h, C.byref(hold.c), C.byref(cfg),
                                            _ptr(out), _ptr(fl)), "forward")
+        return out, fl
+
+    def forward_noalloc(self, batch: Batch, config: Config | None = None):
+        """forward() without device allocation (rodv_phmm_forward_noalloc):
+        the same results, from the library's fixed device scratch; raises
+        PhmmError(ERR_TOO_LARGE) for a batch that does not fit it."""
+        hold = _CBatchHolder(batch)
+        n = batch.num_pairs()
+        out = np.zeros(n, np.float64)
+        fl = np.zeros(n, np.uint8)
+        cfg = config or default_config()
+        _check(self._lib.rodv_phmm_forward_noalloc(self._h, C.byref(hold.c), C.byref(cfg),
+                                                   _ptr(out), _ptr(fl)), "forward_noalloc")
         return out, fl
 
     def schedule_pairhmm(self, haplotypes, reads, result, quality, insertion_qualities,
