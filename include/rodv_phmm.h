@@ -43,6 +43,9 @@ enum {
 #define RODV_PHMM_FLAG_UNDERFLOW 0x02u /* FP64 raw sum == 0: log10 is -inf    */
 #define RODV_PHMM_FLAG_AMBIGUOUS 0x04u /* FP32 raw within 2^-20 of 1e-28f     */
 #define RODV_PHMM_FLAG_INTER 0x08u     /* scored with inter-kernel semantics  */
+#define RODV_PHMM_FLAG_EXTENDED 0x20u  /* FP64 sum underflowed; the result is
+                                          config.extended_range's (finite
+                                          unless UNDERFLOW is set as well)    */
 
 /*
  * A batch: M haplotypes and N reads, concatenated.  The four quality strings
@@ -103,7 +106,16 @@ typedef struct rodv_phmm_config {
     int32_t waves_per_cu;         /* persistent grid size; 0 = engine default   */
     int32_t short_lanes;          /* lanes per pair for reads <= 255 (N-free
                                      haplotypes; <= 240 with N): 16 or 64      */
-    int32_t extended_range;       /* write finite log10 where FP64 underflows   */
+    int32_t extended_range;       /* 1: a pair whose FP64 sum underflows to zero
+                                     (the reference's -inf) is scored again by
+                                     an extended-range FP64 forward: its strips
+                                     of 252 rows in sequence, each strip's input
+                                     row scaled by a power of two, the exponent
+                                     carried exactly; the result is finite
+                                     while a strip's rows lose < ~2 decades per
+                                     base, and flagged RODV_PHMM_FLAG_EXTENDED.
+                                     Every other pair is unchanged, bit for
+                                     bit.  0 (default): the reference's -inf */
     int32_t prior_table;          /* 1: per-pair LDS table of emission priors for
                                      N-free haplotypes (default); 0: select per cell */
     int32_t use_tuning;           /* 1: shapes from rodv_phmm_autotune when the

@@ -334,12 +334,12 @@ double phmm_oracle_raw_f64(const phmm_oracle_pair *p, int semantics) {
     return fwd_f64(p, semantics, 0, 0x1p1020 / (double)p->hlen, NULL);
 }
 
-double phmm_oracle_log10_extended(const phmm_oracle_pair *p) {
+double phmm_oracle_log10_extended(const phmm_oracle_pair *p, int semantics) {
     tables_init();
     if (p->rlen == 0 || p->hlen == 0) return -INFINITY;
     long l2 = 0;
     /* start at 1/H instead of 2^1020/H; the scale is tracked exactly */
-    double s = fwd_f64(p, PHMM_ORACLE_INTRA, 1, 1.0 / (double)p->hlen, &l2);
+    double s = fwd_f64(p, semantics, 1, 1.0 / (double)p->hlen, &l2);
     if (!(s > 0.0)) return -INFINITY;
     return log10(s) + (double)l2 * log10(2.0);
 }
@@ -455,7 +455,7 @@ static void batch_one(void *vc, uint64_t k) {
         lg = log10(r64) - LOG10_K_D;
         break;
     case PHMM_ORACLE_MODE_EXTENDED:
-        lg = phmm_oracle_log10_extended(&p);
+        lg = phmm_oracle_log10_extended(&p, c->semantics);
         break;
     default:
         break;

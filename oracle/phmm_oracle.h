@@ -57,8 +57,9 @@ typedef struct phmm_oracle_pair {
 float phmm_oracle_raw_f32(const phmm_oracle_pair *p, int semantics);
 double phmm_oracle_raw_f64(const phmm_oracle_pair *p, int semantics);
 /* log10 likelihood from an FP64 forward that renormalises each row by a power
- * of two; finite wherever the plain FP64 forward underflows to zero. */
-double phmm_oracle_log10_extended(const phmm_oracle_pair *p);
+ * of two; finite wherever the plain FP64 forward underflows to zero.
+ * semantics: the prior of PHMM_ORACLE_INTRA or PHMM_ORACLE_INTER. */
+double phmm_oracle_log10_extended(const phmm_oracle_pair *p, int semantics);
 /* intra::computeLikelihoods semantics; also reports the raw sums. */
 double phmm_oracle_intra_log10(const phmm_oracle_pair *p, int use_double, float *raw32,
                                double *raw64, uint8_t *flags);

@@ -43,6 +43,7 @@ hipError_t launch_rescued_cells(const FwdArgs &a, unsigned long long *out, hipSt
 hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, uint64_t n,
                        hipStream_t st);
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st);
+hipError_t launch_ext(int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 int adj_max_mr();
 void *adj_kernel_ptr(int MR, bool rescue);
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
@@ -329,6 +330,13 @@ struct rodv_phmm_plan {
     LongUnit *d_long_units = nullptr;
     uint32_t n_x64 = 0;
     uint32_t prog64_base = 0;
+    // extended range: per pair the power of two of its FP64 sum, the
+    // extended-range pass's grid, work counter and boundary rows (2 per wave)
+    int32_t *d_exp64 = nullptr;
+    int ext_grid = 0;
+    uint32_t ext_counter = 0;
+    double *d_bndx = nullptr;
+    uint64_t bndx_elems = 0;
     uint32_t fault_counter = 0;  // hand-over waits of the long-read kernels that ran out
     uint32_t spin_limit = 1u << 20;
     uint32_t rescue_lds = 0;
@@ -374,7 +382,7 @@ void free_plan(rodv_phmm_plan *p) {
                     p->d_read_len, p->d_pair_hap, p->d_pair_read, p->d_pair_inter,
                     p->d_raw32, p->d_raw64, p->d_out, p->d_flags, p->d_rescue,
                     p->d_counters, p->d_bnd64, p->d_sched_list, p->d_sched_off, p->d_sched_scratch,
-                    p->d_rescued_cells};
+                    p->d_rescued_cells, p->d_exp64, p->d_bndx};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &b : p->buckets) {
@@ -962,6 +970,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (cfg.waves_per_cu < 0 || cfg.waves_per_cu > 32) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.simd_width_float > 64 || cfg.simd_width_double > 64) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.haplotype_chains < 0 || cfg.haplotype_chains > 2) return RODV_PHMM_ERR_INVALID_ARG;
+    if (cfg.extended_range != 0 && cfg.extended_range != 1) return RODV_PHMM_ERR_INVALID_ARG;
     if (!(cfg.max_idle_ratio_float == cfg.max_idle_ratio_float) ||
         !(cfg.max_idle_ratio_double == cfg.max_idle_ratio_double))
         return RODV_PHMM_ERR_INVALID_ARG;
@@ -1390,6 +1399,15 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                            (size_t)p->n_x64 * p->bnd64_elems * sizeof(double));
         if (!rc) rc = plan_upload(p, &p->d_long_units, p->long_units.data(), p->long_units.size());
     }
+    if (cfg.extended_range) {
+        // one wave per pair whose FP64 sum underflows, with two boundary
+        // rows of the longest haplotype
+        p->ext_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n_pairs, (uint64_t)ctx->num_cus * 8));
+        p->ext_counter = p->n_counters++;
+        p->bndx_elems = 3ull * (max_h_all + 3ull * 64 + 2);
+        if (!rc) rc = plan_alloc(p, &p->d_exp64, n_pairs);
+        if (!rc) rc = plan_alloc(p, &p->d_bndx, (size_t)p->ext_grid * 2 * p->bndx_elems);
+    }
     if (!rc) rc = plan_alloc(p, &p->d_counters, p->n_counters);
     if (!rc) rc = plan_alloc(p, &p->d_rescued_cells, 1);
     // an arena's uploads and fills reach the device here, in one copy
@@ -1457,6 +1475,7 @@ FwdArgs base_args(rodv_phmm_plan *p) {
     a.flags = p->d_flags;
     a.rescue_list = p->d_rescue;
     a.rescue_count = p->d_counters;
+    a.exp64 = p->d_exp64;
     return a;
 }
 
@@ -1572,10 +1591,21 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
             launches++;
         }
     }
+    if (p->ext_grid) {  // extended range: the pairs whose FP64 sum is zero
+        FwdArgs a = base_args(p);
+        a.counter = p->d_counters + p->ext_counter;
+        a.n_items_dev = p->d_counters;
+        a.lds_stride = p->rescue_lds;
+        a.bnd = p->d_bndx;
+        a.bnd_elems = p->bndx_elems;
+        HIPCHK(launch_ext(p->ext_grid, p->rescue_lds, st, a));
+        launches++;
+    }
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     FinalizeArgs f;
     f.raw32 = p->d_raw32;
     f.raw64 = p->d_raw64;
+    f.exp64 = p->d_exp64;
     f.pair_inter = p->d_pair_inter;
     f.out = p->d_out;
     f.flags = p->d_flags;

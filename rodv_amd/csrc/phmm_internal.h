@@ -113,12 +113,18 @@ struct FwdArgs {
     void *xbnd;
     uint32_t *fault;  // counts the hand-over waits that ran out their bound
     uint32_t spin_limit;  // polls of one hand-over wait
+    // extended range: per pair the power of two its FP64 sum carries
+    // (written for the pairs the extended-range pass scores)
+    int32_t *exp64;
 };
 
 // Flag bit used between the passes of a plan: the FP64 regroup of
 // schedule_pairhmm put this FP32-rejected inter pair in an inter FP64 group.
 // The finalisation turns it into RODV_PHMM_FLAG_INTER and clears it.
 constexpr uint8_t kFlagGrouped64 = 0x10;
+// RODV_PHMM_FLAG_EXTENDED: the FP64 sum underflowed and the extended-range
+// pass scored the pair (its raw64 carries 2^exp64 more).
+constexpr uint8_t kFlagExtended = 0x20;
 
 // schedule_pairhmm's FP64 stage (pairhmm_schedule.cpp:251-304): per region,
 // greedy windows of `wd` FP32-rejected inter pairs in the region's sorted
@@ -140,6 +146,7 @@ struct RegroupArgs {
 struct FinalizeArgs {
     const float *raw32;
     const double *raw64;
+    const int32_t *exp64;  // extended range (may be null)
     const uint8_t *pair_inter;
     double *out;
     uint8_t *flags;  // in: RESCUED set by the FP32 pass; out: all flags

@@ -1250,6 +1250,177 @@ __global__ __launch_bounds__(64) void fwd_rescue_route_kernel(FwdArgs a) {
     }
 }
 
+// ------------------------------------------------- extended range
+// The FP64 forward of a pair whose FP64 sum underflowed to zero (the
+// reference's -inf), again with a finite result: the pair's strips of 63 *
+// kExtMR rows run in sequence in one wave, and the row a strip hands to the
+// next is scaled by a power of two that takes its largest entry to
+// 2^kExtTop.  Scaling by powers of two is exact, so the strip below computes
+// the true values times 2^E, E the sum of the shifts, with the same
+// roundings as an unbounded exponent range would give (up to entries that
+// are subnormal in one of the two).  A strip's entries lie within a factor
+// of at most ~H of its input row's largest, so nothing overflows; the bottom
+// of a strip is finite while its rows lose less than ~2000 binades to its
+// top (about 2 decades per read base at 252 rows), else the pair still
+// underflows.  The scaled sum goes to raw64 and E to exp64 (log10 L =
+// log10(raw64) - E log10(2) - log10(2^1020)); the pair is flagged
+// RODV_PHMM_FLAG_EXTENDED.  Every other pair is left as it was.
+constexpr int kExtMR = 4;
+constexpr int kExtTop = 960;
+
+template <bool INTER>
+__device__ void ext_pair(const FwdArgs &a, uint8_t *lds, int p) {
+    using T = double;
+    constexpr int L = 64;
+    constexpr int MR = kExtMR;
+    constexpr int C = (L - 1) * MR;
+    const int l = threadIdx.x;
+    const uint32_t h = __builtin_amdgcn_readfirstlane(a.pair_hap[p]);
+    const uint32_t r = __builtin_amdgcn_readfirstlane(a.pair_read[p]);
+    const int R = __builtin_amdgcn_readfirstlane((int)a.read_len[r]);
+    const int H = __builtin_amdgcn_readfirstlane((int)a.hap_len[h]);
+    const uint64_t ro64 = a.read_off[r];
+    const uint64_t ro = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ro64 >> 32)) << 32) |
+                        (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ro64);
+    const int off = (MR - R % MR) % MR;
+    const int vR = R - 1 + off;
+    const int nstr = vR / C + 1;
+    const int lR = (vR % C) / MR;
+    const T initd = Tab<T>::initd(a, h);
+
+    // one-hot codes: column j at lds[L - 1 + j]
+    __syncthreads();
+    for (uint32_t k = l * 16u; k < a.lds_stride; k += 64u * 16u)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    __syncthreads();
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
+        const int nch = (H + 15) >> 4;
+        for (int c = l; c < nch; c += 64) *reinterpret_cast<uint4 *>(lds + L + 16 * c) = src[c];
+    }
+    __syncthreads();
+    const uint8_t *cp = lds + (L - 1) - l;
+
+    // two boundary rows per wave, strip cs writing buffer cs & 1
+    T *bbase = reinterpret_cast<T *>(a.bnd) + (uint64_t)blockIdx.x * 2 * a.bnd_elems;
+    const LaneRot<L> rot;
+    const bool feeder = l == L - 1;
+    int E = 0;   // entries are the true values times 2^1020 times 2^E
+    int sh = 0;  // shift of the row the current strip reads
+    T sm = T(0), si = T(0);
+    for (int cs = 0; cs < nstr; cs++) {
+        Priors<T, MR> P;
+        State<T, MR> S;
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int i = cs * C + l * MR + s - off + 1;  // 1-based read row
+            const bool real = l < L - 1 && i >= 1 && i <= R;
+            load_row<T, MR, INTER>(a, P, s, real, ro + (uint64_t)(real ? i - 1 : 0));
+            S.M[s] = T(0);
+            S.I[s] = T(0);
+            S.D[s] = real ? T(0) : initd;
+        }
+        S.Mp = T(0);
+        S.Ip = T(0);
+        S.Dp = rot(S.D[MR - 1]);
+        T yyA = rot(P.yy[MR - 1]), myA = rot(P.my[MR - 1]);
+        if (cs > 0 && l == 0) {
+            const uint64_t ip = ro + (uint64_t)(cs * C - off - 1);  // 0-based row above
+            yyA = Tab<T>::ph2pr(a.tab)[a.gcp[ip] & 127u];
+            myA = Tab<T>::ph2pr(a.tab)[a.del[ip] & 127u];
+            S.Dp = T(0);
+        }
+        const bool last = cs == nstr - 1;
+        const bool inject = cs > 0;
+        T *bnd_out = bbase + (uint64_t)(cs & 1) * a.bnd_elems;
+        const T *bnd_in = bbase + (uint64_t)((cs + 1) & 1) * a.bnd_elems;
+        if (inject) {  // column 1 of the row above, scaled
+            const T bm = ldexp(ld_agent(bnd_in + 3 * (1 + L)), sh);
+            const T bi = ldexp(ld_agent(bnd_in + 3 * (1 + L) + 1), sh);
+            const T bd = ldexp(ld_agent(bnd_in + 3 * (1 + L) + 2), sh);
+            if (feeder) {
+                S.M[MR - 1] = bm;
+                S.I[MR - 1] = bi;
+                S.D[MR - 1] = bd;
+            }
+        }
+        // a strip that is not the last runs until lane L-2 has stored column
+        // H + L + 1, every column the next strip's feeder reads
+        const int Dend = last ? H + lR : H + 2 * L - 1;
+        T bmax = T(0);  // largest entry the storing lane wrote
+        sm = T(0);
+        si = T(0);
+        for (int d0 = 1; d0 <= Dend; d0 += L) {
+            const int dlim = d0 + L - 1 < Dend ? d0 + L - 1 : Dend;
+            // the block's feeder inputs, one coalesced load: lane k holds
+            // column d0 + k + 1 of the row above (index d0 + k + 1 + L)
+            T fm = T(0), fi = T(0), fd = T(0);
+            if (inject && d0 + l <= dlim) {
+                const T *in = bnd_in + 3 * (d0 + l + 1 + L);
+                fm = ldexp(ld_agent(in), sh);
+                fi = ldexp(ld_agent(in + 1), sh);
+                fd = ldexp(ld_agent(in + 2), sh);
+            }
+            for (int d = d0; d <= dlim; d++) {
+                step<T, L, MR, INTER>(S, P, yyA, myA, cp[d], rot);
+                if (last && d - l <= H) {
+                    sm += S.M[MR - 1];
+                    si += S.I[MR - 1];
+                }
+                if (!last && l == L - 2) {  // column d - (L-2) at index d + 2
+                    T *o = bnd_out + 3 * (d + 2);
+                    st_agent(o, S.M[MR - 1]);
+                    st_agent(o + 1, S.I[MR - 1]);
+                    st_agent(o + 2, S.D[MR - 1]);
+                    bmax = fmax(bmax, fmax(S.M[MR - 1], fmax(S.I[MR - 1], S.D[MR - 1])));
+                }
+                if (inject) {  // column d + 1 of the row above
+                    const int k = d - d0;
+                    const T bm = lane_value(fm, k), bi = lane_value(fi, k), bd = lane_value(fd, k);
+                    if (feeder) {
+                        S.M[MR - 1] = bm;
+                        S.I[MR - 1] = bi;
+                        S.D[MR - 1] = bd;
+                    }
+                }
+            }
+        }
+        if (!last) {
+            __threadfence();  // the row's stores before the next strip's loads
+            const T mx = lane_value(bmax, L - 2);
+            // an all-zero row leaves every later entry zero: no shift
+            sh = mx > T(0) ? kExtTop - ilogb(mx) : 0;
+            E += sh;
+        }
+    }
+    if (l == lR) {
+        a.raw64[p] = sm + si;  // pairhmm_impl.cpp:434-437
+        a.exp64[p] = E;
+        a.flags[p] |= kFlagExtended;
+    }
+}
+
+// Over the rescue list: the pairs whose FP64 sum is exactly zero, in the
+// arithmetic (inter or intra) their FP64 pass used.  One pair per wave.
+__global__ __launch_bounds__(64) void fwd_ext_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t n = *a.n_items_dev;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        const int p = (int)__builtin_amdgcn_readfirstlane((int)a.rescue_list[it]);
+        if (__builtin_amdgcn_readfirstlane(a.raw64[p] == 0.0 ? 1 : 0) == 0) continue;
+        const uint32_t inter = __builtin_amdgcn_readfirstlane(
+            (a.pair_inter && (a.flags[p] & kFlagGrouped64)) ? 1u : 0u);
+        if (inter)
+            ext_pair<true>(a, lds, p);
+        else
+            ext_pair<false>(a, lds, p);
+    }
+}
+
 // greedy_grouping (pairhmm_schedule.cpp:46-105) over one region's FP32-rejected
 // inter pairs, one wave per region.  The wave first compacts the region's
 // rejected pairs, in sorted order, into the scratch list (ballot + prefix
@@ -1320,6 +1491,8 @@ __global__ void finalize_kernel(FinalizeArgs f) {
         // fallback of a rejected inter pair -- the same conversion
         double r = f.raw64[k];
         out = log10(r) - 307.050595577260822;
+        // extended range: the sum carries 2^exp64 more
+        if ((fl & kFlagExtended) && f.exp64) out -= (double)f.exp64[k] * 0.30102999566398119521;
         if (!(r > 0.0)) fl |= 0x02;
         inter = inter64;
     } else {
@@ -1501,6 +1674,11 @@ hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st) {
     return hipGetLastError();
 }
 int fwd_f64_rescue_mr() { return 4; }
+
+hipError_t launch_ext(int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
+    void *args[] = {const_cast<FwdArgs *>(&a)};
+    return hipLaunchKernel(reinterpret_cast<void *>(&fwd_ext_kernel), dim3(grid), dim3(64), args, lds_bytes, st);
+}
 
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
     void *args[] = {const_cast<FwdArgs *>(&a)};

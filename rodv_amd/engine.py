@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librodv_phm
 
 OK, ERR_INVALID_ARG, ERR_OOM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_EMPTY, ERR_TOO_LARGE = range(7)
 NOALLOC_SCRATCH_BYTES = 32 << 20  # RODV_PHMM_NOALLOC_SCRATCH_BYTES
-FLAG_RESCUED, FLAG_UNDERFLOW, FLAG_AMBIGUOUS, FLAG_INTER = 1, 2, 4, 8
+FLAG_RESCUED, FLAG_UNDERFLOW, FLAG_AMBIGUOUS, FLAG_INTER, FLAG_EXTENDED = 1, 2, 4, 8, 0x20
 SEMANTICS_INTRA, SEMANTICS_SCHEDULE = 0, 1
 
 # exported symbols of include/rodv_phmm.h

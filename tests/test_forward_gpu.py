@@ -7,8 +7,8 @@ import pytest
 
 from rodv_amd import synth
 from rodv_amd.batch import Batch, parse_assemble_log, parse_gkl_testdata
-from rodv_amd.engine import (ERR_EMPTY, ERR_INVALID_ARG, FLAG_RESCUED, FLAG_UNDERFLOW,
-                             PhmmError, default_config)
+from rodv_amd.engine import (ERR_EMPTY, ERR_INVALID_ARG, FLAG_EXTENDED, FLAG_RESCUED,
+                             FLAG_UNDERFLOW, PhmmError, default_config)
 
 from conftest import GOLDEN
 
@@ -125,6 +125,41 @@ def test_fp64_underflow_long_reads(engine, oracle_mod):
     b = synth.config(4, scale=4 / 256)
     out, fl, o = check_against_oracle(oracle_mod, engine, b)
     assert (fl & FLAG_RESCUED).all()
+
+
+def test_extended_range_long_reads(engine, oracle_mod):
+    # Config-4 long reads, most of which the reference's FP64 path scores
+    # -inf (its sum underflows).  With extended_range those pairs are scored
+    # again with their strips' input rows renormalised: finite, within 1e-4
+    # of the oracle's extended-range forward (itself pinned against an
+    # unscaled log-space forward); every other pair keeps its bits, and the
+    # default still gives the reference's -inf.
+    b = synth.config(4, scale=16 / 256)
+    plain, fl0, _, _ = run(engine, b)
+    ext, fl1, _, _ = run(engine, b, extended_range=1)
+    o_ext, _, _, _ = oracle_mod.run_batch(b, mode=oracle_mod.MODE_EXTENDED, nthreads=8)
+    under = (fl0 & FLAG_UNDERFLOW) != 0
+    assert under.sum() >= 4 and np.all(np.isneginf(plain[under]))
+    np.testing.assert_array_equal(ext[~under], plain[~under])
+    np.testing.assert_array_equal((fl1 & FLAG_EXTENDED) != 0, under)
+    assert not (fl1 & FLAG_UNDERFLOW).any() and np.all(np.isfinite(ext))
+    err = np.abs(ext - o_ext)
+    assert err.max() <= TOL, f"max |dlog10L| {err.max()}"
+    # graph replays of the same plan give the same results
+    plan = engine.plan(b, default_config(semantics=0, extended_range=1, use_graph=1))
+    for _ in range(3):
+        plan.execute()
+        got, gfl = plan.read()
+        np.testing.assert_array_equal(got, ext)
+        np.testing.assert_array_equal(gfl, fl1)
+    plan.close()
+
+
+def test_extended_range_is_validated(engine):
+    b = synth.config(0)
+    with pytest.raises(PhmmError) as e:
+        engine.plan(b, default_config(extended_range=2))
+    assert e.value.code == ERR_INVALID_ARG
 
 
 @pytest.mark.parametrize("cfg", [0, 1])

@@ -193,6 +193,64 @@ def test_extended_range_matches_fp64_and_stays_finite(oracle_mod):
             assert e2[k] < -600  # the reference's FP64 path underflows here
 
 
+def logspace_log10(oracle_mod, b, h, r, inter=False):
+    """FP64 forward of one pair in natural-log space (numpy), row by row: no
+    scaling at all, so it covers the range where the reference's FP64 path
+    underflows.  Same priors as the FP64 reference (context.h:90-127 tables,
+    pairhmm_impl.cpp:88-115), same recurrence (pairhmm_impl.cpp:266-289);
+    D's recurrence along a row, D[j] = my M[j-1] + yy D[j-1], is unrolled to
+    D[j] = my yy^(j-1) sum_{m<j} M[m] yy^-m and taken with a running
+    logaddexp."""
+    ph, mm = oracle_mod.tables_f64()
+    hap = np.frombuffer(b.hap(h), np.uint8)
+    read = np.frombuffer(b.read(r), np.uint8)
+    q, qi, qd, qc = (x.astype(np.int64) & 127 for x in b.quals(r))
+    H = len(hap)
+    with np.errstate(divide="ignore"):
+        lg = np.log
+        M = np.full(H + 1, -np.inf)
+        I = np.full(H + 1, -np.inf)
+        D = np.full(H + 1, lg(1.0 / H))  # row 0 at K = 1
+        jj = np.arange(H + 1, dtype=np.float64)
+        for i in range(len(read)):
+            eps = ph[q[i]]
+            lo, hi = min(qi[i], qd[i]), max(qi[i], qd[i])
+            lmm, lgap = lg(mm[hi * (hi + 1) // 2 + lo]), lg(1.0 - ph[qc[i]])
+            lmx, lxx, lmy, lyy = lg(ph[qi[i]]), lg(ph[qc[i]]), lg(ph[qd[i]]), lg(ph[qc[i]])
+            match = (hap == read[i]) | (hap == ord("N")) | (read[i] == ord("N"))
+            mis = eps * (1.0 / 3.0) if inter else eps / 3.0
+            lpr = np.where(match, lg(1.0 - eps), lg(mis))
+            Mn = np.full(H + 1, -np.inf)
+            Mn[1:] = lpr + np.logaddexp(lmm + M[:-1], lgap + np.logaddexp(I[:-1], D[:-1]))
+            In = np.full(H + 1, -np.inf)
+            In[1:] = np.logaddexp(lmx + M[1:], lxx + I[1:])
+            Dn = np.full(H + 1, -np.inf)
+            acc = np.logaddexp.accumulate(Mn[1:-1] - jj[1:-1] * lyy)  # m = 1 .. H-1
+            Dn[2:] = lmy + (jj[2:] - 1.0) * lyy + acc
+            M, I, D = Mn, In, Dn
+        S = np.logaddexp(np.logaddexp.reduce(M[1:]), np.logaddexp.reduce(I[1:]))
+    return S / np.log(10.0)
+
+
+def test_extended_range_matches_an_unscaled_logspace_forward(oracle_mod):
+    # config-4-like long reads, including ones whose plain FP64 sum is zero:
+    # the renormalised oracle against a forward with no scaling at all
+    long = synth.config(4, scale=6 / 256)
+    ext, _, _, _ = oracle_mod.run_batch(long, mode=oracle_mod.MODE_EXTENDED)
+    _, _, d64, _ = oracle_mod.run_batch(long, mode=oracle_mod.MODE_RAW_F64)
+    under = 0
+    for k in np.argsort(long.read_len)[:5]:  # the five shortest (R 1,007 .. 2,743)
+        ref = logspace_log10(oracle_mod, long, int(k), int(k))
+        assert abs(ext[k] - ref) < 1e-8 * max(1.0, abs(ref)), (k, ext[k], ref)
+        under += d64[k] == 0.0
+    assert under >= 1
+    # inter priors (eps * (1/3)) on the inter semantics
+    ext_i, _, _, _ = oracle_mod.run_batch(long, mode=oracle_mod.MODE_EXTENDED, semantics=oracle_mod.INTER)
+    k = int(np.argmin(long.read_len))
+    ref_i = logspace_log10(oracle_mod, long, k, k, inter=True)
+    assert abs(ext_i[k] - ref_i) < 1e-8 * max(1.0, abs(ref_i))
+
+
 def test_schedule_restatement(oracle_mod):
     # pairhmm_schedule_unittest.cpp: schedule vs intra within 1e-5, every pair
     # computed; empty input is `false`

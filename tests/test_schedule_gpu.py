@@ -223,3 +223,34 @@ def test_long_reads_in_inter_fp64_groups(engine, oracle_mod):
         assert ((route == ROUTE_INTER64) & (R > UNIT)).any()
         assert ((route == ROUTE_INTER64) & (R > 378) & (R <= UNIT)).any()
         assert (fl[R > 378] & FLAG_RESCUED).all()
+
+
+def test_extended_range_under_routing(engine, oracle_mod):
+    # the long unrelated reads of the test above under extended_range: the
+    # inter FP64 groups' underflowed pairs are scored again with the inter
+    # prior, the intra ones with the intra prior, each within 1e-4 of the
+    # oracle's extended-range forward of its route's arithmetic
+    from rodv_amd.engine import FLAG_EXTENDED
+    rng = np.random.default_rng(29)
+    regions = [
+        synth.region(rng, [1800, 2300], [3540, 3560, 3600, 3490, 700, 720, 760, 300],
+                     unrelated=range(8)),
+        synth.region(rng, [900], [600, 610, 640, 650, 4100], unrelated=range(5)),
+    ]
+    b = Batch.concat(regions)
+    cfg = default_config(semantics=1, simd_width_float=4, simd_width_double=4,
+                         max_idle_ratio_float=0.5, max_idle_ratio_double=0.5, extended_range=1)
+    plan = engine.plan(b, cfg)
+    plan.execute()
+    out, fl = plan.read()
+    plan.close()
+    _, o_fl, _, route = oracle_mod.schedule_batch(b, idle_f=0.5, idle_d=0.5, wf=4, wd=4)
+    o_intra, _, _, _ = oracle_mod.run_batch(b, mode=oracle_mod.MODE_EXTENDED, nthreads=8)
+    o_inter, _, _, _ = oracle_mod.run_batch(b, mode=oracle_mod.MODE_EXTENDED,
+                                           semantics=oracle_mod.INTER, nthreads=8)
+    ext = (fl & FLAG_EXTENDED) != 0
+    np.testing.assert_array_equal(ext, (o_fl & FLAG_UNDERFLOW) != 0)
+    assert ((route == ROUTE_INTER64) & ext).any() and ((route != ROUTE_INTER64) & ext).any()
+    want = np.where(route == ROUTE_INTER64, o_inter, o_intra)
+    assert np.all(np.isfinite(out[ext]))
+    assert np.max(np.abs(out[ext] - want[ext])) <= TOL
