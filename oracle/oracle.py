@@ -59,7 +59,7 @@ def lib():
         L.phmm_oracle_param_batch.restype = C.c_int
         L.phmm_oracle_param_batch.argtypes = [
             C.c_uint32, _u8p, _u64p, _u32p, C.c_uint32, _u8p, _u8p, _u64p, _u32p,
-            C.c_uint64, _u32p, _u32p, C.c_void_p, _f64p, C.c_int, _f64p, _f64p]
+            C.c_uint64, _u32p, _u32p, C.c_void_p, _f64p, C.c_int, _f64p, _f64p, C.c_int]
         _lib = L
     return _lib
 
@@ -172,9 +172,11 @@ class Params(C.Structure):
                 ("scale", C.c_double * 8)]
 
 
-def param_batch(batch, theta, weights=None, pairs=None, nthreads=1):
+def param_batch(batch, theta, weights=None, pairs=None, nthreads=1, renorm=True):
     """FP64 forward + reverse-mode gradient of the parametrised model.
-    theta: 10 values (gap_open, gap_cont, scale[8]).  Returns (log10L, grad)."""
+    theta: 10 values (gap_open, gap_cont, scale[8]).  Returns (log10L, grad).
+    renorm: rows renormalised by powers of two (finite for long reads);
+    False: the plain form at K = 2^1020."""
     if pairs is None:
         ph, pr = batch.cross_pairs()
     else:
@@ -194,7 +196,7 @@ def param_batch(batch, theta, weights=None, pairs=None, nthreads=1):
         _p(batch.hap_len, _u32p), len(batch.read_len), _p(batch.read_bases, _u8p),
         _p(batch.base_q, _u8p), _p(batch.read_off, _u64p), _p(batch.read_len, _u32p),
         n, _p(ph, _u32p), _p(pr, _u32p), C.byref(th), _p(w, _f64p), nthreads,
-        _p(out, _f64p), _p(grad, _f64p))
+        _p(out, _f64p), _p(grad, _f64p), int(bool(renorm)))
     return out, grad
 
 

@@ -23,7 +23,10 @@
  *    against the exact scaling identity for long reads.
  *  - phmm_oracle_param_*: no reference counterpart (the reference has no
  *    gradient).  Forward pinned against raw_f64 at the reference transition
- *    values; gradient pinned against central finite differences.
+ *    values; gradient pinned against central finite differences, and the
+ *    renormalised form (rows scaled by powers of two, for long reads)
+ *    against the plain one where that is representable and against finite
+ *    differences where it is not.
  *
  * Build: oracle/Makefile (gcc -O2 -mfma -ffp-contract=off).  fp-contract is
  * off so that only the explicit fmaf() calls below fuse, matching the FMA
@@ -508,10 +511,20 @@ static int qbin(int q) { return (q < 47 ? q : 47) / 6; }
 /*
  * FP64 forward of the parametrised model on full (R+1) x (H+1) matrices, then
  * the exact reverse sweep (adjoint of every cell equation), accumulating
- * dS/d(delta, gamma, e_bin).  K = 2^1020 as in the FP64 reference pass.
+ * dS/d(delta, gamma, e_bin).
+ * renorm == 0: K = 2^1020 as in the FP64 reference pass, no scaling (the
+ * sums underflow for reads of a few thousand bases).
+ * renorm == 1: row 0 at K = 1, and every finished row i is scaled by 2^sh_i
+ * so that its largest entry lies in [1, 2) (fwd_f64's renormalisation).  The
+ * scaled rows satisfy the same recurrence with the cross-row terms of row i
+ * multiplied by 2^sh_i (the D recurrence along a row is unchanged), so the
+ * reverse sweep runs on the scaled entries with those same factors on the
+ * adjoints passed from row i to row i-1; S' = S 2^(sh_1 + ... + sh_R), and
+ * d log S = dS' / S' since the shifts are constants.  Exact up to entries
+ * that are subnormal in one form and not in the other.
  */
 static double param_core(const phmm_oracle_pair *p, const phmm_oracle_params *th,
-                         double grad[10]) {
+                         double grad[10], int renorm) {
     tables_init();
     const uint32_t R = p->rlen, H = p->hlen;
     if (R == 0 || H == 0) {
@@ -529,15 +542,18 @@ static double param_core(const phmm_oracle_pair *p, const phmm_oracle_params *th
     double *D = (double *)calloc(NC, sizeof(double));
     double *e = (double *)malloc(sizeof(double) * R);
     int *bin = (int *)malloc(sizeof(int) * R);
+    int *sh = (int *)calloc((size_t)R + 1, sizeof(int)); /* shift of row i */
     for (uint32_t i = 0; i < R; i++) {
         int qq = p->q[i] & 127;
         bin[i] = qbin(qq);
         e[i] = th->scale[bin[i]] * g_ph2pr_d[qq];
     }
-    const double K = 0x1p1020;
+    const double K = renorm ? 1.0 : 0x1p1020;
+    long E = 0; /* sum of the shifts */
     for (uint32_t j = 0; j <= H; j++) D[j] = K / (double)H;
 #define AT(a, i, j) a[(size_t)(i) * W + (j)]
     for (uint32_t i = 1; i <= R; i++) {
+        double rowmax = 0.0;
         for (uint32_t j = 1; j <= H; j++) {
             int m = is_match(p, i - 1, j - 1, PHMM_ORACLE_INTRA);
             double prior = m ? 1.0 - e[i - 1] : e[i - 1] / 3.0;
@@ -545,11 +561,23 @@ static double param_core(const phmm_oracle_pair *p, const phmm_oracle_params *th
             AT(M, i, j) = t * prior;
             AT(I, i, j) = mx * AT(M, i - 1, j) + xx * AT(I, i - 1, j);
             AT(D, i, j) = my * AT(M, i, j - 1) + yy * AT(D, i, j - 1);
+            rowmax = fmax(rowmax, fmax(AT(M, i, j), fmax(AT(I, i, j), AT(D, i, j))));
+        }
+        if (renorm && rowmax > 0.0) {
+            int ex;
+            frexp(rowmax, &ex); /* rowmax in [2^(ex-1), 2^ex) */
+            sh[i] = 1 - ex;
+            for (uint32_t j = 1; j <= H; j++) {
+                AT(M, i, j) = ldexp(AT(M, i, j), sh[i]);
+                AT(I, i, j) = ldexp(AT(I, i, j), sh[i]);
+                AT(D, i, j) = ldexp(AT(D, i, j), sh[i]);
+            }
+            E += sh[i];
         }
     }
     double S = 0.0;
     for (uint32_t j = 1; j <= H; j++) S += AT(M, R, j) + AT(I, R, j);
-    double lg = log10(S) - log10(K);
+    double lg = log10(S) - log10(K) - (double)E * log10(2.0);
     if (grad) {
         double *Mb = (double *)calloc(NC, sizeof(double));
         double *Ib = (double *)calloc(NC, sizeof(double));
@@ -560,23 +588,24 @@ static double param_core(const phmm_oracle_pair *p, const phmm_oracle_params *th
             AT(Ib, R, j) = 1.0;
         }
         for (uint32_t i = R; i >= 1; i--) {
+            const double f = ldexp(1.0, sh[i]); /* cross-row factor of row i */
             for (uint32_t j = H; j >= 1; j--) {
-                /* D[i][j] = my*M[i][j-1] + yy*D[i][j-1] */
+                /* D[i][j] = my*M[i][j-1] + yy*D[i][j-1] (same row: no factor) */
                 double db = AT(Db, i, j);
                 AT(Mb, i, j - 1) += my * db;
                 AT(Db, i, j - 1) += yy * db;
                 g_delta += db * AT(M, i, j - 1);
                 g_gamma += db * AT(D, i, j - 1);
-                /* I[i][j] = mx*M[i-1][j] + xx*I[i-1][j] */
-                double ib = AT(Ib, i, j);
+                /* I[i][j] = f * (mx*M[i-1][j] + xx*I[i-1][j]) */
+                double ib = AT(Ib, i, j) * f;
                 AT(Mb, i - 1, j) += mx * ib;
                 AT(Ib, i - 1, j) += xx * ib;
                 g_delta += ib * AT(M, i - 1, j);
                 g_gamma += ib * AT(I, i - 1, j);
-                /* M[i][j] = prior * (mm*M' + gapm*(I' + D')) */
+                /* M[i][j] = f * prior * (mm*M' + gapm*(I' + D')) */
                 int m = is_match(p, i - 1, j - 1, PHMM_ORACLE_INTRA);
                 double prior = m ? 1.0 - e[i - 1] : e[i - 1] / 3.0;
-                double mb = AT(Mb, i, j);
+                double mb = AT(Mb, i, j) * f;
                 double t = mm * AT(M, i - 1, j - 1) +
                            gapm * (AT(I, i - 1, j - 1) + AT(D, i - 1, j - 1));
                 double pb = mb * prior;
@@ -605,23 +634,24 @@ static double param_core(const phmm_oracle_pair *p, const phmm_oracle_params *th
     free(D);
     free(e);
     free(bin);
+    free(sh);
     return lg;
 }
 
 double phmm_oracle_param_log10(const phmm_oracle_pair *p, const phmm_oracle_params *th) {
-    return param_core(p, th, NULL);
+    return param_core(p, th, NULL, 1);
 }
 
 double phmm_oracle_param_grad(const phmm_oracle_pair *p, const phmm_oracle_params *th,
                               double grad[10]) {
-    return param_core(p, th, grad);
+    return param_core(p, th, grad, 1);
 }
 
 static void param_one(void *vc, uint64_t k) {
     const batch_ctx *c = (const batch_ctx *)vc;
     phmm_oracle_pair p;
     make_pair(c, k, &p);
-    double lg = param_core(&p, c->th, c->grads ? c->grads + 10 * k : NULL);
+    double lg = param_core(&p, c->th, c->grads ? c->grads + 10 * k : NULL, c->mode);
     if (c->out_log10) c->out_log10[k] = lg;
 }
 
@@ -632,12 +662,13 @@ int phmm_oracle_param_batch(uint32_t num_haps, const uint8_t *hap_bases,
                             uint64_t num_pairs, const uint32_t *pair_hap,
                             const uint32_t *pair_read, const phmm_oracle_params *th,
                             const double *weight, int nthreads, double *out_log10,
-                            double *grad_out) {
+                            double *grad_out, int renorm) {
     (void)num_haps;
     (void)num_reads;
     tables_init();
     batch_ctx c;
     memset(&c, 0, sizeof(c));
+    c.mode = renorm ? 1 : 0;
     c.hap_bases = hap_bases;
     c.hap_off = hap_off;
     c.hap_len = hap_len;

@@ -117,10 +117,13 @@ typedef struct phmm_oracle_params {
 
 double phmm_oracle_param_log10(const phmm_oracle_pair *p, const phmm_oracle_params *th);
 /* Gradient of log10 L w.r.t. (gap_open, gap_cont, scale[0..7]) by reverse mode
- * over the full FP64 matrices.  Returns log10 L. */
+ * over the full FP64 matrices, each row renormalised by a power of two (finite
+ * for long reads).  Returns log10 L. */
 double phmm_oracle_param_grad(const phmm_oracle_pair *p, const phmm_oracle_params *th,
                               double grad[10]);
-/* Batch form: grad_out[10] += sum_k weight[k] * d log10 L_k. */
+/* Batch form: grad_out[10] += sum_k weight[k] * d log10 L_k.  renorm = 1:
+ * rows renormalised (as phmm_oracle_param_grad); 0: the plain FP64 form at
+ * K = 2^1020, which underflows for long reads (kept to pin the other). */
 int phmm_oracle_param_batch(uint32_t num_haps, const uint8_t *hap_bases,
                             const uint64_t *hap_off, const uint32_t *hap_len,
                             uint32_t num_reads, const uint8_t *read_bases, const uint8_t *q,
@@ -128,7 +131,7 @@ int phmm_oracle_param_batch(uint32_t num_haps, const uint8_t *hap_bases,
                             uint64_t num_pairs, const uint32_t *pair_hap,
                             const uint32_t *pair_read, const phmm_oracle_params *th,
                             const double *weight, int nthreads, double *out_log10,
-                            double *grad_out);
+                            double *grad_out, int renorm);
 
 #ifdef __cplusplus
 }

@@ -425,6 +425,407 @@ __global__ __launch_bounds__(64, (MR <= 4 && !RESCUE ? 2 : 1)) void adj_kernel(A
     }
 }
 
+// ------------------------------------------------------------ long reads
+// Reads longer than one strip of 63 * MR rows, in FP64 (their FP32 sums
+// underflow), one pair per wave, strips in sequence:
+//  A. forward over every strip, storing each strip's last row in HBM; the
+//     row a strip hands to the next is scaled by a power of two that takes
+//     its largest entry to 2^kLongTop (the shift is kept per strip), so the
+//     scaled entries stay in range however long the read is (as the
+//     extended-range forward does);
+//  B. strips last to first: recompute the strip's forward from its input
+//     row with the two levels of checkpoints of the one-strip kernel, then
+//     its reverse sweep.  Below lane 62's last row lies the next strip's
+//     first row: before each reverse step lane 63 presents that row's
+//     adjoints (Tb one column right, Ib at the column), which the strip
+//     below handed up, scaled by the shift between the two strips (the
+//     adjoints of scaled entries scale inversely, so adjoint x forward
+//     products share one scale across strips); lane 0 hands its first row's
+//     adjoints up in turn.  Each strip's per-row gradient accumulators are
+//     reduced into per-lane totals (by the strip's rows and quality bins)
+//     before the next.
+// The feeder's inputs and the adjoints from below are staged in LDS per
+// checkpoint segment (one coalesced load each), not loaded per step.
+constexpr int kLongMR = 4;
+constexpr int kLongTop = 960;
+constexpr int kMaxLongStrips = 64;
+
+template <typename T>
+__device__ __forceinline__ T ld_ag(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_ag(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Priors, row transitions and initial state of strip cs (parametrised model).
+template <int MR>
+__device__ __forceinline__ void long_setup(const AdjArgs &a, int cs, int nstr, int R, int off, uint64_t ro,
+                                           double initd, Priors<double, MR> &P, AdjRows<double, MR> &A,
+                                           State<double, MR> &S0) {
+    constexpr int C = (kL - 1) * MR;
+    const int l = threadIdx.x;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int i = cs * C + l * MR + s - off + 1;
+        const bool real = l < kL - 1 && i >= 1 && i <= R;
+        const uint64_t idx = ro + (uint64_t)(real ? i - 1 : 0);
+        const uint32_t qq = a.q[idx] & 127u;
+        const uint32_t code = a.rcode[idx];
+        double X = a.xd[qq], Y = a.yd[qq];
+        Y = code == kReadN ? X : Y;  // N in the read: the match prior
+        P.mm[s] = real ? a.mm_d : 0.0;
+        P.g[s] = real ? a.g_d : 0.0;
+        P.mx[s] = real ? a.delta : 0.0;
+        P.my[s] = real ? a.delta : 0.0;
+        P.xx[s] = real ? a.gamma : 0.0;
+        P.yy[s] = real ? a.gamma : 1.0;
+        P.X[s] = real ? X : 0.0;
+        P.Y[s] = real ? Y : 0.0;
+        P.rpos[s] = real ? code : 0u;
+        S0.M[s] = 0.0;
+        S0.I[s] = 0.0;
+        S0.D[s] = real ? 0.0 : initd;
+    }
+    const LaneXfer<kL, 1> up;
+    S0.Mp = 0.0;
+    S0.Ip = 0.0;
+    S0.Dp = up(S0.D[MR - 1]);
+    A.ab_yy = up(P.yy[MR - 1]);
+    A.ab_my = up(P.my[MR - 1]);
+    if (cs > 0 && l == 0) {  // the row above is the previous strip's last row
+        A.ab_yy = a.gamma;
+        A.ab_my = a.delta;
+        S0.Dp = 0.0;
+    }
+    const LaneXfer<kL, -1> down;
+    const double m = down(P.mm[0]), g = down(P.g[0]), x = down(P.mx[0]), xx = down(P.xx[0]);
+    const bool wrap = l == kL - 1;
+    // below lane L-2's last row: the next strip's first row
+    const bool below = l == kL - 2 && cs < nstr - 1;
+    A.nb_mm = wrap ? 0.0 : (below ? a.mm_d : m);
+    A.nb_g = wrap ? 0.0 : (below ? a.g_d : g);
+    A.nb_mx = wrap ? 0.0 : (below ? a.delta : x);
+    A.nb_xx = wrap ? 0.0 : (below ? a.gamma : xx);
+}
+
+template <int MR>
+__device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
+    using T = double;
+    const int lane = threadIdx.x;
+    const int l = lane;
+    constexpr int NV = 3 * MR + 3;
+    constexpr int C = (kL - 1) * MR;
+    const int p = uni(p_in);
+    const uint32_t h = (uint32_t)uni((int)a.pair_hap[p]), r = (uint32_t)uni((int)a.pair_read[p]);
+    const int R = uni((int)a.read_len[r]), H = uni((int)a.hap_len[h]);
+    const uint64_t ro64 = a.read_off[r];
+    const uint64_t ro = ((uint64_t)(uint32_t)uni((int)(ro64 >> 32)) << 32) |
+                        (uint64_t)(uint32_t)uni((int)(uint32_t)ro64);
+    const int off = (MR - R % MR) % MR;
+    const int vR = R - 1 + off;
+    const int nstr = vR / C + 1;  // <= kMaxLongStrips (checked by the host)
+    const int lR = (vR % C) / MR;
+    const T initd = a.initd_d[h];
+    const int K1 = a.ckpt;
+
+    // LDS: hap codes | level-2 states | feeder inputs of a segment | adjoints
+    // from below of a segment | the strips' shifts
+    T *l2 = reinterpret_cast<T *>(lds + a.lds_stride);
+    T *fst = l2 + (size_t)((K1 + 7) / 8) * NV * kL;
+    T *bst = fst + 3 * K1;
+    T *tot = bst + 2 * (K1 + 1);  // per-lane gradient totals, tot[t * 64 + lane]
+    int *shs = reinterpret_cast<int *>(tot + 10 * kL);
+    __syncthreads();
+    for (uint32_t k = lane * 16; k < a.lds_stride; k += 64 * 16)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
+        const int nch = (H + 15) >> 4;
+        __syncthreads();
+        // one-hot codes with bit 4 set: a read N matches every column
+        for (int c = l; c < nch; c += kL) {
+            const uint4 v = src[c];
+            *reinterpret_cast<uint4 *>(lds + kL + 16 * c) =
+                make_uint4(v.x | 0x10101010u, v.y | 0x10101010u, v.z | 0x10101010u, v.w | 0x10101010u);
+        }
+    }
+    __syncthreads();
+    const uint8_t *cp = lds + (kL - 1) - l;
+    const LaneXfer<kL, 1> up;
+    const LaneXfer<kL, -1> down;
+    T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)blockIdx.x * a.lbnd_stride;  // row of strip cs at cs * lrow
+    T *AB = reinterpret_cast<T *>(a.lab) + (uint64_t)blockIdx.x * 2 * a.lab_elems;
+    T *ck = reinterpret_cast<T *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
+
+    // ---- A: forward over the strips
+    int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
+    T sm = T(0), si = T(0);
+    if (lane == 0) shs[0] = 0;
+    for (int cs = 0; cs < nstr; cs++) {
+        Priors<T, MR> P;
+        AdjRows<T, MR> A;
+        State<T, MR> S;
+        long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S);
+        __syncthreads();
+        const bool last = cs == nstr - 1;
+        const int sh = shs[cs];
+        const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
+        T *bout = B + (uint64_t)cs * a.lrow;
+        if (cs > 0) {  // column 1 of the row above (column j at 3 * (j + L))
+            const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
+                    bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
+            if (l == kL - 1) {
+                S.M[MR - 1] = bm;
+                S.I[MR - 1] = bi;
+                S.D[MR - 1] = bd;
+            }
+        }
+        // a strip that is not the last stores its last row up to column H + 69,
+        // past every column the recomputations' feeders read (D4 + 1)
+        const int Dend = last ? H + lR : H + 2 * kL + 3;
+        T bmax = T(0);
+        sm = T(0);
+        si = T(0);
+        for (int d0 = 1; d0 <= Dend; d0 += kL) {
+            const int dlim = d0 + kL - 1 < Dend ? d0 + kL - 1 : Dend;
+            T fm = T(0), fi = T(0), fd = T(0);
+            if (cs > 0 && d0 + l <= dlim) {
+                const T *in = bin + 3 * (d0 + l + 1 + kL);
+                fm = ldexp(ld_ag(in), sh);
+                fi = ldexp(ld_ag(in + 1), sh);
+                fd = ldexp(ld_ag(in + 2), sh);
+            }
+            for (int d = d0; d <= dlim; d++) {
+                step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
+                if (last && d - l <= H) {
+                    sm += S.M[MR - 1];
+                    si += S.I[MR - 1];
+                }
+                if (!last && l == kL - 2) {  // column d - (L-2) at 3 * (d + 2)
+                    T *o = bout + 3 * (d + 2);
+                    st_ag(o, S.M[MR - 1]);
+                    st_ag(o + 1, S.I[MR - 1]);
+                    st_ag(o + 2, S.D[MR - 1]);
+                    bmax = fmax(bmax, fmax(S.M[MR - 1], fmax(S.I[MR - 1], S.D[MR - 1])));
+                }
+                if (cs > 0) {  // column d + 1 of the row above
+                    const int k = d - d0;
+                    const T bm = lane_value(fm, k), bi = lane_value(fi, k), bd = lane_value(fd, k);
+                    if (l == kL - 1) {
+                        S.M[MR - 1] = bm;
+                        S.I[MR - 1] = bi;
+                        S.D[MR - 1] = bd;
+                    }
+                }
+            }
+        }
+        if (!last) {
+            __threadfence();  // the row's stores before the next strip's loads
+            const T mx = lane_value(bmax, kL - 2);
+            const int nsh = mx > T(0) ? kLongTop - ilogb(mx) : 0;
+            if (lane == 0) shs[cs + 1] = nsh;
+            E += nsh;
+        }
+    }
+    __syncthreads();
+    const T Ssum = lane_value(sm + si, lR);  // pairhmm_impl.cpp:434-437
+    if (!(Ssum > T(0))) {                    // underflow even renormalised: no gradient
+        if (lane == 0) {
+            a.out_log10[p] = -INFINITY;
+            for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)p + t] = 0.0;
+            a.flags[p] = 0x03;
+        }
+        return;
+    }
+    const T sigma = ldexp(1.0, -ilogb(Ssum));
+
+    // ---- B: strips last to first
+#pragma unroll
+    for (int t = 0; t < 10; t++) tot[t * kL + lane] = T(0);
+    for (int cs = nstr - 1; cs >= 0; cs--) {
+        Priors<T, MR> P;
+        AdjRows<T, MR> A;
+        State<T, MR> S0;
+        long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S0);
+        const bool last = cs == nstr - 1;
+        const int sh = shs[cs];
+        const T fb = last ? T(0) : ldexp(1.0, shs[cs + 1]);  // adjoints from below to this strip's scale
+        const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
+        const T *ab_in = AB + (uint64_t)((cs + 1) & 1) * a.lab_elems;
+        T *ab_out = AB + (uint64_t)(cs & 1) * a.lab_elems;
+        if (cs > 0) {
+            const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
+                    bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
+            if (l == kL - 1) {
+                S0.M[MR - 1] = bm;
+                S0.I[MR - 1] = bi;
+                S0.D[MR - 1] = bd;
+            }
+        }
+        // the reverse covers every step at which a lane of read rows is at a
+        // column 1..H: up to lane lR (last strip) or lane L-2
+        const int Dn = last ? H + lR : H + kL - 2;
+        const int D4 = (Dn + kW - 1) / kW * kW;
+        const int nseg = (D4 + K1 - 1) / K1;
+        // feeder inputs of the segment after step d0s: column d0s + 2 + m at fst[3m]
+        auto stage_feeder = [&](int d0s) {
+            __syncthreads();
+            if (cs > 0)
+                for (int k = l; k < 3 * K1; k += kL) {
+                    const int c = d0s + 2 + k / 3;
+                    fst[k] = ldexp(ld_ag(bin + 3 * (c + kL) + k % 3), sh);
+                }
+            __syncthreads();
+        };
+        // a forward step, then the feeder's injection of column d + 1
+        auto fstep = [&](State<T, MR> &S, int d, int d0s) {
+            step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
+            if (cs > 0) {
+                const T *f = fst + 3 * (d - d0s - 1);
+                const T bm = f[0], bi = f[1], bd = f[2];
+                if (l == kL - 1) {
+                    S.M[MR - 1] = bm;
+                    S.I[MR - 1] = bi;
+                    S.D[MR - 1] = bd;
+                }
+            }
+        };
+        // pass 1: the strip's forward with level-1 checkpoints
+        {
+            State<T, MR> S = S0;
+            put_state(ck, NV, 0, S);
+            for (int k = 0; k < nseg; k++) {
+                const int d0s = k * K1;
+                stage_feeder(d0s);
+                const int n = min(K1, D4 - d0s);
+                for (int u = 1; u <= n; u++) {
+                    fstep(S, d0s + u, d0s);
+                    if (u == K1 && d0s + u < D4) put_state(ck, NV, k + 1, S);
+                }
+            }
+        }
+        T Tb[MR], Ib[MR], Db[MR], gd[MR], gdT[MR], gg[MR], ge[MR];
+#pragma unroll
+        for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gdT[s] = gg[s] = ge[s] = T(0);
+        T Tb0p = T(0);
+        // one reverse step: lane L-1 first presents the adjoints of the row
+        // below lane L-2's last row (bst[2m] = Tb, bst[2m + 1] = Ib of column
+        // d0s - 61 + m); lane 0 then hands its first row's adjoints up
+        auto rstep = [&](const State<T, MR> &Sb, const State<T, MR> &Sa, int d, int d0s) {
+            if (!last) {
+                const T tT = bst[2 * (d - d0s)];          // Tb at column d - 61 = j + 1
+                const T iI = bst[2 * (d - d0s) - 1];      // Ib at column d - 62 = j
+                if (l == kL - 1) {
+                    Tb0p = tT;
+                    Ib[0] = iI;
+                }
+            }
+            const int j = d - l;
+            const T seed = (last && l == lR && j >= 1 && j <= H) ? sigma : T(0);
+            adj_step<T, MR>(Sb, Sa, P, A, cp[d], seed, Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
+            if (cs > 0 && l == 0) {
+                st_ag(ab_out + 2 * d, Tb[0]);
+                st_ag(ab_out + 2 * d + 1, Ib[0]);
+            }
+        };
+        auto window4 = [&](const State<T, MR> &W0, int dw, int d0s) {
+            State<T, MR> W1 = W0;
+            fstep(W1, dw + 1, d0s);
+            State<T, MR> W2 = W1;
+            fstep(W2, dw + 2, d0s);
+            State<T, MR> W3 = W2;
+            fstep(W3, dw + 3, d0s);
+            State<T, MR> W4 = W3;
+            fstep(W4, dw + 4, d0s);
+            rstep(W3, W4, dw + 4, d0s);
+            rstep(W2, W3, dw + 3, d0s);
+            rstep(W1, W2, dw + 2, d0s);
+            rstep(W0, W1, dw + 1, d0s);
+        };
+        // passes 2 and 3, segment by segment from the last
+        for (int k = nseg - 1; k >= 0; k--) {
+            const int d0s = k * K1;
+            const int nsteps = min(K1, D4 - d0s);
+            const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
+            stage_feeder(d0s);
+            if (!last)
+                for (int m = l; m <= K1; m += kL) {
+                    const int c = d0s - 61 + m;
+                    const bool in = c >= 1 && c <= H;  // beyond: no adjoint
+                    bst[2 * m] = in ? ld_ag(ab_in + 2 * c) * fb : T(0);
+                    bst[2 * m + 1] = in ? ld_ag(ab_in + 2 * c + 1) * fb : T(0);
+                }
+            __syncthreads();
+            State<T, MR> S;
+            get_state(ck, NV, k, S);
+            for (int w = 0; w < nwin; w++) {
+                put_state(l2, NV, w, S);
+                const int n = min(2 * kW, nsteps - w * 2 * kW);
+                for (int u = 0; u < n; u++) fstep(S, d0s + w * 2 * kW + u + 1, d0s);
+            }
+            __syncthreads();
+            for (int w = nwin - 1; w >= 0; w--) {
+                const int dw = d0s + w * 2 * kW;
+                const int n = min(2 * kW, nsteps - w * 2 * kW);
+                State<T, MR> W0;
+                get_state(l2, NV, w, W0);
+                if (n == 2 * kW) {
+                    State<T, MR> X = W0;
+#pragma unroll
+                    for (int u = 1; u <= kW; u++) fstep(X, dw + u, d0s);
+                    window4(X, dw + kW, d0s);
+                }
+                window4(W0, dw, d0s);
+            }
+        }
+        // the strip's rows into the per-lane totals
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int i = cs * C + l * MR + s - off + 1;
+            const bool rl = l < kL - 1 && i >= 1 && i <= R;
+            const uint32_t qq = a.q[ro + (uint64_t)(rl ? i - 1 : 0)] & 127u;
+            const uint32_t bin_q = a.qbin[qq];
+            tot[0 * kL + lane] += rl ? fma(T(-2), gdT[s], gd[s]) : T(0);
+            tot[1 * kL + lane] += rl ? gg[s] : T(0);
+            const T c = rl ? ge[s] * a.wq[qq] : T(0);
+            tot[(2 + bin_q) * kL + lane] += c;
+        }
+        __threadfence();  // the handed-up adjoints before the strip above loads them
+    }
+    double acc[10];
+#pragma unroll
+    for (int t = 0; t < 10; t++) {
+        double v = tot[t * kL + lane];
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+        acc[t] = v;
+    }
+    if (lane == 0) {
+        const double sS = sigma * Ssum;
+        const double ln10 = 2.302585092994045684;
+        a.out_log10[p] = log10(Ssum) - (1020.0 + (double)E) * 0.30102999566398119521;
+        double *g = a.out_grad + 10 * (uint64_t)p;
+        g[0] = -a.delta * acc[0] / (10.0 * sS);
+        g[1] = -a.gamma * acc[1] / (10.0 * sS);
+        for (int b = 0; b < 8; b++) g[2 + b] = acc[2 + b] / (sS * ln10);
+        a.flags[p] = 0x01;
+    }
+}
+
+__global__ __launch_bounds__(64, 1) void adj_long_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const uint32_t n = a.n_items;
+    // a wave takes at most n items: the bound makes every wave exit
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        adj_long_item<kLongMR>(a, lds, a.items[it]);
+    }
+}
+
 }  // namespace
 
 int adj_max_mr() { return 6; }
@@ -439,6 +840,14 @@ void *adj_kernel_ptr(int MR, bool rescue) {
     default: return nullptr;
     }
 #undef KA
+}
+
+int adj_long_mr() { return kLongMR; }
+int adj_long_max_strips() { return kMaxLongStrips; }
+
+hipError_t launch_adj_long(int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a) {
+    void *args[] = {const_cast<AdjArgs *>(&a)};
+    return hipLaunchKernel(reinterpret_cast<void *>(&adj_long_kernel), dim3(grid), dim3(64), args, lds_bytes, st);
 }
 
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a) {

@@ -47,6 +47,9 @@ hipError_t launch_ext(int grid, size_t lds_bytes, hipStream_t st, const FwdArgs 
 int adj_max_mr();
 void *adj_kernel_ptr(int MR, bool rescue);
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
+int adj_long_mr();
+int adj_long_max_strips();
+hipError_t launch_adj_long(int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
 }  // namespace rodv
 
 using namespace rodv;
@@ -2064,16 +2067,34 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     const int maxmr = adj_max_mr();
     std::vector<std::vector<int32_t>> bucket(maxmr + 1);
     uint32_t max_h = 0;
+    // reads longer than one strip (63 * maxmr rows): the long-read kernel,
+    // its strips of 63 * lmr rows in sequence, longest work first
+    const int lmr = adj_long_mr();
+    std::vector<std::pair<uint64_t, int32_t>> longs;
+    uint32_t max_h_long = 0;
+    int max_strips = 1;
     for (uint64_t k = 0; k < n; k++) {
         const uint32_t R = batch->read_lengths[pr[k]], H = batch->hap_lengths[ph[k]];
         const int mr = (int)((R + 62) / 63);
         if (mr > maxmr) {
-            rodv_phmm_plan_destroy(p);
-            return RODV_PHMM_ERR_UNSUPPORTED;  // reads longer than 63*maxmr rows
+            const int off = (lmr - (int)(R % (uint32_t)lmr)) % lmr;
+            const int nstr = ((int)R - 1 + off) / (63 * lmr) + 1;
+            if (nstr > adj_long_max_strips()) {
+                rodv_phmm_plan_destroy(p);
+                return RODV_PHMM_ERR_UNSUPPORTED;  // more strips than the kernel keeps shifts for
+            }
+            longs.push_back({(uint64_t)R * H, (int32_t)k});
+            max_h_long = std::max(max_h_long, H);
+            max_strips = std::max(max_strips, nstr);
+            continue;
         }
         bucket[mr].push_back((int32_t)k);
         max_h = std::max(max_h, H);
     }
+    std::stable_sort(longs.begin(), longs.end(), [](const std::pair<uint64_t, int32_t> &x,
+                                                    const std::pair<uint64_t, int32_t> &y) {
+        return x.first > y.first;
+    });
     // per-quality tables of the parametrised model (double, then rounded)
     const HostTables &t = tables();
     const double delta = std::pow(10.0, -params->gap_open / 10.0);
@@ -2094,9 +2115,10 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     float *d_xf = nullptr, *d_yf = nullptr;
     double *d_xd = nullptr, *d_yd = nullptr, *d_wq = nullptr, *d_out = nullptr, *d_grad = nullptr;
     uint8_t *d_qbin = nullptr, *d_flags = nullptr;
-    int32_t *d_items = nullptr;
+    int32_t *d_items = nullptr, *d_litems = nullptr;
     uint32_t *d_cnt = nullptr, *d_resc = nullptr;
     void *d_ck = nullptr;
+    double *d_lck = nullptr, *d_lbnd = nullptr, *d_lab = nullptr;
     std::vector<int32_t> all_items;
     std::vector<size_t> item_begin(maxmr + 2, 0);
     for (int mr = 1; mr <= maxmr; mr++) {
@@ -2124,13 +2146,45 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     if (!rc) rc = dev_alloc(&d_grad, 10 * n);
     if (!rc) rc = dev_alloc(&d_flags, n);
     if (!rc) rc = dev_alloc(&d_resc, n);
-    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)(maxmr + 1));
+    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)(maxmr + 1) + 1);  // + the long-read kernel's
     if (!rc) rc = dev_alloc(reinterpret_cast<double **>(&d_ck), (size_t)grid * ck_elems);
+    // long reads: per block, every strip's last row but the last strip's,
+    // two rows of handed-up adjoints and the level-1 checkpoints of a strip
+    const int lgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(longs.size(), (uint64_t)ctx->num_cus * 2));
+    // columns of a row: up to H + 2L + 4 read by the forward, + a checkpoint
+    // segment (<= 256 steps) staged past the last step
+    const uint64_t lcols = (uint64_t)max_h_long + 4 * 64 + 8 + 256;
+    const uint64_t lrow = 3 * lcols, lab_elems = 2 * lcols;
+    const uint64_t lbnd_stride = (uint64_t)std::max(1, max_strips - 1) * lrow;
+    const uint64_t lnv = 3ull * lmr + 3;
+    int lK1 = ckpt_stride;
+    const uint32_t lstride = lds_stride_for(max_h_long, 64);
+    auto long_lds = [&](int K1) {
+        return (size_t)lstride + (size_t)((K1 + 7) / 8) * lnv * 64 * 8 + 3ull * K1 * 8 + 2ull * (K1 + 1) * 8 +
+               10ull * 64 * 8 + 4ull * adj_long_max_strips();
+    };
+    while (lK1 > 4 && long_lds(lK1) > (size_t)maxlds) lK1 /= 2;
+    lK1 = lK1 / 4 * 4;
+    const uint64_t ld4max = ((uint64_t)max_h_long + 64 + 3) / 4 * 4;
+    const uint64_t lck_elems = (ld4max / (uint64_t)lK1 + 2) * lnv * 64;
+    if (!longs.empty()) {
+        std::vector<int32_t> li(longs.size());
+        for (size_t k = 0; k < longs.size(); k++) li[k] = longs[k].second;
+        if (!rc) rc = upload(&d_litems, li.data(), li.size());
+        if (!rc) rc = dev_alloc(&d_lck, (size_t)lgrid * lck_elems);
+        if (!rc) rc = dev_alloc(&d_lbnd, (size_t)lgrid * lbnd_stride);
+        if (!rc) rc = dev_alloc(&d_lab, (size_t)lgrid * 2 * lab_elems);
+        // rows are written before they are read; zeros keep the columns a
+        // strip never writes finite
+        if (!rc && (hipMemset(d_lbnd, 0, (size_t)lgrid * lbnd_stride * 8) != hipSuccess ||
+                    hipMemset(d_lab, 0, (size_t)lgrid * 2 * lab_elems * 8) != hipSuccess))
+            rc = RODV_PHMM_ERR_DEVICE;
+    }
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (!rc && (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess))
         rc = RODV_PHMM_ERR_DEVICE;
-    if (!rc && launch_reset(d_cnt, 3 * (maxmr + 1), st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && launch_reset(d_cnt, 3 * (maxmr + 1) + 1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     if (!rc && hipEventRecord(ev0, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     for (int mr = 1; mr <= maxmr && !rc; mr++) {
         const uint32_t cnt = (uint32_t)(item_begin[mr + 1] - item_begin[mr]);
@@ -2188,6 +2242,47 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                 rc = RODV_PHMM_ERR_DEVICE;
         }
     }
+    if (!rc && !longs.empty()) {
+        AdjArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.hapc = p->d_hapc;
+        a.hapc_off = p->d_hapc_off;
+        a.hap_len = p->d_hap_len;
+        a.initd_f = p->d_initd_f;
+        a.initd_d = p->d_initd_d;
+        a.rcode = p->d_rcode;
+        a.q = p->d_q;
+        a.read_off = p->d_read_off;
+        a.read_len = p->d_read_len;
+        a.pair_hap = p->d_pair_hap;
+        a.pair_read = p->d_pair_read;
+        a.items = d_litems;
+        a.n_items = (uint32_t)longs.size();
+        a.counter = d_cnt + 3 * (maxmr + 1);
+        a.xf = d_xf;
+        a.yf = d_yf;
+        a.xd = d_xd;
+        a.yd = d_yd;
+        a.wq = d_wq;
+        a.qbin = d_qbin;
+        a.delta = delta;
+        a.gamma = gamma;
+        a.mm_d = 1.0 - 2.0 * delta;
+        a.g_d = 1.0 - gamma;
+        a.out_log10 = d_out;
+        a.out_grad = d_grad;
+        a.flags = d_flags;
+        a.ck = d_lck;
+        a.ck_elems = lck_elems;
+        a.lds_stride = lstride;
+        a.ckpt = lK1;
+        a.lbnd = d_lbnd;
+        a.lbnd_stride = lbnd_stride;
+        a.lrow = lrow;
+        a.lab = d_lab;
+        a.lab_elems = lab_elems;
+        if (launch_adj_long(lgrid, long_lds(lK1), st, a) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    }
     std::vector<double> out(n), g(10 * n);
     if (!rc && hipEventRecord(ev1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     if (!rc && hipDeviceSynchronize() != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
@@ -2212,7 +2307,7 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         if (out_log10) std::memcpy(out_log10, out.data(), n * sizeof(double));
     }
     void *bufs[] = {d_xf, d_yf, d_xd, d_yd, d_wq, d_qbin, d_items, d_out, d_grad, d_flags,
-                    d_resc, d_cnt, d_ck};
+                    d_resc, d_cnt, d_ck, d_litems, d_lck, d_lbnd, d_lab};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     rodv_phmm_plan_destroy(p);

@@ -185,6 +185,15 @@ struct AdjArgs {
     uint64_t ck_elems;
     uint32_t lds_stride;  // bytes of hap codes before the level-2 states
     int ckpt;             // level-1 checkpoint stride in steps (multiple of 4)
+    // reads longer than one strip (adj_long_kernel), per block: the last row
+    // of every strip but the last (lrow doubles each, lbnd_stride per
+    // block), and two rows of the adjoints a strip hands to the strip above
+    // (lab_elems doubles each)
+    void *lbnd;
+    uint64_t lbnd_stride;
+    uint64_t lrow;
+    void *lab;
+    uint64_t lab_elems;
 };
 
 }  // namespace rodv

@@ -1,6 +1,8 @@
 """The HIP adjoint against the FP64 oracle gradient, central finite
 differences of the FP64 forward (h = 1e-3), checkpoint-stride invariance and
 the torch.autograd wrapper."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -87,3 +89,62 @@ def test_torch_autograd_function(engine, oracle_mod):
     (ll * w).sum().backward()
     _, o_g = oracle_mod.param_batch(b, THETA, weights=w.numpy(), nthreads=8)
     assert_grad_close(theta.grad.numpy(), o_g)
+
+
+def long_pairs(k_shortest):
+    """The k shortest pairs of config 4's first six (R 1,007 .. 2,743, one
+    pair per region; the pair of R = 2,743 is one whose plain FP64 sum
+    underflows)."""
+    long = synth.config(4, scale=6 / 256)
+    sel = np.sort(np.argsort(long.read_len)[:k_shortest])
+    return dataclasses.replace(long, regions=long.regions[sel])
+
+
+def test_adjoint_long_reads(engine, oracle_mod):
+    # reads longer than one strip (378 rows): strips of 252 rows in
+    # sequence, each strip's input row renormalised by a power of two, FP64;
+    # log10 L and the gradient against the oracle's renormalised reverse
+    # mode over full matrices
+    b = long_pairs(5)
+    w = np.array([1.0, 0.5, -0.7, 1.3, 0.9])
+    out, g = engine.adjoint(b, THETA, weights=w)
+    o_out, o_g = oracle_mod.param_batch(b, THETA, weights=w, nthreads=5)
+    l0, _ = oracle_mod.param_batch(b, THETA, nthreads=5, renorm=False)
+    assert np.isneginf(l0).any()  # beyond the plain FP64 range
+    assert np.all(np.isfinite(out))
+    assert np.max(np.abs(out - o_out)) <= 1e-4
+    assert_grad_close(g, o_g)
+
+
+def test_adjoint_long_reads_central_differences(engine):
+    # the engine's own log10 L at theta +- h (h = 1e-3) on two long reads,
+    # one beyond the plain FP64 range
+    long = synth.config(4, scale=6 / 256)
+    order = np.argsort(long.read_len)
+    b = dataclasses.replace(long, regions=long.regions[np.sort(order[[0, 4]])])
+    out, g = engine.adjoint(b, THETA)
+    assert np.all(np.isfinite(out))
+    h = 1e-3
+    fd = np.zeros(10)
+    for k in range(10):
+        tp, tm = THETA.copy(), THETA.copy()
+        tp[k] += h
+        tm[k] -= h
+        fp, _ = engine.adjoint(b, tp)
+        fm, _ = engine.adjoint(b, tm)
+        fd[k] = np.sum(fp - fm) / (2 * h)
+    assert_grad_close(g, fd)
+
+
+def test_adjoint_long_and_short_reads_together(engine, oracle_mod):
+    # one batch with reads on both kernels: per-pair log10 L and the
+    # weighted gradient sum over both
+    rng = np.random.default_rng(31)
+    short = synth.random_batch(rng, 2, 3, hap_len=(100, 300), read_len=(50, 300), qual=(5, 40))
+    from rodv_amd.batch import Batch
+    b = Batch.concat([short, long_pairs(2)])
+    w = np.linspace(-1.0, 1.0, b.num_pairs())
+    out, g = engine.adjoint(b, THETA, weights=w)
+    o_out, o_g = oracle_mod.param_batch(b, THETA, weights=w, nthreads=8)
+    assert np.max(np.abs(out - o_out)) <= 1e-4
+    assert_grad_close(g, o_g)

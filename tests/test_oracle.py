@@ -286,6 +286,41 @@ def test_param_gradient_matches_finite_differences(oracle_mod):
         assert abs(g[k] - fd) <= 1e-5 * max(1.0, abs(fd)), (k, g[k], fd)
 
 
+def test_param_gradient_renormalised_equals_plain(oracle_mod):
+    # rows renormalised by powers of two against the plain K = 2^1020 form,
+    # on reads short enough for the plain form: the same values
+    rng = np.random.default_rng(21)
+    b = synth.random_batch(rng, 3, 5, hap_len=(100, 500), read_len=(50, 400), qual=(5, 40))
+    theta = np.array([40.0, 12.0] + [1.0, 0.9, 1.1, 1.0, 0.95, 1.05, 1.0, 1.2])
+    w = np.linspace(0.5, 1.5, 15)
+    l1, g1 = oracle_mod.param_batch(b, theta, weights=w, nthreads=4)
+    l0, g0 = oracle_mod.param_batch(b, theta, weights=w, nthreads=4, renorm=False)
+    np.testing.assert_allclose(l1, l0, rtol=1e-12)
+    np.testing.assert_allclose(g1, g0, rtol=1e-10)
+
+
+def test_param_gradient_long_read_finite_differences(oracle_mod):
+    # a 1,500-base read unrelated to its haplotype: log10 L ~ -1,300, where
+    # the plain FP64 form underflows; the renormalised gradient against
+    # central differences of the renormalised log10 L (h = 1e-3)
+    rng = np.random.default_rng(22)
+    b = synth.region(rng, [1700], [1500], unrelated=[0], qual=25)
+    theta = np.array([40.0, 12.0] + [1.0, 0.9, 1.1, 1.0, 1.0, 1.05, 1.0, 1.2])
+    l0, _ = oracle_mod.param_batch(b, theta, renorm=False)
+    assert np.isneginf(l0[0])
+    lg, g = oracle_mod.param_batch(b, theta)
+    assert np.isfinite(lg[0]) and lg[0] < -700
+    h = 1e-3
+    for k in (0, 1, 6):  # gap open, gap continuation, the bin of Q25
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fp, _ = oracle_mod.param_batch(b, tp)
+        fm, _ = oracle_mod.param_batch(b, tm)
+        fd = float((fp[0] - fm[0]) / (2 * h))
+        assert abs(g[k] - fd) <= 1e-5 * max(1.0, abs(fd)), (k, g[k], fd)
+
+
 def test_param_model_at_reference_values(oracle_mod):
     # with gap qualities equal to the reads' (45/45/10) and unit scales the
     # parametrised model differs from the table forward only through the
