@@ -148,8 +148,9 @@ def oracle_check(batch, out, fl, r32, cfg, semantics, n_regions=100, max_cells=2
     idx = np.concatenate([np.arange(start[k], start[k] + npairs[k]) for k in sel])
     nt = max(1, min(16, len(os.sched_getaffinity(0))))
     t0 = time.perf_counter()
+    route = None
     if semantics == "schedule":
-        o_out, o_fl, o_r32, _ = O.schedule_batch(
+        o_out, o_fl, o_r32, route = O.schedule_batch(
             sub, use_double=bool(cfg.use_double), idle_f=cfg.max_idle_ratio_float,
             idle_d=cfg.max_idle_ratio_double, wf=cfg.simd_width_float, wd=cfg.simd_width_double,
             nthreads=nt)
@@ -157,8 +158,21 @@ def oracle_check(batch, out, fl, r32, cfg, semantics, n_regions=100, max_cells=2
     else:
         o_out, o_r32, _, o_fl = O.run_batch(sub, nthreads=nt)
         mask = FLAG_RESCUED | FLAG_UNDERFLOW
-    dt = time.perf_counter() - t0
     g_out, g_fl, g_r32 = out[idx], fl[idx], r32[idx]
+    n_ext = 0
+    if cfg.extended_range:
+        # the pairs the reference scores -inf: against the oracle's
+        # extended-range forward of their route's arithmetic
+        ext = np.isneginf(o_out)
+        n_ext = int(ext.sum())
+        if n_ext:
+            e_in, _, _, _ = O.run_batch(sub, mode=O.MODE_EXTENDED, nthreads=nt)
+            if route is not None:
+                e_it, _, _, _ = O.run_batch(sub, mode=O.MODE_EXTENDED, semantics=O.INTER, nthreads=nt)
+                e_in = np.where(route == O.ROUTE_INTER64, e_it, e_in)
+            o_out = np.where(ext, e_in, o_out)
+            mask &= ~FLAG_UNDERFLOW  # finite now
+    dt = time.perf_counter() - t0
     fin = np.isfinite(o_out)
     err = float(np.max(np.abs(g_out[fin] - o_out[fin]))) if fin.any() else 0.0
     return {"regions": int(len(sel)), "pairs": int(len(idx)), "cells": int(cells),
@@ -167,6 +181,7 @@ def oracle_check(batch, out, fl, r32, cfg, semantics, n_regions=100, max_cells=2
             "rescue_set_equal": bool(np.array_equal(g_fl & FLAG_RESCUED, o_fl & FLAG_RESCUED)),
             "flags_equal": bool(np.array_equal(g_fl & mask, o_fl & mask)),
             "neg_inf_equal": bool(np.array_equal(np.isneginf(g_out), np.isneginf(o_out))),
+            "extended_pairs_checked": n_ext,
             "rescued": int(((o_fl & FLAG_RESCUED) != 0).sum()),
             "semantics": semantics, "oracle_threads": nt, "oracle_seconds": round(dt, 1)}
 
@@ -260,6 +275,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (reference for PMC byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--extended-range", type=int, choices=(0, 1), default=0,
+                    help="score pairs whose FP64 sum underflows again with renormalised strips")
     ap.add_argument("--dropin", type=int, default=1,
                     help="time schedule_pairhmm per call through the C++ reference-style caller "
                          "on the single-region configurations")
@@ -304,7 +321,8 @@ def main():
     t0 = time.time()
     run_cfg = default_config(use_graph=1, prior_table=args.prior_table,
                              waves_per_cu=args.waves_per_cu, semantics=sem,
-                             haplotype_chains=args.haplotype_chains)
+                             haplotype_chains=args.haplotype_chains,
+                             extended_range=args.extended_range)
     plan = eng.plan(batch, run_cfg)
     log(f"plan + upload {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
