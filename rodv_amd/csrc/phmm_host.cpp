@@ -255,6 +255,13 @@ struct Bucket {
     LongUnit *d_units = nullptr;
     uint32_t n_xslots = 0;
     uint32_t prog_base = 0;
+    // the FP32 pass of long reads in two launches: units1 runs strip 0 of
+    // every pair alone (a one-strip pair whole), handing its last row to a
+    // slot; a pair whose row is all zero there is rescued with a zero sum,
+    // and `units` (strips 1.. of the others, from that slot) skips it
+    std::vector<LongUnit> units1;
+    LongUnit *d_units1 = nullptr;
+    uint32_t counter1 = 0;  // work counter of the units1 launch
 };
 
 // one launch of the plain (per-region) schedule
@@ -395,6 +402,7 @@ void free_plan(rodv_phmm_plan *p) {
         if (b.d_chain_hap) (void)hipFree(b.d_chain_hap);
         if (b.d_bnd) (void)hipFree(b.d_bnd);
         if (b.d_units) (void)hipFree(b.d_units);
+        if (b.d_units1) (void)hipFree(b.d_units1);
     }
     if (p->d_long_units) (void)hipFree(p->d_long_units);
     for (auto *d : p->d_plain_items)
@@ -698,14 +706,15 @@ int strips_for(uint32_t R, int L, int MR) {
 }
 
 // A pair's units for the long-read kernels: runs of at most
-// rescue_long_waves() strips (one per wave), each but the last handing its
-// boundary row to the next through a new slot.  False if the strip count
-// does not fit the units' 16-bit fields.
-bool append_units(std::vector<LongUnit> &units, uint32_t &nslots, int32_t pair, int nstr) {
+// rescue_long_waves() strips (one per wave) from strip s_begin, each but the
+// last handing its boundary row to the next through a new slot; the first
+// reads the row above strip s_begin from slot xin (kNoSlot: none).  False if
+// the strip count does not fit the units' 16-bit fields.
+bool append_units(std::vector<LongUnit> &units, uint32_t &nslots, int32_t pair, int nstr,
+                  int s_begin = 0, uint32_t xin = kNoSlot) {
     if (nstr < 1 || nstr > 0xffff) return false;
     const int W = rescue_long_waves();
-    uint32_t xin = kNoSlot;
-    for (int s0 = 0; s0 < nstr; s0 += W) {
+    for (int s0 = s_begin; s0 < nstr; s0 += W) {
         const int s1 = std::min(nstr, s0 + W);
         LongUnit u;
         u.pair = pair;
@@ -1232,13 +1241,24 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                 bk.nbnd = (uint32_t)rescue_long_waves() + 1;
                 bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus));
                 bk.grid_cap = bk.grid;
-                // batched: the pairs split into units of strips, so that a
-                // long pair's units run on several CUs at once
-                for (int32_t q : bk.items)
-                    if (q >= 0 && !append_units(bk.units, bk.n_xslots, q,
-                                                strips_for(b->read_lengths[pr[q]], 64, bk.MR)))
+                // batched: strip 0 of every pair first (units1), then the
+                // other strips split into units, so that a long pair's units
+                // run on several CUs at once
+                for (int32_t q : bk.items) {
+                    if (q < 0) continue;
+                    const int nstr = strips_for(b->read_lengths[pr[q]], 64, bk.MR);
+                    if (nstr < 1 || nstr > 0xffff) return RODV_PHMM_ERR_INVALID_ARG;
+                    LongUnit u1;
+                    u1.pair = q;
+                    u1.strips = 0u | 1u << 16;
+                    u1.xin = kNoSlot;
+                    u1.xout = nstr > 1 ? bk.n_xslots++ : kNoSlot;
+                    bk.units1.push_back(u1);
+                    if (nstr > 1 && !append_units(bk.units, bk.n_xslots, q, nstr, 1, u1.xout))
                         return RODV_PHMM_ERR_INVALID_ARG;
-                bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.units.size(), (uint64_t)ctx->num_cus));
+                }
+                bk.grid = (int)std::max<uint64_t>(
+                    1, std::min<uint64_t>(std::max(bk.units.size(), bk.units1.size()), (uint64_t)ctx->num_cus));
             }
         }
     }
@@ -1352,6 +1372,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                                (size_t)bk.n_xslots * bk.bnd_elems * sizeof(float));
         }
         if (!rc && !bk.units.empty()) rc = plan_upload(p, &bk.d_units, bk.units.data(), bk.units.size());
+        if (!rc && !bk.units1.empty()) rc = plan_upload(p, &bk.d_units1, bk.units1.data(), bk.units1.size());
     }
     p->n_counters = 2 + (uint32_t)p->buckets.size() + (uint32_t)p->plain.size() + 2;
     p->long_counter = p->n_counters - 2;
@@ -1360,6 +1381,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     for (auto &bk : p->buckets) {  // progress counters of the long-read slots
         bk.prog_base = p->n_counters;
         p->n_counters += bk.n_xslots;
+        if (!bk.units1.empty()) bk.counter1 = p->n_counters++;
     }
     // FP64 rescue: one pair per wave, two waves per SIMD (its ~240 VGPRs
     // allow no more); boundary rows only if some read > 378 (one strip)
@@ -1517,13 +1539,25 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 a.bnd = bk.d_bnd;
                 a.bnd_elems = bk.bnd_elems;
                 a.max_strips = bk.max_strips;
-                if (bk.d_units) {  // fwd_long_kernel over units of strips
-                    a.units = bk.d_units;
-                    a.n_items = (uint32_t)bk.units.size();
+                if (bk.d_units1) {  // fwd_long_kernel over units of strips
                     a.prog = p->d_counters + bk.prog_base;
                     a.fault = p->d_counters + p->fault_counter;
                     a.spin_limit = p->spin_limit;
                     a.xbnd = bk.d_bnd + (size_t)bk.grid * bk.nbnd * bk.bnd_elems;
+                    // strip 0 of every pair, finding the pairs whose FP32 sum
+                    // is zero; then the other strips of the rest
+                    FwdArgs a1 = a;
+                    a1.units = bk.d_units1;
+                    a1.n_items = (uint32_t)bk.units1.size();
+                    a1.counter = p->d_counters + bk.counter1;
+                    a1.dead_mode = 1;
+                    const size_t lds1 = (size_t)(bk.lds_stride + bk.tab_bytes);
+                    HIPCHK(launch_fwd_block(bk.kern, bk.grid, bk.block, lds1, bs, a1));
+                    launches++;
+                    if (bk.units.empty()) continue;
+                    a.units = bk.d_units;
+                    a.n_items = (uint32_t)bk.units.size();
+                    a.dead_mode = 2;
                 }
                 // chained items: G prior tables and two code buffers per wave
                 const size_t lds = bk.chain ? (size_t)(64 / bk.L) * bk.tab_bytes + 2u * bk.lds_stride

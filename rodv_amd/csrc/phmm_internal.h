@@ -116,6 +116,10 @@ struct FwdArgs {
     // extended range: per pair the power of two its FP64 sum carries
     // (written for the pairs the extended-range pass scores)
     int32_t *exp64;
+    // long-read FP32 pass in two launches: 1 = strip 0 of each pair, which
+    // finds the pairs whose last row of strip 0 is all zero (kFlagDead);
+    // 2 = the other strips, skipping those pairs; 0 = one launch
+    uint32_t dead_mode;
 };
 
 // Flag bit used between the passes of a plan: the FP64 regroup of
@@ -125,6 +129,10 @@ constexpr uint8_t kFlagGrouped64 = 0x10;
 // RODV_PHMM_FLAG_EXTENDED: the FP64 sum underflowed and the extended-range
 // pass scored the pair (its raw64 carries 2^exp64 more).
 constexpr uint8_t kFlagExtended = 0x20;
+// Between the two FP32 launches of long reads: the pair's strip 0 ended in an
+// all-zero row, so every later row and its FP32 sum are exactly zero (it is
+// rescued); the finalisation clears it.
+constexpr uint8_t kFlagDead = 0x40;
 
 // schedule_pairhmm's FP64 stage (pairhmm_schedule.cpp:251-304): per region,
 // greedy windows of `wd` FP32-rejected inter pairs in the region's sorted

@@ -1043,6 +1043,10 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
     bool inject = false, store = false, last = false;
     int Dend = 0, d = 1;
     int cr = w;  // this wave's current strip, s0 + cr
+    // FP32 strip 0 run alone (dead_mode 1): the bits of the last row's
+    // entries at columns 1..H that the storing lane wrote, ORed
+    const bool detect = !RESCUE && a.dead_mode == 1;
+    uint32_t orbits = 0;
     for (int t = 0; t < nblk; t++) {
         const int st0 = (cr / kLW) * P0 + 2 * (cr % kLW);
         bool pub = false;  // this block of the unit's last strip goes to slot xout
@@ -1119,6 +1123,12 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
                     st_agent(o, S.M[MR - 1]);
                     st_agent(o + 1, S.I[MR - 1]);
                     st_agent(o + 2, S.D[MR - 1]);
+                    if constexpr (!RESCUE) {
+                        const int j = d - (L - 2);
+                        if (detect && j >= 1 && j <= H)
+                            orbits |= __float_as_uint((float)S.M[MR - 1]) | __float_as_uint((float)S.I[MR - 1]) |
+                                      __float_as_uint((float)S.D[MR - 1]);
+                    }
                 }
                 if (inject) {  // column d + 1 of the strip above
                     const int k = d - d0;
@@ -1131,6 +1141,25 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
                 }
             }
             if (t == st0 + nb - 1) {
+                if constexpr (!RESCUE) {
+                    if (detect && !last) {
+                        // strip 0's last row all zero at columns 1..H: every
+                        // later row, and the FP32 sum, are exactly zero -- the
+                        // reference's result, which rescues the pair; the
+                        // other strips are not run
+                        const uint32_t any = __builtin_amdgcn_readlane(orbits, L - 2);
+                        if (l == 0) {
+                            if (any == 0u) {
+                                a.raw32[p] = 0.0f;
+                                a.flags[p] = 0x01 | kFlagDead;
+                                const uint32_t k = atomicAdd(a.rescue_count, 1u);
+                                a.rescue_list[k] = (uint32_t)p;
+                            } else {
+                                a.flags[p] = 0;
+                            }
+                        }
+                    }
+                }
                 if (last && l == lR) {
                     const T Ssum = sm + si;  // pairhmm_impl.cpp:434-437
                     if constexpr (RESCUE) {
@@ -1207,6 +1236,11 @@ __global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
             s1 = (int)(u.strips >> 16);
             xin = u.xin;
             xout = u.xout;
+            // the pair's FP32 sum is zero: found by the strip-0 launch, which
+            // completed before this one (every unit of the pair skips alike)
+            if (a.dead_mode == 2 &&
+                __builtin_amdgcn_readfirstlane((a.flags[p] & kFlagDead) ? 1 : 0))
+                continue;
         } else {
             p = a.items[it];
         }
@@ -1483,7 +1517,7 @@ __global__ void finalize_kernel(FinalizeArgs f) {
     uint8_t fl = f.flags[k];
     const bool inter32 = f.pair_inter && f.pair_inter[k];
     const bool inter64 = (fl & kFlagGrouped64) != 0;
-    fl &= (uint8_t)~kFlagGrouped64;
+    fl &= (uint8_t)~(kFlagGrouped64 | kFlagDead);
     double out;
     bool inter = false;
     if (fl & 0x01) {
