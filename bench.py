@@ -263,9 +263,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint", type=int, default=1)
     ap.add_argument("--prior-table", type=int, default=1)
-    ap.add_argument("--haplotype-chains", type=int, choices=(0, 1, 2), default=1,
-                    help="chained items: 1 where a length bucket is large enough (default), "
-                         "2 always, 0 one pair per slot")
+    ap.add_argument("--haplotype-chains", type=int, choices=(0, 1, 2, 3, 4), default=1,
+                    help="chained items: 1 where a length bucket is large enough, short reads "
+                         "two per lane group where that packs better (default), 2 always, "
+                         "3 and 4 as 1 and 2 with one read per lane group, 0 one pair per slot")
     ap.add_argument("--autotune", type=int, default=1,
                     help="tune pairs per wavefront / strip width / waves per CU on a 4%% sample "
                          "of the workload's distribution before timing")

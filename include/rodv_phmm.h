@@ -123,10 +123,12 @@ typedef struct rodv_phmm_config {
     int32_t haplotype_chains;     /* reads of a region that share their
                                      haplotypes swept across up to 16 of them in
                                      one item, without a pipeline drain between
-                                     haplotypes: 1 (default) where a length
-                                     bucket still fills the device with such
-                                     items, 2 always, 0 never (one pair per
-                                     slot)                                      */
+                                     haplotypes, short reads two to a lane group
+                                     where that packs their rows better: 1
+                                     (default) where a length bucket still fills
+                                     the device with such items, 2 always, 3
+                                     and 4 as 1 and 2 with one read per lane
+                                     group, 0 never (one pair per slot)         */
 } rodv_phmm_config;
 
 /* Counters of the last execution of a plan. */
@@ -140,6 +142,8 @@ typedef struct rodv_phmm_stats {
     double fp32_kernel_ms;   /* summed FP32 forward kernel time (profiled)  */
     double fp64_kernel_ms;   /* FP64 rescue kernel time (profiled)          */
     double total_ms;         /* whole execution (profiled)                  */
+    uint64_t stacked_pairs;  /* pairs in chained items with two reads per
+                                lane group                                  */
 } rodv_phmm_stats;
 
 typedef struct rodv_phmm_ctx rodv_phmm_ctx;

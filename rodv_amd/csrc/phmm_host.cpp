@@ -27,7 +27,7 @@ namespace rodv {
 void *fwd_f32_kernel(int L, int MR, bool inter);
 void *fwd_f64_rescue_kernel(bool inter);
 void *fwd_tab_kernel_ptr(int L, int MR, bool inter);
-void *fwd_chain_kernel_ptr(int L, int MR, bool inter);
+void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S);
 uint32_t fwd_tab_bytes(int L, int MR);
 int fwd_f64_rescue_mr();
 int rescue_long_waves();
@@ -43,6 +43,9 @@ hipError_t launch_rescued_cells(const FwdArgs &a, unsigned long long *out, hipSt
 hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, uint64_t n,
                        hipStream_t st);
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st);
+uint32_t rescue_tab_bytes(int MR);
+hipError_t launch_rescue_bin(const FwdArgs &a, int grid, hipStream_t st);
+hipError_t launch_rescue_tab(int part, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_ext(int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 int adj_max_mr();
 void *adj_kernel_ptr(int MR, bool rescue);
@@ -226,9 +229,10 @@ struct Bucket {
     bool inter = false;
     bool tab = false;        // table-prior kernel (N-free haplotypes, one strip)
     bool chain = false;      // chained items (fwd_chain_kernel); tab is set too
+    int stack = 1;           // chained items: reads per group (1, or 2 stacked in its rows)
     uint32_t tab_bytes = 0;  // LDS prior table per group
     void *kern = nullptr;
-    std::vector<int32_t> items;  // G entries per item
+    std::vector<int32_t> items;  // G entries per item (G * stack for chained items)
     // chained items: item i's haplotypes are chain_hap[chain_off[i] ..
     // chain_off[i+1]), items holds the G pairs of its first haplotype
     std::vector<uint32_t> chain_off;
@@ -350,6 +354,17 @@ struct rodv_phmm_plan {
     uint32_t fault_counter = 0;  // hand-over waits of the long-read kernels that ran out
     uint32_t spin_limit = 1u << 20;
     uint32_t rescue_lds = 0;
+    // FP64 rescue in classes (FwdArgs::bin): the bin list, per haplotype its
+    // N flag, the classes' offsets and capacities, the counters' base in
+    // d_counters (2 * kRescueClasses + 1 bin counts, then the two table
+    // launches' work counters), the table launches' grids and code stride
+    uint32_t *d_bin = nullptr;
+    uint8_t *d_hap_n = nullptr;
+    uint32_t bin_base[kRescueClasses + 1] = {};
+    uint32_t bin_cap[kRescueClasses + 1] = {};
+    uint32_t bin_counter = 0;
+    int rtab_grid[2] = {0, 0};
+    uint32_t rtab_stride = 0;
     // host copies needed at launch
     std::vector<Bucket> buckets;
     std::vector<PlainLaunch> plain;
@@ -392,7 +407,7 @@ void free_plan(rodv_phmm_plan *p) {
                     p->d_read_len, p->d_pair_hap, p->d_pair_read, p->d_pair_inter,
                     p->d_raw32, p->d_raw64, p->d_out, p->d_flags, p->d_rescue,
                     p->d_counters, p->d_bnd64, p->d_sched_list, p->d_sched_off, p->d_sched_scratch,
-                    p->d_rescued_cells, p->d_exp64, p->d_bndx};
+                    p->d_rescued_cells, p->d_exp64, p->d_bndx, p->d_bin, p->d_hap_n};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &b : p->buckets) {
@@ -559,28 +574,55 @@ bool codes_fit(uint32_t H, int L, int MR) {
     return (64ull / (uint64_t)L) * per <= 60u * 1024u;
 }
 
+// Rows per lane of a 16-lane group that holds two reads, of RA rows above
+// and RB rows below (fwd_chain_kernel's two-read layout: B ends in the last
+// slot, A in the last slot of lane (16 MR - RB - 1) / MR - 1, and the group's
+// first slot is an identity slot, which takes what the last lane rotates
+// into lane 0); 0 if more than 16 would be needed.  Every MR above the
+// returned one fits too.
+int stack_rows(uint32_t RA, uint32_t RB) {
+    for (int m = 1; m <= 16; m++) {
+        const int lo = (16 * m - (int)RB - 1) / m - 1;
+        if (lo >= 0 && (lo + 1) * m >= (int)RA + 1) return m;
+    }
+    return 0;
+}
+
+// Issue efficiency of rows read rows in a 16-lane group of MR rows per lane:
+// the padding to 16 MR slots times the step's 8 MR cell instructions over
+// those plus ~5 per step that do not scale with MR (the code address, the
+// row-above D, the two sums).
+double group_eff(uint32_t rows, int MR) {
+    return (double)rows / (16.0 * MR) * (8.0 * MR) / (8.0 * MR + 5.0);
+}
+
 // Chained items of one table bucket: reads of a region whose pairs in the
 // bucket cover the same haplotypes (two or more, each at least L + 2 long,
-// so that the kernel's hand-overs never overlap), G reads per item, swept
-// across those haplotypes longest first.  A chain is cut into segments of
-// at most kseg haplotypes, kseg as large as leaves at least `min_items`
-// items (0: no such bound) and at most kChainMaxHaps; a bucket too small
-// for kseg >= 2 keeps single pairs (long items in few waves would leave the
-// device idle).
-// `mem` holds the bucket's pairs; on return only those left as single
-// pairs.  Fills the chain bucket's items (the G pairs of each item's first
-// haplotype), chain_off, chain_hap, max_h and cells; items are ordered by
-// columns, most first.
+// so that the kernel's hand-overs never overlap), G groups of S reads per
+// item, swept across those haplotypes longest first.  A chain is cut into
+// segments of at most kseg haplotypes, kseg as large as leaves at least
+// `min_items` items (0: no such bound) and at most kChainMaxHaps; a bucket
+// too small for kseg >= 2 keeps single pairs (long items in few waves would
+// leave the device idle).
+// `mem` holds the bucket's pairs; on return only those left out.  S = 1
+// fills `bk`: items (the G pairs of each item's first haplotype), chain_off,
+// chain_hap, max_h and cells; items are ordered by columns, most first.
+// S = 2 (16 lanes): each group stacks two reads, an item takes the rows per
+// lane its groups need and goes to the bucket of `stacked` with that MR
+// (created from bk's L, inter and wpc); an item is formed only where its
+// group_eff beats the reads' one-read shapes by 3 %, the reads of the others
+// stay in `mem`.
 constexpr uint64_t kChainMaxHaps = 16;
 constexpr uint64_t kChainItemsPerWave = 16;
 void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, const uint32_t *pr,
                  const std::vector<uint32_t> &pair_region, const uint32_t *hap_len,
                  const uint32_t *read_len, const std::vector<uint64_t> &hapc_off,
-                 const std::vector<float> &initd_f, uint64_t min_items) {
+                 const std::vector<float> &initd_f, uint64_t min_items, int S = 1,
+                 std::vector<Bucket> *stacked = nullptr) {
     const size_t G = (size_t)(64 / bk.L);
     const uint32_t hmin = (uint32_t)bk.L + 2u;
     // LDS of the chain kernel: G prior tables and two code buffers, <= 60 KiB
-    const int64_t room = ((61440 - (int64_t)G * fwd_tab_bytes(bk.L, bk.MR)) / 2) & ~(int64_t)255;
+    const int64_t room = ((61440 - (int64_t)G * fwd_tab_bytes(bk.L, S == 1 ? bk.MR : 16)) / 2) & ~(int64_t)255;
     const int64_t hmax = room - (int64_t)lds_stride_for(0, bk.L) - 15;
     std::sort(mem.begin(), mem.end(), [&](uint32_t x, uint32_t y) {
         if (pair_region[x] != pair_region[y]) return pair_region[x] < pair_region[y];
@@ -624,8 +666,10 @@ void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, con
         }
         return true;
     };
-    // groups of G reads with the same haplotypes: (first run in `chained`)
-    std::vector<size_t> groups;
+    // items of G * S reads with the same haplotypes: (first run in
+    // `chained`, rows per lane)
+    const size_t GS = G * (size_t)S;
+    std::vector<std::pair<size_t, int>> groups;
     std::vector<Run> chained;
     uint64_t chain_pairs = 0;
     for (size_t i = 0; i < runs.size();) {
@@ -638,16 +682,48 @@ void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, con
             else
                 single.insert(single.end(), mem.begin() + runs[k].b, mem.begin() + runs[k].e);
         }
-        const size_t full = grp.size() / G * G;
-        for (size_t k = 0; k < full; k += G) {
-            groups.push_back(chained.size());
-            chained.insert(chained.end(), grp.begin() + k, grp.begin() + k + G);
-            chain_pairs += (uint64_t)G * (grp[0].e - grp[0].b);
+        if (S == 2)  // similar lengths stacked together
+            std::stable_sort(grp.begin(), grp.end(), [&](const Run &x, const Run &y) {
+                return read_len[pr[mem[x.b]]] > read_len[pr[mem[y.b]]];
+            });
+        const size_t full = grp.size() / GS * GS;
+        size_t k = 0;
+        for (; k < full; k += GS) {
+            int mr = bk.MR;
+            if (S == 2) {
+                // the item's rows per lane, and whether stacking pays
+                double e1 = 0, e2 = 0;
+                mr = 0;
+                for (size_t q = 0; q < G; q++) {
+                    const uint32_t RA = read_len[pr[mem[grp[k + 2 * q].b]]];
+                    const uint32_t RB = read_len[pr[mem[grp[k + 2 * q + 1].b]]];
+                    const int m = stack_rows(RA, RB);
+                    if (m == 0) {
+                        mr = 0;
+                        break;
+                    }
+                    mr = std::max(mr, std::max(m, 2));  // (no one-row instance)
+                }
+                if (mr) {
+                    for (size_t q = 0; q < G; q++) {
+                        const uint32_t RA = read_len[pr[mem[grp[k + 2 * q].b]]];
+                        const uint32_t RB = read_len[pr[mem[grp[k + 2 * q + 1].b]]];
+                        e1 += group_eff(RA, (int)(RA + 16) / 16) + group_eff(RB, (int)(RB + 16) / 16);
+                        e2 += 2.0 * group_eff(RA + RB, mr);
+                    }
+                    if (e2 < 1.03 * e1) mr = 0;
+                }
+                if (!mr) break;  // longest first: the rest of the group would not stack either
+            }
+            groups.push_back({chained.size(), mr});
+            chained.insert(chained.end(), grp.begin() + k, grp.begin() + k + GS);
+            chain_pairs += (uint64_t)GS * (grp[0].e - grp[0].b);
         }
-        for (size_t k = full; k < grp.size(); k++)
+        for (; k < grp.size(); k++)
             single.insert(single.end(), mem.begin() + grp[k].b, mem.begin() + grp[k].e);
         i = j;
     }
+    // (an item of S reads per group is S items' work: min_items / S of them)
     const uint64_t kseg = min_items ? std::min<uint64_t>(kChainMaxHaps, chain_pairs / ((uint64_t)G * min_items))
                                     : kChainMaxHaps;
     if (kseg < 2) {
@@ -660,15 +736,16 @@ void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, con
     struct Seg {
         uint64_t cols;
         size_t run;
+        int mr;
         uint32_t k0, k1;
     };
     std::vector<Seg> segs;
-    for (size_t gi : groups) {
-        const Run &r0 = chained[gi];
+    for (const auto &gi : groups) {
+        const Run &r0 = chained[gi.first];
         const uint32_t K = r0.e - r0.b;
         const uint32_t ns = (uint32_t)((K + kseg - 1) / kseg);
         for (uint32_t q = 0; q < ns; q++) {
-            Seg sg{0, gi, (uint32_t)((uint64_t)K * q / ns), (uint32_t)((uint64_t)K * (q + 1) / ns)};
+            Seg sg{0, gi.first, gi.second, (uint32_t)((uint64_t)K * q / ns), (uint32_t)((uint64_t)K * (q + 1) / ns)};
             for (uint32_t k = sg.k0; k < sg.k1; k++) sg.cols += hap_len[ph[mem[r0.b + k]]];
             segs.push_back(sg);
         }
@@ -676,7 +753,25 @@ void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, con
     std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cols > y.cols; });
     for (const Seg &sg : segs) {
         const Run *r = &chained[sg.run];
-        bk.chain_off.push_back((uint32_t)bk.chain_hap.size());
+        Bucket *tb = &bk;
+        if (S == 2) {
+            tb = nullptr;
+            for (Bucket &c : *stacked)
+                if (c.MR == sg.mr && c.inter == bk.inter) tb = &c;
+            if (!tb) {
+                Bucket c;
+                c.L = bk.L;
+                c.MR = sg.mr;
+                c.wpc = bk.wpc;
+                c.inter = bk.inter;
+                c.tab = true;
+                c.chain = true;
+                c.stack = 2;
+                stacked->push_back(std::move(c));
+                tb = &stacked->back();
+            }
+        }
+        tb->chain_off.push_back((uint32_t)tb->chain_hap.size());
         for (uint32_t k = sg.k0; k < sg.k1; k++) {
             const uint32_t h = ph[mem[r[0].b + k]];
             ChainHap e{};
@@ -684,16 +779,18 @@ void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, con
             e.initd = initd_f[h];
             e.hoff = hapc_off[h];
             e.pdelta = (int32_t)(mem[r[0].b + k] - mem[r[0].b + sg.k0]);
-            bk.chain_hap.push_back(e);
-            bk.max_h = std::max(bk.max_h, e.H);
+            tb->chain_hap.push_back(e);
+            tb->max_h = std::max(tb->max_h, e.H);
         }
-        for (size_t g = 0; g < G; g++) {
-            bk.items.push_back((int32_t)mem[r[g].b + sg.k0]);
+        for (size_t g = 0; g < GS; g++) {
+            tb->items.push_back((int32_t)mem[r[g].b + sg.k0]);
             for (uint32_t k = r[g].b + sg.k0; k < r[g].b + sg.k1; k++)
-                bk.cells += (uint64_t)read_len[pr[mem[k]]] * hap_len[ph[mem[k]]];
+                tb->cells += (uint64_t)read_len[pr[mem[k]]] * hap_len[ph[mem[k]]];
         }
     }
-    if (!bk.chain_off.empty()) bk.chain_off.push_back((uint32_t)bk.chain_hap.size());
+    // (the stacked buckets take their closing offset from the caller, after
+    // every table bucket has added its items)
+    if (S == 1 && !bk.chain_off.empty()) bk.chain_off.push_back((uint32_t)bk.chain_hap.size());
     mem.swap(single);
 }
 
@@ -981,7 +1078,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (cfg.short_lanes != 16 && cfg.short_lanes != 64) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.waves_per_cu < 0 || cfg.waves_per_cu > 32) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.simd_width_float > 64 || cfg.simd_width_double > 64) return RODV_PHMM_ERR_INVALID_ARG;
-    if (cfg.haplotype_chains < 0 || cfg.haplotype_chains > 2) return RODV_PHMM_ERR_INVALID_ARG;
+    if (cfg.haplotype_chains < 0 || cfg.haplotype_chains > 4) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.extended_range != 0 && cfg.extended_range != 1) return RODV_PHMM_ERR_INVALID_ARG;
     if (!(cfg.max_idle_ratio_float == cfg.max_idle_ratio_float) ||
         !(cfg.max_idle_ratio_double == cfg.max_idle_ratio_double))
@@ -1158,9 +1255,39 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     // ---- chained items: a bucket of their own next to each table bucket
     if (cfg.haplotype_chains) {
         const size_t nb0 = p->buckets.size();
+        // two reads per group where that packs a short read's rows better
+        // (16-lane buckets of reads up to 127 rows), then one read per group
+        std::vector<Bucket> stacked;
+        // modes 1 and 3: chains only where the bucket keeps the device full;
+        // 2 and 4: always; 3 and 4 keep one read per group
+        const bool always = cfg.haplotype_chains == 2 || cfg.haplotype_chains == 4;
+        const bool stack = cfg.haplotype_chains <= 2;
+        for (size_t bi = 0; bi < nb0 && stack; bi++) {
+            const Bucket &b0 = p->buckets[bi];
+            if (!b0.tab || b0.L != 16 || b0.MR > 8) continue;
+            Bucket tmpl;
+            tmpl.L = b0.L;
+            tmpl.MR = b0.MR;
+            tmpl.wpc = b0.wpc;
+            tmpl.inter = b0.inter;
+            const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu
+                                                   : (tmpl.wpc > 0 ? tmpl.wpc : kDefaultWavesPerCU);
+            form_chains(members[bi], tmpl, ph.data(), pr.data(), pair_region, b->hap_lengths,
+                        b->read_lengths, hapc_off, initd_f,
+                        always ? 0 : kChainItemsPerWave * (uint64_t)ctx->num_cus * (uint64_t)wpc,
+                        2, &stacked);
+        }
+        for (Bucket &c : stacked) {
+            c.chain_off.push_back((uint32_t)c.chain_hap.size());
+            c.kern = fwd_chain_kernel_ptr(c.L, c.MR, c.inter, 2);
+            if (!c.kern) return RODV_PHMM_ERR_UNSUPPORTED;
+            p->buckets.push_back(std::move(c));
+            members.emplace_back();
+            plain_mem.emplace_back();
+        }
         for (size_t bi = 0; bi < nb0; bi++) {
             if (!p->buckets[bi].tab) continue;
-            void *ck = fwd_chain_kernel_ptr(p->buckets[bi].L, p->buckets[bi].MR, p->buckets[bi].inter);
+            void *ck = fwd_chain_kernel_ptr(p->buckets[bi].L, p->buckets[bi].MR, p->buckets[bi].inter, 1);
             if (!ck) continue;
             Bucket cb;
             cb.L = p->buckets[bi].L;
@@ -1175,7 +1302,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                                                    : (cb.wpc > 0 ? cb.wpc : kDefaultWavesPerCU);
             form_chains(members[bi], cb, ph.data(), pr.data(), pair_region, b->hap_lengths,
                         b->read_lengths, hapc_off, initd_f,
-                        cfg.haplotype_chains == 2 ? 0 : kChainItemsPerWave * (uint64_t)ctx->num_cus * (uint64_t)wpc);
+                        always ? 0 : kChainItemsPerWave * (uint64_t)ctx->num_cus * (uint64_t)wpc);
             if (cb.chain_off.empty()) continue;
             p->buckets.push_back(std::move(cb));
             members.emplace_back();
@@ -1353,6 +1480,33 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (!rc) rc = plan_alloc(p, &p->d_out, n_pairs);
     if (!rc) rc = plan_alloc(p, &p->d_flags, n_pairs);
     if (!rc) rc = plan_alloc(p, &p->d_rescue, n_pairs);
+    {
+        // FP64 rescue classes: a pair's capacity is reserved in its class
+        uint64_t ncls[kRescueClasses + 1] = {};
+        uint32_t max_h_tab = 0;
+        for (uint64_t k = 0; k < n_pairs; k++) {
+            const uint32_t H = b->hap_lengths[ph[k]];
+            const int c = rescue_class(b->read_lengths[pr[k]], H, hap_has_n[ph[k]] != 0);
+            ncls[c < 0 ? kRescueClasses : c]++;
+            if (c >= 0) max_h_tab = std::max(max_h_tab, H);
+        }
+        uint64_t acc = 0;
+        for (int c = 0; c <= kRescueClasses; c++) {
+            p->bin_base[c] = (uint32_t)acc;
+            p->bin_cap[c] = (uint32_t)ncls[c];
+            acc += ncls[c];
+        }
+        uint64_t lo = 0, hi = 0;
+        for (int c = 0; c < kRescueClasses; c++) (c < 4 ? lo : hi) += ncls[c];
+        // two pairs per wave: 2 (classes 4..7, 255 VGPRs) or 3 (0..3) waves per SIMD
+        p->rtab_grid[0] = (int)std::min<uint64_t>((hi + 1) / 2, (uint64_t)ctx->num_cus * 8);
+        p->rtab_grid[1] = (int)std::min<uint64_t>((lo + 1) / 2, (uint64_t)ctx->num_cus * 12);
+        p->rtab_stride = lds_stride_for(max_h_tab, 32);
+        if (!rc && (lo || hi)) {
+            rc = plan_alloc(p, &p->d_bin, n_pairs);
+            if (!rc) rc = plan_upload(p, &p->d_hap_n, hap_has_n.data(), M);
+        }
+    }
     for (auto &bk : p->buckets) {
         if (!rc) rc = plan_upload(p, &bk.d_items, bk.items.data(), bk.items.size());
         if (!rc && bk.chain) {
@@ -1424,6 +1578,8 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                            (size_t)p->n_x64 * p->bnd64_elems * sizeof(double));
         if (!rc) rc = plan_upload(p, &p->d_long_units, p->long_units.data(), p->long_units.size());
     }
+    p->bin_counter = p->n_counters;
+    p->n_counters += 2 * kRescueClasses + 1 + 2;
     if (cfg.extended_range) {
         // one wave per pair whose FP64 sum underflows, with two boundary
         // rows of the longest haplotype
@@ -1462,6 +1618,10 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     p->stats.num_pairs = n_pairs;
     p->stats.cells = p->cells;
     p->stats.num_buckets = (uint32_t)p->buckets.size();
+    for (const Bucket &bk : p->buckets)
+        if (bk.stack == 2)
+            for (size_t k = 0; k + 1 < bk.chain_off.size(); k++)
+                p->stats.stacked_pairs += (uint64_t)(bk.chain_off[k + 1] - bk.chain_off[k]) * (bk.items.size() / (bk.chain_off.size() - 1));
     *out = p;
     p = nullptr;  // the caller owns it now
     return RODV_PHMM_OK;
@@ -1501,6 +1661,14 @@ FwdArgs base_args(rodv_phmm_plan *p) {
     a.rescue_list = p->d_rescue;
     a.rescue_count = p->d_counters;
     a.exp64 = p->d_exp64;
+    a.bin = p->d_bin;
+    a.bin_cnt = p->d_counters + p->bin_counter;
+    a.hap_n = p->d_hap_n;
+    for (int c = 0; c <= kRescueClasses; c++) {
+        a.bin_base[c] = p->bin_base[c];
+        a.bin_cap[c] = p->bin_cap[c];
+    }
+    a.bin_route = p->any_inter ? 1u : 0u;
     return a;
 }
 
@@ -1614,6 +1782,23 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
         a.lds_stride = p->rescue_lds;
         a.bnd = p->d_bnd64;
         a.bnd_elems = p->bnd64_elems;
+        if (p->d_bin) {
+            // rescued pairs of the table classes two per wave, the others
+            // (bin list kRescueClasses) one per wave below
+            HIPCHK(launch_rescue_bin(a, std::max(1, (int)p->ctx->num_cus), st));
+            launches++;
+            for (int part = 0; part < 2; part++) {
+                if (!p->rtab_grid[part]) continue;
+                FwdArgs t = a;
+                t.counter = a.bin_cnt + 2 * kRescueClasses + 1 + part;
+                t.lds_stride = p->rtab_stride;
+                const size_t lds = 2u * (p->rtab_stride + rescue_tab_bytes(part == 0 ? 8 : 4));
+                HIPCHK(launch_rescue_tab(part, p->rtab_grid[part], lds, st, t));
+                launches++;
+            }
+            a.rescue_list = p->d_bin + p->bin_base[kRescueClasses];
+            a.n_items_dev = a.bin_cnt + 2 * kRescueClasses;
+        }
         HIPCHK(launch_fwd(fwd_f64_rescue_kernel(p->any_inter), p->rescue_grid, p->rescue_lds, st, a));
         launches++;
         if (p->long_grid) {  // the rescued reads longer than one strip

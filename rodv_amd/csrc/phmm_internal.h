@@ -120,7 +120,28 @@ struct FwdArgs {
     // finds the pairs whose last row of strip 0 is all zero (kFlagDead);
     // 2 = the other strips, skipping those pairs; 0 = one launch
     uint32_t dead_mode;
+    // FP64 rescue in classes.  Class c < kRescueClasses: reads of the 32-lane
+    // table pass with c + 1 rows per lane; its bin_cap[c] entries from
+    // bin[bin_base[c]] hold its inter pairs from the front (bin_cnt[c] of
+    // them) and its intra pairs from the back (bin_cnt[kRescueClasses + c]).
+    // The other rescued pairs are bin[bin_base[kRescueClasses] ..), counted
+    // in bin_cnt[2 * kRescueClasses].
+    uint32_t *bin;
+    uint32_t *bin_cnt;
+    const uint8_t *hap_n;  // 1: the haplotype has an N
+    uint32_t bin_base[9];
+    uint32_t bin_cap[9];
+    uint32_t bin_route;    // 1: a pair the FP64 regroup grouped takes the inter arithmetic
 };
+
+// The FP64 rescue's table pass: 32 lanes per pair, reads of at most
+// 32 * kRescueClasses - 1 rows, haplotypes without N and of at most
+// kRescueTabMaxH columns (constexpr: host and device code call it).
+constexpr int kRescueClasses = 8;
+constexpr uint32_t kRescueTabMaxH = 16384;
+constexpr int rescue_class(uint32_t R, uint32_t H, bool hap_n) {
+    return (R <= 32u * kRescueClasses - 1 && H <= kRescueTabMaxH && !hap_n) ? (int)(R / 32u) : -1;
+}
 
 // Flag bit used between the passes of a plan: the FP64 regroup of
 // schedule_pairhmm put this FP32-rejected inter pair in an inter FP64 group.

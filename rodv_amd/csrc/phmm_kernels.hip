@@ -663,7 +663,60 @@ __device__ __forceinline__ void put_raw32(const FwdArgs &a, int p, float Ssum) {
     a.flags[p] = fl;
 }
 
-template <int L, int MR, bool INTER>
+// Where the NS reads of a group sit in its L * MR slots (slot l * MR + s is
+// row slot s of lane l).  NS = 1: the read's row R in the last slot of lane
+// lo[0], after off = (MR - R % MR) % MR identity slots at the top (the table
+// kernel's layout).  NS = 2: read B ends at the group's last slot, read A at
+// the last slot of lane lo[0] = (L * MR - R_B - 1) / MR - 1, so that at least
+// one identity slot lies between them; that slot is read B's row 0 (M = I =
+// 0, D = K/H at every column, like the slots above a single read).  The
+// planner also leaves the group's first slot an identity slot (lane L-1
+// holds read B's last row, which the rotation brings into lane 0).  The
+// lane holding a read's last row accumulates that pair's sum.
+template <int L, int MR, int NS>
+struct GroupRows {
+    int st[NS], en[NS];  // first and last slot of each read
+    int lo[NS];         // the lane of each read's last row
+    int p0[NS];         // the read's pair for the chain's first haplotype (-1: none)
+    uint64_t ro[NS];
+    __device__ __forceinline__ GroupRows(const ItemDesc *d) {
+        if constexpr (NS == 1) {
+            const bool v = d[0].p >= 0;
+            const int R = v ? (int)d[0].R : 0;
+            const int off = v ? (MR - R % MR) % MR : 0;
+            st[0] = off;
+            en[0] = off + R - 1;
+            lo[0] = v ? (R - 1 + off) / MR : 0;
+            p0[0] = d[0].p;
+            ro[0] = d[0].ro;
+        } else {
+            const bool vb = d[1].p >= 0;
+            const int RB = vb ? (int)d[1].R : 0;
+            st[1] = L * MR - RB;
+            en[1] = L * MR - 1;
+            lo[1] = L - 1;
+            p0[1] = d[1].p;
+            ro[1] = d[1].ro;
+            const bool va = d[0].p >= 0;
+            const int RA = va ? (int)d[0].R : 0;
+            lo[0] = (L * MR - RB - 1) / MR - 1;
+            en[0] = (lo[0] + 1) * MR - 1;
+            st[0] = en[0] - RA + 1;
+            p0[0] = d[0].p;
+            ro[0] = d[0].ro;
+        }
+    }
+    // the read of slot i (-1: identity slot)
+    __device__ __forceinline__ int read_of(int i) const {
+        int r = -1;
+#pragma unroll
+        for (int j = 0; j < NS; j++)
+            if (p0[j] >= 0 && i >= st[j] && i <= en[j]) r = j;
+        return r;
+    }
+};
+
+template <int L, int MR, bool INTER, int NS>
 __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     constexpr int G = 64 / L;
     constexpr int MRT = TabLayout<MR>::MRT;
@@ -674,11 +727,21 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     const int l = lane % L;
     const uint32_t c0 = __builtin_amdgcn_readfirstlane(a.chain_off[it]);
     const uint32_t K = __builtin_amdgcn_readfirstlane(a.chain_off[it + 1]) - c0;
-    const ItemDesc d = a.desc[(uint64_t)it * G + g];
-    const bool valid = d.p >= 0;
-    const int R = (int)d.R;
-    const int off = valid ? (MR - R % MR) % MR : 0;
-    const int lR = valid ? (R - 1 + off) / MR : 0;
+    ItemDesc dv[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) dv[j] = a.desc[((uint64_t)it * G + g) * NS + j];
+    const GroupRows<L, MR, NS> gr(dv);
+    const float initd0 = dv[0].initd;  // one haplotype for the group's reads
+    // this lane's pair (for the chain's first haplotype) if it holds a read's
+    // last row, else -1; the group's first and last such lanes
+    int own = -1, lmin = 0x7fffffff, lmax = -1;
+#pragma unroll
+    for (int j = 0; j < NS; j++)
+        if (gr.p0[j] >= 0) {
+            if (l == gr.lo[j]) own = gr.p0[j];
+            lmin = min(lmin, gr.lo[j]);
+            lmax = max(lmax, gr.lo[j]);
+        }
     uint8_t *tab = lds + (uint32_t)g * 4u * kBlk;
     uint8_t *buf = lds + (uint32_t)G * 4u * kBlk;  // two code buffers of lds_stride bytes
 
@@ -686,9 +749,19 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     bool realv[MR];
 #pragma unroll
     for (int s = 0; s < MR; s++) {
-        const int i = l * MR + s - off + 1;
-        realv[s] = valid && i >= 1 && i <= R;
-        raw[s] = load_raw(a, d.ro + (uint64_t)(realv[s] ? i - 1 : 0));
+        const int i = l * MR + s;
+        const int j = gr.read_of(i);
+        realv[s] = j >= 0;
+        // (selects, not an index: the arrays stay in registers)
+        uint64_t ro = gr.ro[0];
+        int st = gr.st[0];
+#pragma unroll
+        for (int q = 1; q < NS; q++)
+            if (j == q) {
+                ro = gr.ro[q];
+                st = gr.st[q];
+            }
+        raw[s] = load_raw(a, ro + (uint64_t)(j >= 0 ? i - st : 0));
     }
     const int H0 = __builtin_amdgcn_readfirstlane((int)a.chain_hap[c0].H);
     const uint32_t hoff0_lo = __builtin_amdgcn_readfirstlane((uint32_t)a.chain_hap[c0].hoff);
@@ -721,7 +794,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     for (int s = 0; s < MR; s++) {
         S.M[s] = 0.0f;
         S.I[s] = 0.0f;
-        S.D[s] = realv[s] ? 0.0f : d.initd;
+        S.D[s] = realv[s] ? 0.0f : initd0;
     }
     const LaneRot<L> rot;
     Carry C;
@@ -735,8 +808,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
 
     const uint8_t *cl = buf + (P0 - 1) - l;  // code of step x: cl[x]
     const uint8_t *tlane = tab + (uint32_t)l * 16;
-    const int p0 = d.p;
-    int p = p0;  // this group's pair for the current haplotype
+    int pd = 0;  // pair offset of the current haplotype: own + pd is its pair
     float sm = 0.0f, si = 0.0f;
     int dd = 1;
     int Sk = 0;  // columns of the haplotypes before the current one
@@ -744,6 +816,42 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     float pa[MR];
     load_priors<L, MR>(pa, tlane + ld_code(cl + dd) * kBlk);
     uint32_t c1 = ld_code(cl + dd + 1);
+    if constexpr (NS == 2) {
+        // Read B's first row lies below an identity slot in a lane l > 0,
+        // which computes columns 1 - l .. 0 first: there the identity row's
+        // D = K/H makes M nonzero.  So every lane enters the first
+        // haplotype as at a hand-over from one that ends at column 0: its
+        // state is set to the column-0 boundary in the step that computes
+        // its column 1 (the first haplotype has at least L + 2 columns).
+        float d0[MR];
+#pragma unroll
+        for (int s = 0; s < MR; s++) d0[s] = realv[s] ? 0.0f : initd0;
+        const float dD0 = rot(d0[MR - 1]);
+        for (int u = 0; u < L; u++) {
+            const uint32_t c2 = ld_code(cl + dd + 2);
+            if (dd - l == 1) {
+                sm = 0.0f;
+                si = 0.0f;
+#pragma unroll
+                for (int s = 0; s < MR; s++) {
+                    S.M[s] = 0.0f;
+                    S.I[s] = 0.0f;
+                    S.D[s] = d0[s];
+                }
+                C.dM = 0.0f;
+                C.dI = 0.0f;
+                C.dD = dD0;
+            }
+            float pb[MR];
+            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+#pragma unroll
+            for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+            c1 = c2;
+            dd++;
+        }
+    }
     for (uint32_t k = 0; k + 1 < K; k++) {
         const ChainHap *hn = a.chain_hap + c0 + k + 1;
         const int Hn = __builtin_amdgcn_readfirstlane((int)hn->H);
@@ -784,10 +892,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
         // enters haplotype k+1 at step Sn + 1 + l
         float dres[MR];
 #pragma unroll
-        for (int s = 0; s < MR; s++) {
-            const int i = l * MR + s - off + 1;
-            dres[s] = (valid && i >= 1 && i <= R) ? 0.0f : initdn;
-        }
+        for (int s = 0; s < MR; s++) dres[s] = gr.read_of(l * MR + s) >= 0 ? 0.0f : initdn;
         const float dDres = rot(dres[MR - 1]);
         // steps Sn - 1 .. Sn + L, two at a time with the prior registers
         // swapping roles (no copies)
@@ -795,7 +900,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
             if (dd + 1 - l == Sn) cl = cln;
             const uint32_t c2 = ld_code(cl + dd + 2);
             const bool cross = dd - l == Sn + 1;
-            if (cross && valid && l == lR) put_raw32(a, p, sm + si);  // pairhmm_impl.cpp:434-437
+            if (cross && own >= 0) put_raw32(a, own + pd, sm + si);  // pairhmm_impl.cpp:434-437
             if (cross) {
                 sm = 0.0f;
                 si = 0.0f;
@@ -832,11 +937,11 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
         }
         Sk = Sn;
         H = Hn;
-        p = p0 + pdn;
+        pd = pdn;
     }
     // the last haplotype ends like a single pair
-    const int key = valid ? Sk + H + lR : 0x7fffffff;
-    const int keymax = valid ? Sk + H + lR : -1;
+    const int key = lmax >= 0 ? Sk + H + lmin : 0x7fffffff;
+    const int keymax = lmax >= 0 ? Sk + H + lmax : -1;
     const int Dmain = wave_min4(key, G);
     const int Dend = wave_max4(keymax, G);
     for (; dd + kU - 1 <= Dmain; dd += kU) {
@@ -864,7 +969,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
         for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
         c1 = c2;
     }
-    if (valid && l == lR) put_raw32(a, p, sm + si);
+    if (own >= 0) put_raw32(a, own + pd, sm + si);
     // every lane has finished its LDS reads before the next item rewrites
     // the table and the first code buffer
     wave_lds_sync();
@@ -874,7 +979,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
 // Persistent, like fwd_tab_kernel, over chained items.  LDS: the G prior
 // tables, then the two code buffers, which start all zero (prefetches past
 // a haplotype read codes left there, always a valid block 0..3).
-template <int L, int MR, bool INTER>
+template <int L, int MR, bool INTER, int NS>
 __global__ __launch_bounds__(64) void fwd_chain_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int G = 64 / L;
@@ -886,7 +991,7 @@ __global__ __launch_bounds__(64) void fwd_chain_kernel(FwdArgs a) {
     uint32_t it = 0;
     if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
     it = __builtin_amdgcn_readfirstlane(it);
-    for (uint32_t iter = 0; iter < n && it < n; iter++) it = process_chain<L, MR, INTER>(a, lds, it);
+    for (uint32_t iter = 0; iter < n && it < n; iter++) it = process_chain<L, MR, INTER, NS>(a, lds, it);
 }
 
 template <typename T, int L, int MR, bool INTER, bool RESCUE>
@@ -1284,6 +1389,322 @@ __global__ __launch_bounds__(64) void fwd_rescue_route_kernel(FwdArgs a) {
     }
 }
 
+// ------------------------------------------------- FP64 rescue, prior table
+// Rescued pairs of reads of at most 255 rows and N-free haplotypes, two per
+// wave: 32 lanes per pair in the table kernel's layout (every lane holds read
+// rows; when R % MR != 0 the top slot of lane 0 is an identity slot) with
+// MR = ceil((R + 1) / 32) rows per lane.  The emission priors of a pair come
+// from its LDS table (two doubles per 16-byte chunk, a lane's chunks for one
+// column read with ds_read_b128) instead of a select per cell, and the
+// transfers between lanes are ds_bpermute.  The two pairs of a wave share MR
+// and the arithmetic (inter or intra), so the rescued pairs are binned first
+// (rescue_bin_kernel).  Every read cell is computed with process_item's
+// instructions in its order on the same tables: the sums are the same bits.
+constexpr int kRtL = 32;
+
+template <int MR>
+struct TabLayout64 {
+    static constexpr int MRT = (MR + 1) / 2 * 2;  // slots padded to 16 bytes
+};
+
+template <int MR>
+__device__ __forceinline__ void load_priors64(double (&pr)[MR], const uint8_t *p) {
+    constexpr int MRT = TabLayout64<MR>::MRT;
+    const double2 *tp = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+    for (int q = 0; q < MRT / 2; q++) {
+        const double2 v = tp[q * kRtL];
+        if (2 * q + 0 < MR) pr[2 * q + 0] = v.x;
+        if (2 * q + 1 < MR) pr[2 * q + 1] = v.y;
+    }
+}
+
+// The carried row above the lane's first row, as Carry (FP32) holds it.
+struct Carry64 {
+    double dM, dI, dD;
+    double uM, uI;
+    double yyA, myA;
+};
+
+template <int MR, bool INTER>
+__device__ __forceinline__ void step_tab64(State<double, MR> &S, Carry64 &C, const Priors<double, MR> &P,
+                                           const double (&pr)[MR], const LaneRot<kRtL> &rot,
+                                           double (&pb)[MR], const uint8_t *pnext) {
+    load_priors64<MR>(pb, pnext);
+    double nM[MR], nD[MR];
+    double dgM = C.dM, dgI = C.dI, dgD = C.dD;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        double t;
+        if (INTER) {  // pairhmm_inter.cpp:279-283
+            t = dgI * P.g[s];
+            t = fmaT(dgM, P.mm[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        } else {  // pairhmm_impl.cpp:266-275
+            t = dgM * P.mm[s];
+            t = fmaT(dgI, P.g[s], t);
+            t = fmaT(dgD, P.g[s], t);
+        }
+        nM[s] = t * pr[s];
+        const double v = S.M[s] * P.my[s];
+        nD[s] = fmaT(S.D[s], P.yy[s], v);
+        dgM = S.M[s];
+        dgI = S.I[s];
+        dgD = S.D[s];
+    }
+    double upM = C.uM, upI = C.uI;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const double u = upM * P.mx[s];
+        const double nI = fmaT(upI, P.xx[s], u);
+        upM = nM[s];
+        upI = nI;
+        S.I[s] = nI;
+        S.M[s] = nM[s];
+        S.D[s] = nD[s];
+    }
+    C.dD = fmaT(C.dD, C.yyA, C.dM * C.myA);
+    C.dM = C.uM;
+    C.dI = C.uI;
+    C.uM = rot(S.M[MR - 1]);
+    C.uI = rot(S.I[MR - 1]);
+}
+
+// One item of two pairs (p < 0: an empty slot).  LDS per group: the codes
+// (lds_stride bytes) and the table, 4 blocks of 32 * MRT doubles.
+template <int MR, bool INTER>
+__device__ void rescue_tab_item(const FwdArgs &a, uint8_t *lds, int p) {
+    constexpr int L = kRtL;
+    constexpr int G = 64 / L;
+    constexpr int MRT = TabLayout64<MR>::MRT;
+    constexpr uint32_t kBlk = (uint32_t)L * MRT * 8;
+    constexpr int kU64 = 4;
+    const int lane = threadIdx.x;
+    const int g = lane / L;
+    const int l = lane % L;
+    const bool valid = p >= 0;
+    uint32_t h = 0, r = 0;
+    int R = 0, H = 0;
+    uint64_t ro = 0, hoff = 0;
+    double initd = 0.0;
+    if (valid) {
+        h = a.pair_hap[p];
+        r = a.pair_read[p];
+        R = (int)a.read_len[r];
+        H = (int)a.hap_len[h];
+        ro = a.read_off[r];
+        hoff = a.hapc_off[h];
+        initd = a.initd_d[h];
+    }
+    const int off = valid ? (MR - R % MR) % MR : 0;
+    const int lR = valid ? (R - 1 + off) / MR : 0;
+    uint8_t *codes = lds + (uint32_t)g * (a.lds_stride + 4u * kBlk);
+    uint8_t *tab = codes + a.lds_stride;
+
+    for (uint32_t k = (uint32_t)l * 16; k < a.lds_stride; k += L * 16)
+        *reinterpret_cast<uint4 *>(codes + k) = make_uint4(0, 0, 0, 0);
+    wave_lds_sync();
+    RowRaw raw[MR];
+    bool realv[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int i = l * MR + s - off + 1;
+        realv[s] = valid && i >= 1 && i <= R;
+        raw[s] = load_raw(a, ro + (uint64_t)(realv[s] ? i - 1 : 0));
+    }
+    if (valid) {  // column j at codes[L-1+j], block 0..3 per byte
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + hoff);
+        const int nch = (H + 15) >> 4;
+        for (int c = l; c < nch; c += L) {
+            const uint4 v = src[c];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            uint32_t o[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[k] = ((w[k] >> 1) & 0x7f7f7f7fu) - ((w[k] >> 3) & 0x1f1f1f1fu);
+            *reinterpret_cast<uint4 *>(codes + L + 16 * c) = make_uint4(o[0], o[1], o[2], o[3]);
+        }
+    }
+    Priors<double, MR> P;
+    double X[MR], Y[MR];
+    uint32_t rc[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        row_priors<double, MR, INTER>(a, P, s, realv[s], raw[s]);
+        X[s] = P.X[s];
+        Y[s] = P.Y[s];
+        rc[s] = P.rpos[s];
+    }
+    double *tl = reinterpret_cast<double *>(tab + (uint32_t)l * 16);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int s = 0; s < MR; s++)
+            tl[c * (kBlk / 8) + (s >> 1) * (L * 2) + (s & 1)] = (rc[s] == (uint32_t)c) ? X[s] : Y[s];
+    }
+    wave_lds_sync();
+
+    State<double, MR> S;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        S.M[s] = 0.0;
+        S.I[s] = 0.0;
+        S.D[s] = realv[s] ? 0.0 : initd;
+    }
+    const LaneRot<L> rot;
+    Carry64 C;
+    C.dM = 0.0;
+    C.dI = 0.0;
+    C.dD = rot(S.D[MR - 1]);
+    C.uM = 0.0;
+    C.uI = 0.0;
+    C.yyA = rot(P.yy[MR - 1]);
+    C.myA = rot(P.my[MR - 1]);
+
+    const int key = valid ? H + lR : 0x7fffffff;
+    const int keymax = valid ? H + lR : -1;
+    const int Dmain = wave_min4(key, G);
+    const int Dend = wave_max4(keymax, G);
+    const uint8_t *cl = codes + (L - 1) - l;
+    const uint8_t *tlane = tab + (uint32_t)l * 16;
+    double sm = 0.0, si = 0.0;
+    int dd = 1;
+    double pa[MR];
+    load_priors64<MR>(pa, tlane + ld_code(cl + dd) * kBlk);
+    uint32_t c1 = ld_code(cl + dd + 1);
+    for (; dd + kU64 - 1 <= Dmain; dd += kU64) {
+#pragma unroll
+        for (int u = 0; u < kU64; u++) {
+            const uint32_t c2 = ld_code(cl + dd + u + 2);
+            double pb[MR];
+            step_tab64<MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+#pragma unroll
+            for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+            c1 = c2;
+        }
+    }
+    for (; dd <= Dend; dd++) {
+        const uint32_t c2 = ld_code(cl + dd + 2);
+        double pb[MR];
+        step_tab64<MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+        if (dd - l <= H) {
+            sm += S.M[MR - 1];
+            si += S.I[MR - 1];
+        }
+#pragma unroll
+        for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+        c1 = c2;
+    }
+    if (valid && l == lR) a.raw64[p] = sm + si;  // pairhmm_impl.cpp:434-437
+    wave_lds_sync();
+}
+
+// v[k] for a wave-uniform k, without indexing the kernel argument at run time
+__device__ __forceinline__ uint32_t pick9(const uint32_t (&v)[9], int k) {
+    uint32_t r = v[0];
+#pragma unroll
+    for (int q = 1; q < 9; q++) r = k == q ? v[q] : r;
+    return r;
+}
+
+// Bins the rescue list (rescue_count entries, appended by the FP32 pass) for
+// the FP64 passes.  Pairs of one class and arithmetic in a wave take one
+// atomic per wave and class.
+__global__ __launch_bounds__(256) void rescue_bin_kernel(FwdArgs a) {
+    const uint32_t n = *a.rescue_count;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // a whole wave walks the loop together (n is the same for all lanes)
+    const uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint32_t e0 = wbase; e0 < n; e0 += stride) {
+        const uint32_t e = e0 + lane;
+        int key = -1;  // 0..7 inter, 8..15 intra, 16 other
+        uint32_t p = 0;
+        if (e < n) {
+            p = a.rescue_list[e];
+            const uint32_t R = a.read_len[a.pair_read[p]];
+            const uint32_t hh = a.pair_hap[p];
+            const int c = rescue_class(R, a.hap_len[hh], a.hap_n[hh] != 0);
+            if (c < 0)
+                key = 2 * kRescueClasses;
+            else
+                key = (a.bin_route && (a.flags[p] & kFlagGrouped64)) ? c : kRescueClasses + c;
+        }
+        uint64_t todo = __ballot(key >= 0);
+        while (todo) {
+            const int lead = __builtin_ctzll(todo);
+            const int k = __builtin_amdgcn_readlane(key, lead);
+            const uint64_t mk = __ballot(key == k);
+            uint32_t b = 0;
+            if ((int)lane == lead) b = atomicAdd(a.bin_cnt + k, (uint32_t)__popcll(mk));
+            b = (uint32_t)__builtin_amdgcn_readlane((int)b, lead);
+            if (key == k) {
+                const uint32_t slot = b + (uint32_t)__popcll(mk & below);
+                const int c = k < kRescueClasses ? k : k < 2 * kRescueClasses ? k - kRescueClasses : kRescueClasses;
+                const uint32_t base = pick9(a.bin_base, c);
+                a.bin[k >= kRescueClasses && k < 2 * kRescueClasses ? base + pick9(a.bin_cap, c) - 1 - slot
+                                                                     : base + slot] = p;
+            }
+            todo &= ~mk;
+        }
+    }
+}
+
+// Items of the table pass: two pairs of one class and arithmetic, classes
+// c in [c_lo, c_hi] with the most rows per lane first, inter before intra.
+template <int Q, int C_HI>
+__device__ __forceinline__ void rescue_tab_switch(const FwdArgs &a, uint8_t *lds, int c, bool inter,
+                                                  uint32_t e, uint32_t ck) {
+    if (c == Q) {
+        int p = -1;
+        if (e < ck) p = (int)a.bin[inter ? a.bin_base[Q] + e : a.bin_base[Q] + a.bin_cap[Q] - 1 - e];
+        if (inter)
+            rescue_tab_item<Q + 1, true>(a, lds, p);
+        else
+            rescue_tab_item<Q + 1, false>(a, lds, p);
+    } else if constexpr (Q < C_HI) {
+        rescue_tab_switch<Q + 1, C_HI>(a, lds, c, inter, e, ck);
+    }
+}
+
+template <int C_LO, int C_HI>
+__global__ __launch_bounds__(64) void fwd_rescue_tab_kernel(FwdArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int NK = 2 * (C_HI - C_LO + 1);
+    uint32_t cnt[NK], ni[NK];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int k = 0; k < NK; k++) {
+        const int c = C_HI - k / 2;
+        cnt[k] = __builtin_amdgcn_readfirstlane(a.bin_cnt[(k & 1) ? kRescueClasses + c : c]);
+        ni[k] = (cnt[k] + 1) / 2;
+        tot += ni[k];
+    }
+    for (uint32_t iter = 0; iter <= tot; iter++) {
+        uint32_t it = 0;
+        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= tot) break;
+        int k = 0;
+        uint32_t j = it;
+#pragma unroll
+        for (int q = 0; q + 1 < NK; q++)
+            if (k == q && j >= ni[q]) {
+                j -= ni[q];
+                k = q + 1;
+            }
+        uint32_t ck = 0;
+#pragma unroll
+        for (int q = 0; q < NK; q++)
+            if (k == q) ck = cnt[q];
+        const int c = C_HI - k / 2;
+        const bool inter = (k & 1) == 0;
+        rescue_tab_switch<C_LO, C_HI>(a, lds, c, inter, 2 * j + threadIdx.x / kRtL, ck);
+    }
+}
+
 // ------------------------------------------------- extended range
 // The FP64 forward of a pair whose FP64 sum underflowed to zero (the
 // reference's -inf), again with a finite result: the pair's strips of 63 *
@@ -1463,6 +1884,42 @@ __global__ __launch_bounds__(64) void fwd_ext_kernel(FwdArgs a) {
 // accepted window is a contiguous block the scan then skips), accepted when
 // idle = 1 - sum(R*H) / (wd * max R * max H) <= idle_d, in the reference's
 // integer and double arithmetic.
+// The window of W candidates starting at cand[s]: accepted when its idle
+// ratio, as pairhmm_schedule.cpp computes it, is at most idle_d.  KW > 0:
+// W <= KW, the loop unrolled so that the W candidates' loads are in flight
+// together.
+template <int KW>
+__device__ __forceinline__ bool window_ok(const RegroupArgs &a, const uint32_t *cand, uint32_t s,
+                                          uint32_t W) {
+    uint32_t mh = 0, mr = 0;
+    uint64_t sum = 0;
+    auto take = [&](uint32_t k) {
+        const uint32_t p = cand[s + k];
+        const uint32_t H = a.hap_len[a.pair_hap[p]], R = a.read_len[a.pair_read[p]];
+        mh = H > mh ? H : mh;
+        mr = R > mr ? R : mr;
+        sum += (uint64_t)H * R;
+    };
+    if constexpr (KW > 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < (uint32_t)KW; k++)
+            if (k < W) take(k);
+    } else {
+        for (uint32_t k = 0; k < W; k++) take(k);
+    }
+    const uint64_t cap = (uint64_t)mh * mr * W;
+    const uint64_t idle = cap > sum ? cap - sum : 0;
+    const double ratio = cap == 0 ? 0.0 : (double)idle / (double)cap;
+    return ratio <= a.idle_d;
+}
+
+// One wave per region.  The region's rejected pairs are compacted in order
+// (eight 64-entry chunks per pass, their loads in flight together); then the
+// acceptance of the window at every start is decided by the lanes in
+// parallel, 64 starts at a time, and the greedy walk over the starts --
+// accept at i and go to i + W, else go to i + 1 -- runs on the ballot mask
+// in scalar registers.  The walk visits the same starts as the sequential
+// loop, so the same windows are accepted.
 __global__ __launch_bounds__(64) void regroup_kernel(RegroupArgs a) {
     const uint32_t g = blockIdx.x;
     if (g >= a.n_regions || a.wd == 0) return;
@@ -1471,41 +1928,46 @@ __global__ __launch_bounds__(64) void regroup_kernel(RegroupArgs a) {
     uint32_t *cand = a.scratch + a.off[g];
     const uint32_t n = (uint32_t)(a.off[g + 1] - a.off[g]);
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    constexpr int kC = 8;
     uint32_t cnt = 0;
-    for (uint32_t base = 0; base < n; base += 64) {
-        const uint32_t i = base + lane;
-        uint32_t p = 0;
-        bool c = false;
-        if (i < n) {
-            p = list[i];
-            c = (a.flags[p] & 0x01) != 0;
+    for (uint32_t base = 0; base < n; base += 64 * kC) {
+        uint32_t p[kC];
+        bool c[kC];
+#pragma unroll
+        for (int m = 0; m < kC; m++) {
+            const uint32_t i = base + 64 * m + lane;
+            p[m] = i < n ? list[i] : 0u;
         }
-        const uint64_t m = __ballot(c);
-        if (c) cand[cnt + (uint32_t)__popcll(m & below)] = p;
-        cnt += (uint32_t)__popcll(m);
+#pragma unroll
+        for (int m = 0; m < kC; m++) {
+            const uint32_t i = base + 64 * m + lane;
+            c[m] = i < n && (a.flags[p[m]] & 0x01) != 0;
+        }
+#pragma unroll
+        for (int m = 0; m < kC; m++) {
+            const uint64_t mk = __ballot(c[m]);
+            if (c[m]) cand[cnt + (uint32_t)__popcll(mk & below)] = p[m];
+            cnt += (uint32_t)__popcll(mk);
+        }
     }
-    __syncthreads();  // the compacted list, written by all lanes, is read by lane 0
-    if (lane != 0) return;
+    cnt = __builtin_amdgcn_readfirstlane(cnt);
     const uint32_t W = a.wd;
-    uint32_t i = 0;
-    while (i < cnt && cnt - i >= W) {
-        uint32_t mh = 0, mr = 0;
-        uint64_t sum = 0;
-        for (uint32_t k = i; k < i + W; k++) {
-            const uint32_t p = cand[k];
-            const uint32_t H = a.hap_len[a.pair_hap[p]], R = a.read_len[a.pair_read[p]];
-            mh = H > mh ? H : mh;
-            mr = R > mr ? R : mr;
-            sum += (uint64_t)H * R;
-        }
-        const uint64_t cap = (uint64_t)mh * mr * W;
-        const uint64_t idle = cap > sum ? cap - sum : 0;
-        const double ratio = cap == 0 ? 0.0 : (double)idle / (double)cap;
-        if (ratio <= a.idle_d) {
-            for (uint32_t k = i; k < i + W; k++) a.flags[cand[k]] |= kFlagGrouped64;
-            i += W;
-        } else {
-            i += 1;
+    if (cnt < W) return;
+    __syncthreads();  // the compacted list, written by all lanes, is read by all
+    const uint32_t nstart = cnt - W + 1;  // window starts 0 .. cnt - W
+    uint32_t i = 0;                       // the walk's position (uniform)
+    for (uint32_t base = 0; base < nstart && i < nstart; base += 64) {
+        const uint32_t s = base + lane;
+        bool ok = false;
+        if (s < nstart && s >= i) ok = W <= 16 ? window_ok<16>(a, cand, s, W) : window_ok<0>(a, cand, s, W);
+        const uint64_t okm = __ballot(ok);
+        while (i < nstart && i < base + 64) {
+            if ((okm >> (i - base)) & 1ull) {
+                if (lane < W) a.flags[cand[i + lane]] |= kFlagGrouped64;
+                i += W;
+            } else {
+                i += 1;
+            }
         }
     }
 }
@@ -1638,9 +2100,21 @@ void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
 }
 
 // Chain kernels exist for every table-kernel shape.
-void *fwd_chain_kernel_ptr(int L, int MR, bool inter) {
+void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S) {
+#define TK2(m) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<16, m, true, 2>) : reinterpret_cast<void *>(&fwd_chain_kernel<16, m, false, 2>);
+    if (S == 2) {  // two reads per group: 16 lanes, 2..16 rows per lane
+        if (L != 16) return nullptr;
+        switch (MR) {
+            TK2(2) TK2(3) TK2(4) TK2(5) TK2(6) TK2(7) TK2(8) TK2(9) TK2(10) TK2(11) TK2(12) TK2(13) TK2(14)
+            TK2(15) TK2(16)
+        default: return nullptr;
+        }
+    }
+#undef TK2
+    if (S != 1) return nullptr;
 #define TK(l, m) \
-    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<l, m, true>) : reinterpret_cast<void *>(&fwd_chain_kernel<l, m, false>);
+    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<l, m, true, 1>) : reinterpret_cast<void *>(&fwd_chain_kernel<l, m, false, 1>);
     if (L == 8) {
         switch (MR) {
             TK(8, 6) TK(8, 7) TK(8, 8) TK(8, 9) TK(8, 10) TK(8, 11) TK(8, 12)
@@ -1700,6 +2174,22 @@ hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_
                        : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false>);
     void *args[] = {const_cast<FwdArgs *>(&a)};
     return hipLaunchKernel(kern, dim3(grid), dim3(64 * kLW), args, lds_bytes, st);
+}
+
+// FP64 rescue in classes: the binning pass, and the table pass over classes
+// 4..7 (part 0) or 0..3 (part 1), each a persistent launch of one-wave
+// workgroups; LDS per wave: two groups of lds_stride code bytes and a table
+// of rescue_tab_bytes(highest MR of the part).
+uint32_t rescue_tab_bytes(int MR) { return 4u * kRtL * (uint32_t)((MR + 1) / 2 * 2) * 8u; }
+hipError_t launch_rescue_bin(const FwdArgs &a, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(rescue_bin_kernel, dim3(grid), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t launch_rescue_tab(int part, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
+    void *args[] = {const_cast<FwdArgs *>(&a)};
+    void *k = part == 0 ? reinterpret_cast<void *>(&fwd_rescue_tab_kernel<4, 7>)
+                        : reinterpret_cast<void *>(&fwd_rescue_tab_kernel<0, 3>);
+    return hipLaunchKernel(k, dim3(grid), dim3(64), args, lds_bytes, st);
 }
 
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st) {

@@ -92,7 +92,7 @@ class Stats(C.Structure):
         ("num_pairs", C.c_uint64), ("cells", C.c_uint64), ("rescued_pairs", C.c_uint64),
         ("rescued_cells", C.c_uint64), ("launches", C.c_uint64), ("num_buckets", C.c_uint32),
         ("fp32_kernel_ms", C.c_double), ("fp64_kernel_ms", C.c_double),
-        ("total_ms", C.c_double)]
+        ("total_ms", C.c_double), ("stacked_pairs", C.c_uint64)]
 
 
 class Params(C.Structure):

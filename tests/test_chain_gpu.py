@@ -1,6 +1,7 @@
 """Chained items (reads of a region swept across up to 16 of their
-haplotypes in one item, fwd_chain_kernel): bit-identical to one pair per
-slot, and to the oracle, across the hand-over's edge cases.  The tests force
+haplotypes in one item, fwd_chain_kernel, short reads two per lane group):
+bit-identical to one pair per slot, and to the oracle, across the
+hand-over's and the two-read layout's edge cases.  The tests force
 chains (haplotype_chains=2); the default forms them only in buckets large
 enough to keep the device full, which the last test checks."""
 import ctypes
@@ -18,13 +19,13 @@ from test_schedule_gpu import check_schedule
 pytestmark = pytest.mark.gpu
 
 
-def _run(engine, batch, chains, **cfg):
+def _run(engine, batch, chains, stacked=False, **cfg):
     plan = engine.plan(batch, default_config(haplotype_chains=chains, **cfg))
     plan.execute()
     got = plan.read(raw=True)
-    nb = plan.stats()["num_buckets"]
+    st = plan.stats()
     plan.close()
-    return got, nb
+    return got, (st["stacked_pairs"] if stacked else st["num_buckets"])
 
 
 def _same(engine, batch, expect_chains=True, mode=2, **cfg):
@@ -69,6 +70,36 @@ def test_chains_against_oracle(engine, oracle_mod):
     check_schedule(engine, oracle_mod, _edge_batch(np.random.default_rng(43)), haplotype_chains=2)
 
 
+def _stack_batch(rng):
+    # short reads two per 16-lane group: equal and unequal lengths in one
+    # group, the tightest fits (two reads of 127 rows in 16 rows per lane, of
+    # 63 in 8), lengths across the one-read shapes' boundaries, unrelated
+    # reads (FP64 rescue) and an N in a stacked read, groups left over
+    regs = [
+        synth.region(rng, [300, 280, 260], [127] * 8),
+        synth.region(rng, [200, 190], [63] * 8 + [64] * 8),
+        synth.region(rng, [150, 140, 130, 120], list(rng.integers(20, 128, 16))),
+        synth.region(rng, [90, 80, 70], [50] * 12, unrelated=[1, 5]),
+        synth.region(rng, [400, 300], [100] * 9, n_read=[3]),
+        synth.region(rng, [260, 250], [96] * 8 + [31] * 8),
+    ]
+    return Batch.concat(regs)
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_stacked_reads(engine, oracle_mod, semantics):
+    b = _stack_batch(np.random.default_rng(61 + semantics))
+    _, stacked = _run(engine, b, 2, stacked=True, semantics=semantics)
+    _, none = _run(engine, b, 4, stacked=True, semantics=semantics)
+    assert stacked > 0 and none == 0
+    _same(engine, b, semantics=semantics)          # two reads per group where it pays
+    _same(engine, b, mode=4, semantics=semantics)  # one read per group
+    if semantics == 0:
+        check_against_oracle(oracle_mod, engine, b, haplotype_chains=2)
+    else:
+        check_schedule(engine, oracle_mod, b, haplotype_chains=2)
+
+
 @pytest.mark.parametrize("ppw", [8, 4, 2, 1])
 def test_chains_every_group_width(engine, oracle_mod, ppw):
     # 8, 4, 2 and 1 pairs per wave (8-, 16-, 32- and 64-lane chains)
@@ -97,6 +128,8 @@ def test_default_chains_large_batch(engine):
     # the default mode, which must then give the one-pair-per-slot bits
     b = synth.config(2, scale=0.3)
     _same(engine, b, mode=1, semantics=1)
+    _, stacked = _run(engine, b, 1, stacked=True, semantics=1)
+    assert stacked > 0
     # a small batch keeps one pair per slot by default
     _, nb1 = _run(engine, synth.config(1), 1, semantics=1)
     _, nb0 = _run(engine, synth.config(1), 0, semantics=1)
@@ -106,7 +139,7 @@ def test_default_chains_large_batch(engine):
 def test_chain_mode_out_of_range_is_rejected(engine):
     from rodv_amd.engine import ERR_INVALID_ARG, PhmmError
     b = synth.random_batch(np.random.default_rng(3), 2, 5)
-    for bad in (-1, 3):
+    for bad in (-1, 5):
         with pytest.raises(PhmmError) as e:
             engine.plan(b, default_config(haplotype_chains=bad))
         assert e.value.code == ERR_INVALID_ARG

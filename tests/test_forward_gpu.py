@@ -39,6 +39,12 @@ def check_against_oracle(oracle_mod, engine, batch, raw_exact=True, **cfg):
         assert len(bad) == 0, f"{len(bad)} raw FP32 mismatches, first {bad[:5]}: {r32[bad[:5]]} vs {o32[bad[:5]]}"
     # rescue set identical
     np.testing.assert_array_equal(fl & FLAG_RESCUED, ofl & FLAG_RESCUED)
+    # the FP64 sums of the rescued pairs bit-exact too (the extended-range
+    # pass rescales the ones it scores)
+    if raw_exact:
+        resc = ((ofl & FLAG_RESCUED) != 0) & ((fl & FLAG_EXTENDED) == 0)
+        bad = np.nonzero(resc & (r64.view(np.uint64) != o64.view(np.uint64)))[0]
+        assert len(bad) == 0, f"{len(bad)} raw FP64 mismatches, first {bad[:5]}: {r64[bad[:5]]} vs {o64[bad[:5]]}"
     np.testing.assert_array_equal(fl & FLAG_UNDERFLOW, ofl & FLAG_UNDERFLOW)
     fin = np.isfinite(o_out)
     np.testing.assert_array_equal(np.isfinite(out), fin)
@@ -110,6 +116,39 @@ def test_rescue_unrelated_reads(engine, oracle_mod):
     b = synth.random_batch(rng, 5, 40, hap_len=(150, 300), read_len=(40, 200), p_sub=0.75)
     out, fl, _ = check_against_oracle(oracle_mod, engine, b)
     assert (fl & FLAG_RESCUED).sum() > 10
+
+
+def rescue_class_batch(seed):
+    """Regions of reads at and around every boundary of the FP64 rescue's
+    classes (32 lanes, MR = ceil((R + 1) / 32) rows per lane up to 255 rows,
+    then the one-pair-per-wave pass): per region two haplotypes (one with an
+    N in every third region) and six reads, three of them unrelated so that
+    their FP32 sums underflow (from ~17 bases on)."""
+    rng = np.random.default_rng(seed)
+    lens = [1, 20, 24, 31, 32, 40, 63, 64, 95, 96, 127, 128, 150, 159, 160, 191, 192, 223, 224,
+            240, 254, 255, 256, 300, 378]
+    regs = []
+    for k, R in enumerate(lens):
+        hl = [R + int(rng.integers(1, 150)), R + int(rng.integers(1, 150))]
+        regs.append(synth.region(rng, hl, [R] * 6, unrelated=(0, 2, 4),
+                                 n_hap=(1,) if k % 3 == 0 else (), qual=40))
+    b = Batch.concat(regs)
+    # gap continuation Phred 40: an unrelated read costs ~4 decades per base
+    # even through the insertion state, so reads from ~17 bases underflow
+    b.gcp_q[:] = 40
+    return b
+
+
+def test_fp64_rescue_classes(engine, oracle_mod):
+    b = rescue_class_batch(21)
+    out, fl, _ = check_against_oracle(oracle_mod, engine, b)
+    R = np.asarray(b.read_len)[b.cross_pairs()[1]]
+    resc = (fl & FLAG_RESCUED) != 0
+    assert resc.sum() > 100
+    assert len(set((R[resc & (R <= 255)] // 32).tolist())) == 8  # every class has rescued pairs
+    assert (resc & (R > 255)).any()
+    # the same with haplotype chains and the plain launch order
+    check_against_oracle(oracle_mod, engine, b, haplotype_chains=0)
 
 
 def test_long_haplotypes_short_reads(engine, oracle_mod):

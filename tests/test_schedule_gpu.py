@@ -52,6 +52,16 @@ def check_schedule(engine, oracle_mod, batch, wf=16, wd=8, idle_f=0.1, idle_d=0.
     return fl, route
 
 
+def test_fp64_rescue_classes_under_routing(engine, oracle_mod):
+    # every FP64 rescue class in both arithmetics: 4-wide FP32 groups of
+    # equal-length reads, rejected pairs regrouped two at a time
+    from test_forward_gpu import rescue_class_batch
+    b = rescue_class_batch(22)
+    fl, route = check_schedule(engine, oracle_mod, b, wf=4, wd=2)
+    assert (route == ROUTE_INTER64).sum() > 20
+    assert ((fl & FLAG_RESCUED) != 0).sum() > 100
+
+
 def test_hand_cases_route_like_the_oracle(engine, oracle_mod):
     rng = np.random.default_rng(1)
     regions = [
