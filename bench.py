@@ -367,7 +367,8 @@ def main():
     rescued = int(st["rescued_pairs"])
     rescued_cells = int(st["rescued_cells"])
     log(f"fp32 kernels {fp32_ms:.2f} ms, fp64 rescue {fp64_ms:.2f} ms, rescued {rescued} "
-        f"({rescued_cells:.3e} cells)")
+        f"({rescued_cells:.3e} cells), {int(st['stacked_pairs'])} pairs in two-read chained items, "
+        f"{int(st['num_buckets'])} FP32 buckets")
     parity = None
     if args.oracle_regions > 0:
         t0 = time.time()
@@ -456,6 +457,7 @@ def main():
         "semantics": args.semantics,
         "autotune": tuning,
         "parity": parity,
+        "stacked_pairs": int(st["stacked_pairs"]),
         "rescued_pairs": rescued,
         "rescued_cells": rescued_cells,
         "rescued_fraction": round(rescued / max(1, batch.num_pairs()), 6),
