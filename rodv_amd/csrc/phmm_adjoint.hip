@@ -3,9 +3,10 @@
 // include/rodv_phmm.h (global gap-open delta and gap-continuation gamma,
 // per-quality-bin scales of the base error probability).
 //
-// One pair per wave (L = 64 lanes, MR read rows per lane), the same lane and
-// row layout as the forward (identity rows, row R in the last slot of its
-// lane).  Three passes per pair:
+// Reads of one strip: L = 16, 32 or 64 lanes per pair (64 / L pairs per
+// wave, adj2_kernel), MR read rows per lane, the select-prior forward's lane
+// and row layout (lane L-1 and `off` slots at the top of lane 0 hold
+// identity rows, row R in the last slot of its lane).  Three passes:
 //  1. forward over all anti-diagonal steps, saving the whole lane state every
 //     `ckpt` steps to HBM (level-1 checkpoints);
 //  2. for each checkpoint segment, last to first: recompute the segment from
@@ -15,7 +16,8 @@
 //     half from a copy advanced 4 steps) and run the adjoint of those steps in
 //     reverse, lanes reading the row below their last row from lane l+1.
 // Values cross lanes through ds_bpermute (LaneXfer), not DPP, so that two
-// waves of a SIMD can co-issue the arithmetic.
+// waves of a SIMD can co-issue the arithmetic.  Longer reads run strip by
+// strip in one wave in FP64 (adj_long_kernel, below).
 // Adjoint seeds are sigma = 2^-ilogb(S) so that adjoint x forward products
 // stay O(expected counts); the gradient of log10 L follows at the end.
 // Per real cell (i, j) with Tb = p * Mb (p the emission prior):
@@ -165,121 +167,6 @@ __device__ __forceinline__ void adj_step(const State<T, MR> &Sb, const State<T, 
     Tb0p = oldT0;
 }
 
-// Passes 2 and 3 and the reduction, for a pair whose forward sum is usable.
-template <typename T, int MR, bool RESCUE>
-__device__ __forceinline__ void adj_reverse(const AdjArgs &a, uint8_t *lds, int p, int H, int lR, int D4,
-                                         int nseg, int K1, T *ck, State<T, MR> S,
-                                         const Priors<T, MR> &P, const AdjRows<T, MR> &A, T Ssum,
-                                         const uint8_t *cp, int R, int off, uint64_t ro) {
-    const int lane = threadIdx.x;
-    const int l = lane;
-    constexpr int NV = 3 * MR + 3;
-    T *l2 = reinterpret_cast<T *>(lds + a.lds_stride);  // level-2 states
-    const LaneXfer<kL, 1> up;
-    const LaneXfer<kL, -1> down;
-    const int e = ilogb((double)Ssum);
-    const T sigma = (T)ldexp(1.0, -e);
-
-    // ---- passes 2 and 3: reverse sweep
-    T Tb[MR], Ib[MR], Db[MR], gd[MR], gdT[MR], gg[MR], ge[MR];
-#pragma unroll
-    for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gdT[s] = gg[s] = ge[s] = T(0);
-    T Tb0p = T(0);
-    // the 4 steps after state W0 (state after step dw): recompute them into
-    // registers, then their adjoints last to first
-    auto window4 = [&](const State<T, MR> &W0, int dw) {
-        State<T, MR> W1, W2, W3, W4;
-        W1 = W0;
-        step<T, kL, MR, false>(W1, P, A.ab_yy, A.ab_my, cp[dw + 1], up);
-        W2 = W1;
-        step<T, kL, MR, false>(W2, P, A.ab_yy, A.ab_my, cp[dw + 2], up);
-        W3 = W2;
-        step<T, kL, MR, false>(W3, P, A.ab_yy, A.ab_my, cp[dw + 3], up);
-        W4 = W3;
-        step<T, kL, MR, false>(W4, P, A.ab_yy, A.ab_my, cp[dw + 4], up);
-        auto seed_at = [&](int d) {
-            const int j = d - l;
-            return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
-        };
-        adj_step<T, MR>(W3, W4, P, A, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
-        adj_step<T, MR>(W2, W3, P, A, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
-        adj_step<T, MR>(W1, W2, P, A, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
-        adj_step<T, MR>(W0, W1, P, A, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
-    };
-    for (int k = nseg - 1; k >= 0; k--) {
-        const int d0 = k * K1;                       // state after step d0
-        const int nsteps = min(K1, D4 - d0);         // a multiple of 4
-        const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
-        // level 2: the state at the start of every 8 steps, in LDS
-        get_state(ck, NV, k, S);
-        __syncthreads();
-        for (int w = 0; w < nwin; w++) {
-            put_state(l2, NV, w, S);
-            const int n = min(2 * kW, nsteps - w * 2 * kW);
-            for (int u = 0; u < n; u++) step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d0 + w * 2 * kW + u + 1], up);
-        }
-        __syncthreads();
-        for (int w = nwin - 1; w >= 0; w--) {
-            const int dw = d0 + w * 2 * kW;          // window covers steps dw+1 .. dw+n
-            const int n = min(2 * kW, nsteps - w * 2 * kW);
-            State<T, MR> W0;
-            get_state(l2, NV, w, W0);
-            if (n == 2 * kW) {
-                // second half first: advance a copy 4 steps, then its window
-                State<T, MR> X = W0;
-#pragma unroll
-                for (int u = 1; u <= kW; u++) step<T, kL, MR, false>(X, P, A.ab_yy, A.ab_my, cp[dw + u], up);
-                window4(X, dw + kW);
-            }
-            window4(W0, dw);
-        }
-    }
-
-    // ---- reduce over read rows and lanes
-    T vd = T(0), vg = T(0), vb[8];
-#pragma unroll
-    for (int b = 0; b < 8; b++) vb[b] = T(0);
-#pragma unroll
-    for (int s = 0; s < MR; s++) {
-        // the row's realness, d e_i / d scale = 10^(-q_i/10) and quality bin,
-        // read again here rather than kept in registers through the sweeps
-        const int i = l * MR + s - off + 1;
-        const bool rl = l < kL - 1 && i >= 1 && i <= R;
-        const uint32_t qq = a.q[ro + (uint64_t)(rl ? i - 1 : 0)] & 127u;
-        const T real = rl ? T(1) : T(0);
-        const uint32_t bin = a.qbin[qq];
-        vd = fmaT(real, fmaT(T(-2), gdT[s], gd[s]), vd);
-        vg = fmaT(real, gg[s], vg);
-        const T c = real * ge[s] * (T)a.wq[qq];
-#pragma unroll
-        for (int b = 0; b < 8; b++) vb[b] += (bin == (uint32_t)b) ? c : T(0);
-    }
-    double acc[10];
-    acc[0] = (double)vd;
-    acc[1] = (double)vg;
-#pragma unroll
-    for (int b = 0; b < 8; b++) acc[2 + b] = (double)vb[b];
-#pragma unroll
-    for (int t = 0; t < 10; t++) {
-        double v = acc[t];
-        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-        acc[t] = v;
-    }
-    if (lane == 0) {
-        // scaled adjoints: acc = sigma * dS/dtheta;  d log10 L = dS / (S ln 10)
-        const double sS = (double)sigma * (double)Ssum;
-        const double ln10 = 2.302585092994045684;
-        const double K_log10 = RESCUE ? 1020.0 * 0.30102999566398119521 : 120.0 * 0.30102999566398119521;
-        a.out_log10[p] = log10((double)Ssum) - K_log10;
-        double *g = a.out_grad + 10 * (uint64_t)p;
-        g[0] = -a.delta * acc[0] / (10.0 * sS);
-        g[1] = -a.gamma * acc[1] / (10.0 * sS);
-        for (int b = 0; b < 8; b++) g[2 + b] = acc[2 + b] / (sS * ln10);
-        if (!RESCUE) a.flags[p] = 0;
-        else a.flags[p] = 0x01;
-    }
-}
-
 template <typename T>
 __device__ __forceinline__ T lane_value(T v, int src);  // value of lane `src` (uniform)
 template <>
@@ -294,60 +181,202 @@ __device__ __forceinline__ double lane_value(double v, int src) {
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// One pair per call.  Every per-pair scalar is made wave-uniform (SGPR) so
-// that all loops and branches below are scalar and the wave stays converged.
-template <typename T, int MR, bool RESCUE>
-__device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
-    const int lane = threadIdx.x;
-    const int l = lane;
-    constexpr int NV = 3 * MR + 3;
-    const int p = uni(p_in);
-    const uint32_t h = (uint32_t)uni((int)a.pair_hap[p]), r = (uint32_t)uni((int)a.pair_read[p]);
-    const int R = uni((int)a.read_len[r]), H = uni((int)a.hap_len[h]);
-    const uint64_t ro64 = a.read_off[r];
-    const uint64_t ro = ((uint64_t)(uint32_t)uni((int)(ro64 >> 32)) << 32) |
-                        (uint64_t)(uint32_t)uni((int)(uint32_t)ro64);
-    const int off = (MR - R % MR) % MR;
-    const int lR = (R - 1 + off) / MR;
-    const T initd = PTab<T>::initd(a, h);
+// ------------------------------------------------- short reads, G pairs per wave
+// The one-strip adjoint with L = 16, 32 or 64 lanes per pair (G = 64 / L
+// pairs per wave, every per-pair scalar per group of lanes) and the model's
+// transitions as wave-uniform scalars.  A read row has mm = 1 - 2 delta,
+// g = 1 - gamma, mx = my = delta, xx = yy = gamma; an identity row differs
+// in yy = 1 alone (its zero emission prior keeps its M at zero, and lane
+// 0's row above, lane L-1's last slot, is taken as (M, I) = (0, 0) instead
+// of rotated in, so the identity rows' I stay zero above the read).  Below
+// row R the identity rows may carry nonzero I, which reaches no read row
+// and, in the reverse sweep, gets no adjoint (their adjoints start at zero
+// and nothing seeds them).  The identity slots above the read do get
+// adjoints: their D = K/H and t enter the gamma and emission accumulators,
+// which stay per slot and are masked to the read rows at the end; the
+// delta accumulators see only zeros there (M = 0, Tb = 0) and are per lane.
+// max over the G groups of a per-group value (each group's lanes hold it)
+template <int G>
+__device__ __forceinline__ int wave_max_g(int v) {
+    int m = __builtin_amdgcn_readfirstlane(v);
+#pragma unroll
+    for (int k = 1; k < G; k++) m = max(m, __builtin_amdgcn_readlane(v, k * (64 / G)));
+    return m;
+}
+// the value of lane src (per lane)
+__device__ __forceinline__ float group_value(float v, int src) { return __shfl(v, src); }
+__device__ __forceinline__ double group_value(double v, int src) { return __shfl(v, src); }
 
-    // hap codes: column j at lds[L-1+j]
+template <typename T>
+struct UTrans {
+    T mm, g, dl, gm;  // 1 - 2 delta, 1 - gamma, delta, gamma
+};
+
+template <typename T, int MR>
+struct URows {
+    T yy[MR], X[MR], Y[MR];
+    uint32_t rpos[MR];
+};
+
+// forward step of the uniform-transition model (the intra FMA order)
+template <typename T, int L, int MR>
+__device__ __forceinline__ void ustep(State<T, MR> &S, const URows<T, MR> &P, const UTrans<T> &u, T yyA,
+                                      uint32_t hc, const LaneXfer<L, 1> &rot, bool lane0) {
+    T upM0 = rot(S.M[MR - 1]), upI0 = rot(S.I[MR - 1]);
+    upM0 = lane0 ? T(0) : upM0;
+    upI0 = lane0 ? T(0) : upI0;
+    T dgM = S.Mp, dgI = S.Ip, dgD = S.Dp;
+    T upM = upM0, upI = upI0;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
+        T t = dgM * u.mm;
+        t = fmaT(dgI, u.g, t);
+        t = fmaT(dgD, u.g, t);
+        const T nM = t * pr;
+        const T nI = fmaT(upI, u.gm, upM * u.dl);
+        const T nD = fmaT(S.D[s], P.yy[s], S.M[s] * u.dl);
+        dgM = S.M[s];
+        dgI = S.I[s];
+        dgD = S.D[s];
+        upM = nM;
+        upI = nI;
+        S.M[s] = nM;
+        S.I[s] = nI;
+        S.D[s] = nD;
+    }
+    S.Dp = fmaT(S.Dp, yyA, S.Mp * u.dl);
+    S.Mp = upM0;
+    S.Ip = upI0;
+}
+
+// Adjoint of one forward step (adj_step with uniform transitions); nb_* are
+// the transitions into the row below the lane's last slot (0 for lane L-1,
+// whose row below is the wrap).  gd, gdT per lane; gg, ge per slot.
+template <typename T, int L, int MR>
+__device__ __forceinline__ void adj_step_u(const State<T, MR> &Sb, const State<T, MR> &Sa,
+                                           const URows<T, MR> &P, const UTrans<T> &u, T nb_mm, T nb_g,
+                                           T nb_mx, T nb_xx, uint32_t hc, T seed, T (&Tb)[MR],
+                                           T (&Ib)[MR], T (&Db)[MR], T &Tb0p, T &gd, T &gdT,
+                                           T (&gg)[MR], T (&ge)[MR], const LaneXfer<L, -1> &from_below) {
+    const T dgM0 = Sb.Mp, dgI0 = Sb.Ip, dgD0 = Sb.Dp;
+    const T upM0 = Sa.Mp, upI0 = Sa.Ip;
+    T bT = from_below(Tb0p);   // Tb(i+1, j+1)
+    T bI = from_below(Ib[0]);  // Ib(i+1, j)
+    T b_mm = nb_mm, b_g = nb_g, b_mx = nb_mx, b_xx = nb_xx;
+    const T oldT0 = Tb[0];
+#pragma unroll
+    for (int s = MR - 1; s >= 0; s--) {
+        const T sd = (s == MR - 1) ? seed : T(0);
+        const T oldT = Tb[s];
+        const T Mb = fmaT(u.dl, Db[s], fmaT(b_mx, bI, fmaT(b_mm, bT, sd)));
+        const T Ibn = fmaT(b_xx, bI, fmaT(b_g, bT, sd));
+        const T Dbn = fmaT(P.yy[s], Db[s], b_g * bT);
+        const T dgM = s == 0 ? dgM0 : Sb.M[s - 1];
+        const T dgI = s == 0 ? dgI0 : Sb.I[s - 1];
+        const T dgD = s == 0 ? dgD0 : Sb.D[s - 1];
+        const T upM = s == 0 ? upM0 : Sa.M[s - 1];
+        const T upI = s == 0 ? upI0 : Sa.I[s - 1];
+        const T lfM = Sb.M[s], lfD = Sb.D[s];
+        const T sID = dgI + dgD;
+        const T t = fmaT(u.g, sID, dgM * u.mm);
+        T pr, de;
+        if constexpr (sizeof(T) == 4) {
+            const int m = __builtin_amdgcn_sbfe((int)hc, P.rpos[s], 1);
+            pr = __int_as_float((__float_as_int(P.X[s]) & m) | (__float_as_int(P.Y[s]) & ~m));
+            de = __int_as_float((__float_as_int(-1.0f) & m) | (__float_as_int(1.0f / 3.0f) & ~m));
+        } else {
+            const bool match = ((hc >> P.rpos[s]) & 1u) != 0u;
+            pr = match ? P.X[s] : P.Y[s];
+            de = match ? T(-1) : T(1) / T(3);
+        }
+        const T Tbn = pr * Mb;
+        gdT = fmaT(Tbn, dgM, gdT);
+        gd = fmaT(Dbn, lfM, fmaT(Ibn, upM, gd));
+        gg[s] = fmaT(Dbn, lfD, fmaT(Ibn, upI, fmaT(-Tbn, sID, gg[s])));
+        ge[s] = fmaT(Mb * t, de, ge[s]);
+        Tb[s] = Tbn;
+        Ib[s] = Ibn;
+        Db[s] = Dbn;
+        bT = oldT;
+        bI = Ibn;
+        b_mm = u.mm;
+        b_g = u.g;
+        b_mx = u.dl;
+        b_xx = u.gm;
+    }
+    Tb0p = oldT0;
+}
+
+template <typename T> struct UT;
+template <> struct UT<float> {
+    static __device__ __forceinline__ UTrans<float> get(const AdjArgs &a) {
+        return UTrans<float>{a.mm_f, a.g_f, a.dl_f, a.gm_f};
+    }
+};
+template <> struct UT<double> {
+    static __device__ __forceinline__ UTrans<double> get(const AdjArgs &a) {
+        return UTrans<double>{a.mm_d, a.g_d, a.delta, a.gamma};
+    }
+};
+
+// G pairs per call (p < 0: an empty group).  Wave-uniform loops run to the
+// largest group's bounds; a group past its own is masked (its sums) or
+// harmless (its adjoints are zero outside its columns).
+template <typename T, int L, int MR, bool RESCUE>
+__device__ void adj2_item(const AdjArgs &a, uint8_t *lds, int p) {
+    constexpr int G = 64 / L;
+    constexpr int NV = 3 * MR + 3;
+    constexpr int C = (L - 1) * MR;  // rows of the strip (lane L-1 holds identity rows)
+    const int lane = threadIdx.x;
+    const int g = lane / L;
+    const int l = lane % L;
+    const bool valid = p >= 0;
+    uint32_t h = 0, r = 0;
+    int R = 0, H = 0;
+    uint64_t ro = 0;
+    if (valid) {
+        h = a.pair_hap[p];
+        r = a.pair_read[p];
+        R = (int)a.read_len[r];
+        H = (int)a.hap_len[h];
+        ro = a.read_off[r];
+    }
+    const int off = valid ? (MR - R % MR) % MR : 0;
+    const int lR = valid ? (R - 1 + off) / MR : 0;
+    const T initd = valid ? PTab<T>::initd(a, h) : T(0);
+    const UTrans<T> u = UT<T>::get(a);
+
+    // hap codes of group g: column j at lds[g * lds_stride + L - 1 + j]
     __syncthreads();
-    for (uint32_t k = lane * 16; k < a.lds_stride; k += 64 * 16)
+    for (uint32_t k = lane * 16; k < (uint32_t)G * a.lds_stride; k += 64 * 16)
         *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
-    {
+    __syncthreads();
+    if (valid) {
         const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
         const int nch = (H + 15) >> 4;
-        // one-hot codes with bit 4 set: a read N (its row's bit is 4) matches
-        // every column, as in the reference
-        for (int c = l; c < nch; c += kL) {
+        // one-hot codes with bit 4 set: a read N (row bit 4) matches every column
+        for (int c = l; c < nch; c += L) {
             const uint4 v = src[c];
-            *reinterpret_cast<uint4 *>(lds + kL + 16 * c) =
+            *reinterpret_cast<uint4 *>(lds + g * a.lds_stride + L + 16 * c) =
                 make_uint4(v.x | 0x10101010u, v.y | 0x10101010u, v.z | 0x10101010u, v.w | 0x10101010u);
         }
     }
     __syncthreads();
-    const uint8_t *cp = lds + (kL - 1) - l;
+    const uint8_t *cp = lds + g * a.lds_stride + (L - 1) - l;
 
-    Priors<T, MR> P;
-    AdjRows<T, MR> A;
+    URows<T, MR> P;
     State<T, MR> S0;
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         const int i = l * MR + s - off + 1;
-        const bool real = l < kL - 1 && i >= 1 && i <= R;
+        const bool real = valid && l < L - 1 && i >= 1 && i <= R;
         const uint64_t idx = ro + (uint64_t)(real ? i - 1 : 0);
         const uint32_t qq = a.q[idx] & 127u;
-        uint32_t code = a.rcode[idx];
+        const uint32_t code = a.rcode[idx];
         T X = PTab<T>::X(a, qq), Y = PTab<T>::Y(a, qq);
-        const bool isN = code == kReadN;
-        Y = isN ? X : Y;  // N in the read: always the match prior (bit 4 of the codes)
-        P.mm[s] = real ? PTab<T>::mm(a) : T(0);
-        P.g[s] = real ? PTab<T>::g(a) : T(0);
-        P.mx[s] = real ? PTab<T>::dl(a) : T(0);
-        P.my[s] = real ? PTab<T>::dl(a) : T(0);
-        P.xx[s] = real ? PTab<T>::gm(a) : T(0);
-        P.yy[s] = real ? PTab<T>::gm(a) : T(1);
+        Y = code == kReadN ? X : Y;
+        P.yy[s] = real ? u.gm : T(1);
         P.X[s] = real ? X : T(0);
         P.Y[s] = real ? Y : T(0);
         P.rpos[s] = real ? code : 0u;
@@ -355,28 +384,20 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         S0.I[s] = T(0);
         S0.D[s] = real ? T(0) : initd;
     }
-    const LaneXfer<kL, 1> up;
-    // step's carry: the row above before the first column (M = I = 0, its
-    // initial D), and that row's yy and my
+    const LaneXfer<L, 1> up;
+    const LaneXfer<L, -1> down;
+    const bool lane0 = l == 0;
     S0.Mp = T(0);
     S0.Ip = T(0);
     S0.Dp = up(S0.D[MR - 1]);
-    A.ab_yy = up(P.yy[MR - 1]);
-    A.ab_my = up(P.my[MR - 1]);
-    {
-        // transitions of the row below this lane's last row: lane l+1's slot 0
-        const LaneXfer<kL, -1> down;
-        T m = down(P.mm[0]), g = down(P.g[0]), x = down(P.mx[0]), xx = down(P.xx[0]);
-        const bool wrap = l == kL - 1;
-        A.nb_mm = wrap ? T(0) : m;
-        A.nb_g = wrap ? T(0) : g;
-        A.nb_mx = wrap ? T(0) : x;
-        A.nb_xx = wrap ? T(0) : xx;
-    }
+    const T yyA = up(P.yy[MR - 1]);
+    const bool wrap = l == L - 1;
+    const T nb_mm = wrap ? T(0) : u.mm, nb_g = wrap ? T(0) : u.g;
+    const T nb_mx = wrap ? T(0) : u.dl, nb_xx = wrap ? T(0) : u.gm;
 
     // ---- pass 1: forward with level-1 checkpoints
-    const int Dend = H + lR;                          // lane lR reaches column H
-    const int D4 = (Dend + kW - 1) / kW * kW;         // whole 4-step windows
+    const int Dend = valid ? H + lR : 0;  // lane lR reaches column H
+    const int D4 = (wave_max_g<G>(Dend) + kW - 1) / kW * kW;
     const int K1 = a.ckpt;
     const int nseg = (D4 + K1 - 1) / K1;
     T *ck = reinterpret_cast<T *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
@@ -384,44 +405,149 @@ __device__ void adj_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     T sm = T(0), si = T(0);
     put_state(ck, NV, 0, S);
     for (int d = 1; d <= D4; d++) {
-        step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
+        ustep<T, L, MR>(S, P, u, yyA, cp[d], up, lane0);
         if (d <= Dend && d - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
         }
         if (d % K1 == 0 && d < D4) put_state(ck, NV, d / K1, S);
     }
-    const T Ssum = lane_value(sm + si, lR);
-    const bool underflow = RESCUE ? !(Ssum > T(0)) : !((float)Ssum >= 1e-28f);
-    if (underflow) {
-        if (lane == 0) {
-            if (!RESCUE) {  // FP32 underflow: the FP64 instance redoes this pair
-                a.flags[p] = 0x01;
-                const uint32_t k = atomicAdd(a.rescue_count, 1u);
-                a.rescue_list[k] = (uint32_t)p;
-            } else {        // FP64 underflow too: no gradient
-                a.out_log10[p] = -INFINITY;
-                for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)p + t] = 0.0;
-                a.flags[p] = 0x03;
-            }
+    const T Ssum = group_value(sm + si, g * L + lR);
+    const bool under = valid && (RESCUE ? !(Ssum > T(0)) : !((float)Ssum >= 1e-28f));
+    if (under && l == 0) {
+        if (!RESCUE) {  // FP32 underflow: the FP64 instance redoes this pair
+            a.flags[p] = 0x01;
+            const uint32_t k = atomicAdd(a.rescue_count, 1u);
+            a.rescue_list[k] = (uint32_t)p;
+        } else {        // FP64 underflow too: no gradient
+            a.out_log10[p] = -INFINITY;
+            for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)p + t] = 0.0;
+            a.flags[p] = 0x03;
         }
-    } else {
-        adj_reverse<T, MR, RESCUE>(a, lds, p, H, lR, D4, nseg, K1, ck, S, P, A, Ssum, cp, R, off, ro);
+    }
+    const bool live = valid && !under;
+    if (!__builtin_amdgcn_readfirstlane(__ballot(live) != 0ull ? 1 : 0)) return;
+
+    // ---- passes 2 and 3: the reverse sweep (seed 0 in the groups left out)
+    const int e = live ? ilogb((double)Ssum) : 0;
+    const T sigma = live ? (T)ldexp(1.0, -e) : T(0);
+    T *l2 = reinterpret_cast<T *>(lds + G * a.lds_stride);  // level-2 states
+    T Tb[MR], Ib[MR], Db[MR], gg[MR], ge[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gg[s] = ge[s] = T(0);
+    T Tb0p = T(0), gd = T(0), gdT = T(0);
+    auto window4 = [&](const State<T, MR> &W0, int dw) {
+        State<T, MR> W1, W2, W3, W4;
+        W1 = W0;
+        ustep<T, L, MR>(W1, P, u, yyA, cp[dw + 1], up, lane0);
+        W2 = W1;
+        ustep<T, L, MR>(W2, P, u, yyA, cp[dw + 2], up, lane0);
+        W3 = W2;
+        ustep<T, L, MR>(W3, P, u, yyA, cp[dw + 3], up, lane0);
+        W4 = W3;
+        ustep<T, L, MR>(W4, P, u, yyA, cp[dw + 4], up, lane0);
+        auto seed_at = [&](int d) {
+            const int j = d - l;
+            return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
+        };
+        adj_step_u<T, L, MR>(W3, W4, P, u, nb_mm, nb_g, nb_mx, nb_xx, cp[dw + 4], seed_at(dw + 4), Tb, Ib, Db,
+                             Tb0p, gd, gdT, gg, ge, down);
+        adj_step_u<T, L, MR>(W2, W3, P, u, nb_mm, nb_g, nb_mx, nb_xx, cp[dw + 3], seed_at(dw + 3), Tb, Ib, Db,
+                             Tb0p, gd, gdT, gg, ge, down);
+        adj_step_u<T, L, MR>(W1, W2, P, u, nb_mm, nb_g, nb_mx, nb_xx, cp[dw + 2], seed_at(dw + 2), Tb, Ib, Db,
+                             Tb0p, gd, gdT, gg, ge, down);
+        adj_step_u<T, L, MR>(W0, W1, P, u, nb_mm, nb_g, nb_mx, nb_xx, cp[dw + 1], seed_at(dw + 1), Tb, Ib, Db,
+                             Tb0p, gd, gdT, gg, ge, down);
+    };
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int d0 = k * K1;
+        const int nsteps = min(K1, D4 - d0);  // a multiple of 4
+        const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
+        get_state(ck, NV, k, S);
+        __syncthreads();
+        for (int w = 0; w < nwin; w++) {
+            put_state(l2, NV, w, S);
+            const int n = min(2 * kW, nsteps - w * 2 * kW);
+            for (int v = 0; v < n; v++) ustep<T, L, MR>(S, P, u, yyA, cp[d0 + w * 2 * kW + v + 1], up, lane0);
+        }
+        __syncthreads();
+        for (int w = nwin - 1; w >= 0; w--) {
+            const int dw = d0 + w * 2 * kW;
+            const int n = min(2 * kW, nsteps - w * 2 * kW);
+            // the window's start state is read from LDS for each half, so
+            // that it is not held through the second half's window
+            if (n == 2 * kW) {
+                State<T, MR> X;
+                get_state(l2, NV, w, X);
+#pragma unroll
+                for (int v = 1; v <= kW; v++) ustep<T, L, MR>(X, P, u, yyA, cp[dw + v], up, lane0);
+                window4(X, dw + kW);
+            }
+            State<T, MR> W0;
+            get_state(l2, NV, w, W0);
+            window4(W0, dw);
+        }
+    }
+
+    // ---- reduce over the group's read rows and lanes
+    T vg = T(0), vb[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) vb[b] = T(0);
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int i = l * MR + s - off + 1;
+        const bool rl = valid && l < L - 1 && i >= 1 && i <= R;
+        const uint32_t qq = a.q[ro + (uint64_t)(rl ? i - 1 : 0)] & 127u;
+        const T real = rl ? T(1) : T(0);
+        const uint32_t bin = a.qbin[qq];
+        vg = fmaT(real, gg[s], vg);
+        const T c = real * ge[s] * (T)a.wq[qq];
+#pragma unroll
+        for (int b = 0; b < 8; b++) vb[b] += (bin == (uint32_t)b) ? c : T(0);
+    }
+    double acc[10];
+    acc[0] = (double)fmaT(T(-2), gdT, gd);
+    acc[1] = (double)vg;
+#pragma unroll
+    for (int b = 0; b < 8; b++) acc[2 + b] = (double)vb[b];
+#pragma unroll
+    for (int t = 0; t < 10; t++) {
+        double v = acc[t];
+        for (int o = L / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+        acc[t] = v;
+    }
+    if (live && l == 0) {
+        // scaled adjoints: acc = sigma * dS/dtheta;  d log10 L = dS / (S ln 10)
+        const double sS = (double)sigma * (double)Ssum;
+        const double ln10 = 2.302585092994045684;
+        const double K_log10 = RESCUE ? 1020.0 * 0.30102999566398119521 : 120.0 * 0.30102999566398119521;
+        a.out_log10[p] = log10((double)Ssum) - K_log10;
+        double *gr = a.out_grad + 10 * (uint64_t)p;
+        gr[0] = -a.delta * acc[0] / (10.0 * sS);
+        gr[1] = -a.gamma * acc[1] / (10.0 * sS);
+        for (int b = 0; b < 8; b++) gr[2 + b] = acc[2 + b] / (sS * ln10);
+        a.flags[p] = RESCUE ? 0x01 : 0;
     }
 }
 
-template <typename T, int MR, bool RESCUE>
-__global__ __launch_bounds__(64, (MR <= 4 && !RESCUE ? 2 : 1)) void adj_kernel(AdjArgs a) {
+template <typename T, int L, int MR, bool RESCUE>
+__global__ __launch_bounds__(64, RESCUE ? 1 : 2) void adj2_kernel(AdjArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const uint32_t n = RESCUE ? *a.n_items_dev : a.n_items;
-    // a wave takes at most n items: the bound makes every wave exit
+    constexpr int G = 64 / L;
+    const uint32_t n = RESCUE ? (*a.n_items_dev + G - 1) / G : a.n_items;
+    const uint32_t nlist = RESCUE ? *a.n_items_dev : 0u;
     for (uint32_t iter = 0; iter <= n; iter++) {
         uint32_t it = 0;
         if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
         it = __builtin_amdgcn_readfirstlane(it);
         if (it >= n) break;
-        const int p = RESCUE ? (int)a.rescue_list[it] : a.items[it];
-        adj_item<T, MR, RESCUE>(a, lds, p);
+        const uint32_t k = it * G + threadIdx.x / L;
+        int p;
+        if constexpr (RESCUE)
+            p = k < nlist ? (int)a.rescue_list[k] : -1;
+        else
+            p = a.items[k];
+        adj2_item<T, L, MR, RESCUE>(a, lds, p);
     }
 }
 
@@ -828,18 +954,19 @@ __global__ __launch_bounds__(64, 1) void adj_long_kernel(AdjArgs a) {
 
 }  // namespace
 
-int adj_max_mr() { return 6; }
-
-void *adj_kernel_ptr(int MR, bool rescue) {
-#define KA(m)                                                                        \
-    case m:                                                                          \
-        return rescue ? reinterpret_cast<void *>(&adj_kernel<double, m, true>)       \
-                      : reinterpret_cast<void *>(&adj_kernel<float, m, false>);
-    switch (MR) {
-        KA(1) KA(2) KA(3) KA(4) KA(5) KA(6)
-    default: return nullptr;
-    }
-#undef KA
+// rows per lane of the G-pairs-per-wave kernel: 1 .. adj2_max_mr() at 16,
+// 32 and 64 lanes per pair
+int adj2_max_mr() { return 6; }
+void *adj2_kernel_ptr(int L, int MR, bool rescue) {
+#define KB(l, m)                                                                       \
+    case m:                                                                            \
+        return rescue ? reinterpret_cast<void *>(&adj2_kernel<double, l, m, true>)     \
+                      : reinterpret_cast<void *>(&adj2_kernel<float, l, m, false>);
+    if (L == 16) switch (MR) { KB(16, 1) KB(16, 2) KB(16, 3) KB(16, 4) KB(16, 5) KB(16, 6) default: return nullptr; }
+    if (L == 32) switch (MR) { KB(32, 1) KB(32, 2) KB(32, 3) KB(32, 4) KB(32, 5) KB(32, 6) default: return nullptr; }
+    if (L == 64) switch (MR) { KB(64, 1) KB(64, 2) KB(64, 3) KB(64, 4) KB(64, 5) KB(64, 6) default: return nullptr; }
+#undef KB
+    return nullptr;
 }
 
 int adj_long_mr() { return kLongMR; }

@@ -47,8 +47,8 @@ uint32_t rescue_tab_bytes(int MR);
 hipError_t launch_rescue_bin(const FwdArgs &a, int grid, hipStream_t st);
 hipError_t launch_rescue_tab(int part, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_ext(int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
-int adj_max_mr();
-void *adj_kernel_ptr(int MR, bool rescue);
+int adj2_max_mr();
+void *adj2_kernel_ptr(int L, int MR, bool rescue);
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
 int adj_long_mr();
 int adj_long_max_strips();
@@ -2258,6 +2258,20 @@ int rodv_phmm_forward(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     return guarded([&] { return forward_impl(ctx, batch, cfg, out_log10, out_flags); });
 }
 
+// Shape of a one-strip read in adj2_kernel: a strip holds (L - 1) * MR rows
+// (lane L-1 holds identity rows); the fewest lanes per pair that hold the
+// read with at most kAdjRows rows per lane (more pairs per wave, a shorter
+// fill and drain of the anti-diagonal pipeline), else 64 lanes with up to
+// adj2_max_mr() rows.
+constexpr int kAdjRows = 5;
+std::pair<int, int> adj_shape(uint32_t R) {
+    for (int L : {16, 32}) {
+        const int mr = (int)((R + (uint32_t)L - 2) / (uint32_t)(L - 1));
+        if (mr <= kAdjRows) return {L, std::max(1, mr)};
+    }
+    return {64, std::max(1, (int)((R + 62) / 63))};
+}
+
 static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                       const rodv_phmm_params *params, const double *weights, int ckpt_stride,
                       double *out_log10, double *grad, double *kernel_ms) {
@@ -2283,8 +2297,11 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         rodv_phmm_plan_destroy(p);
         return RODV_PHMM_ERR_DEVICE;
     }
-    const int maxmr = adj_max_mr();
-    std::vector<std::vector<int32_t>> bucket(maxmr + 1);
+    const int maxmr = adj2_max_mr();
+    // one-strip reads: buckets by shape (lanes per pair, rows per lane) of
+    // adj2_kernel, G = 64 / L pairs per item
+    std::vector<std::pair<int, int>> shapes;
+    std::vector<std::vector<int32_t>> bucket;
     uint32_t max_h = 0;
     // reads longer than one strip (63 * maxmr rows): the long-read kernel,
     // its strips of 63 * lmr rows in sequence, longest work first
@@ -2307,7 +2324,14 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
             max_strips = std::max(max_strips, nstr);
             continue;
         }
-        bucket[mr].push_back((int32_t)k);
+        const std::pair<int, int> sh = adj_shape(R);
+        size_t bi = 0;
+        while (bi < shapes.size() && shapes[bi] != sh) bi++;
+        if (bi == shapes.size()) {
+            shapes.push_back(sh);
+            bucket.emplace_back();
+        }
+        bucket[bi].push_back((int32_t)k);
         max_h = std::max(max_h, H);
     }
     std::stable_sort(longs.begin(), longs.end(), [](const std::pair<uint64_t, int32_t> &x,
@@ -2338,17 +2362,23 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     uint32_t *d_cnt = nullptr, *d_resc = nullptr;
     void *d_ck = nullptr;
     double *d_lck = nullptr, *d_lbnd = nullptr, *d_lab = nullptr;
+    // per bucket its items (G slots each, -1 padding the last) and a rescue
+    // list of the bucket's pair count
+    const size_t nbk = shapes.size();
     std::vector<int32_t> all_items;
-    std::vector<size_t> item_begin(maxmr + 2, 0);
-    for (int mr = 1; mr <= maxmr; mr++) {
-        item_begin[mr] = all_items.size();
-        all_items.insert(all_items.end(), bucket[mr].begin(), bucket[mr].end());
+    std::vector<size_t> item_begin(nbk + 1, 0), pair_begin(nbk + 1, 0);
+    for (size_t bi = 0; bi < nbk; bi++) {
+        const size_t G = (size_t)(64 / shapes[bi].first);
+        item_begin[bi] = all_items.size();
+        pair_begin[bi + 1] = pair_begin[bi] + bucket[bi].size();
+        all_items.insert(all_items.end(), bucket[bi].begin(), bucket[bi].end());
+        while ((all_items.size() - item_begin[bi]) % G) all_items.push_back(-1);
     }
-    item_begin[maxmr + 1] = all_items.size();
+    item_begin[nbk] = all_items.size();
     // two waves per SIMD: the adjoint kernels use up to 256 VGPRs, so more
     // would not be resident at once
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(n, (uint64_t)ctx->num_cus * 8));
-    const uint32_t stride = lds_stride_for(max_h, 64);
+    const uint32_t stride = lds_stride_for(max_h, 64);  // per group of any shape
     const uint64_t d4max = ((uint64_t)max_h + 63 + 3) / 4 * 4;
     const uint64_t nseg_max = d4max / 4 + 2;  // enough for any stride >= 4
     const uint64_t ck_elems = nseg_max * (3ull * maxmr + 3) * 64;
@@ -2365,7 +2395,7 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     if (!rc) rc = dev_alloc(&d_grad, 10 * n);
     if (!rc) rc = dev_alloc(&d_flags, n);
     if (!rc) rc = dev_alloc(&d_resc, n);
-    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)(maxmr + 1) + 1);  // + the long-read kernel's
+    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)nbk + 1);  // + the long-read kernel's
     if (!rc) rc = dev_alloc(reinterpret_cast<double **>(&d_ck), (size_t)grid * ck_elems);
     // long reads: per block, every strip's last row but the last strip's,
     // two rows of handed-up adjoints and the level-1 checkpoints of a strip
@@ -2403,10 +2433,11 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (!rc && (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess))
         rc = RODV_PHMM_ERR_DEVICE;
-    if (!rc && launch_reset(d_cnt, 3 * (maxmr + 1) + 1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && launch_reset(d_cnt, 3 * (uint32_t)nbk + 1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     if (!rc && hipEventRecord(ev0, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
-    for (int mr = 1; mr <= maxmr && !rc; mr++) {
-        const uint32_t cnt = (uint32_t)(item_begin[mr + 1] - item_begin[mr]);
+    for (size_t bi = 0; bi < nbk && !rc; bi++) {
+        const int L = shapes[bi].first, mr = shapes[bi].second, G = 64 / L;
+        const uint32_t cnt = (uint32_t)((item_begin[bi + 1] - item_begin[bi]) / (size_t)G);
         if (!cnt) continue;
         AdjArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -2421,7 +2452,7 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         a.read_len = p->d_read_len;
         a.pair_hap = p->d_pair_hap;
         a.pair_read = p->d_pair_read;
-        a.items = d_items + item_begin[mr];
+        a.items = d_items + item_begin[bi];
         a.n_items = cnt;
         a.xf = d_xf;
         a.yf = d_yf;
@@ -2440,8 +2471,8 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         a.out_log10 = d_out;
         a.out_grad = d_grad;
         a.flags = d_flags;
-        a.rescue_list = d_resc + item_begin[mr];
-        a.rescue_count = d_cnt + 3 * mr;
+        a.rescue_list = d_resc + pair_begin[bi];
+        a.rescue_count = d_cnt + 3 * bi;
         a.ck = d_ck;
         a.ck_elems = ck_elems;
         a.lds_stride = stride;
@@ -2451,13 +2482,14 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
             // level-2 windows live in LDS: shrink the checkpoint stride if needed
             int K1 = ckpt_stride;
             // (one state per 8 steps of a segment)
-            while (K1 > 4 && stride + (size_t)((K1 + 7) / 8) * nv * 64 * esz > (size_t)maxlds) K1 /= 2;
+            const size_t codes = (size_t)G * stride;
+            while (K1 > 4 && codes + (size_t)((K1 + 7) / 8) * nv * 64 * esz > (size_t)maxlds) K1 /= 2;
             K1 = K1 / 4 * 4;
             a.ckpt = K1;
-            a.counter = d_cnt + 3 * mr + 1 + pass;
+            a.counter = d_cnt + 3 * bi + 1 + pass;
             a.n_items_dev = pass == 0 ? nullptr : a.rescue_count;
-            const size_t lds = stride + (size_t)((K1 + 7) / 8) * nv * 64 * esz;
-            if (launch_adj(adj_kernel_ptr(mr, pass == 1), grid, lds, st, a) != hipSuccess)
+            const size_t lds = codes + (size_t)((K1 + 7) / 8) * nv * 64 * esz;
+            if (launch_adj(adj2_kernel_ptr(L, mr, pass == 1), grid, lds, st, a) != hipSuccess)
                 rc = RODV_PHMM_ERR_DEVICE;
         }
     }
@@ -2477,7 +2509,7 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         a.pair_read = p->d_pair_read;
         a.items = d_litems;
         a.n_items = (uint32_t)longs.size();
-        a.counter = d_cnt + 3 * (maxmr + 1);
+        a.counter = d_cnt + 3 * nbk;
         a.xf = d_xf;
         a.yf = d_yf;
         a.xd = d_xd;

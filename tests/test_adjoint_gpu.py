@@ -43,6 +43,45 @@ def test_adjoint_config3_sample(engine, oracle_mod):
     assert_grad_close(g, o_g)
 
 
+def test_adjoint_every_shape(engine, oracle_mod):
+    # reads at both ends of every (lanes per pair, rows per lane) shape of the
+    # one-strip kernel (16 lanes up to 75 rows, 32 up to 155, 64 up to 378),
+    # two and four pairs per wave next to single ones, empty slots
+    rng = np.random.default_rng(31)
+    lens = [1, 2, 15, 16, 30, 31, 45, 46, 60, 61, 75, 76, 93, 124, 125, 155, 156, 189, 190, 252, 315,
+            378]
+    regs = [synth.region(rng, [int(R) + int(rng.integers(1, 120)), int(R) + 40], [int(R)] * 3,
+                         unrelated=(1,) if R > 80 else (), qual=int(rng.integers(8, 41)))
+            for R in lens]
+    from rodv_amd.batch import Batch
+    b = Batch.concat(regs)
+    w = rng.normal(size=b.num_pairs())
+    out, g = engine.adjoint(b, THETA, weights=w)
+    o_out, o_g = oracle_mod.param_batch(b, THETA, weights=w, nthreads=8)
+    fin = np.isfinite(o_out)
+    np.testing.assert_array_equal(np.isfinite(out), fin)
+    assert np.max(np.abs(out[fin] - o_out[fin])) <= 1e-4
+    assert_grad_close(g, o_g)
+
+
+def test_adjoint_config3_sample_central_differences(engine, oracle_mod):
+    # a sample of the 4,096-pair workload (2 haplotypes x 128 reads, R = 150,
+    # H = 400), against central differences of the FP64 oracle's forward
+    b = synth.config(3, scale=2 / 32)
+    w = np.random.default_rng(5).normal(size=b.num_pairs())
+    _, g = engine.adjoint(b, THETA, weights=w)
+    h = 1e-3
+    fd = np.zeros(10)
+    for k in range(10):
+        tp, tm = THETA.copy(), THETA.copy()
+        tp[k] += h
+        tm[k] -= h
+        fp, _ = oracle_mod.param_batch(b, tp, nthreads=8)
+        fm, _ = oracle_mod.param_batch(b, tm, nthreads=8)
+        fd[k] = np.sum(w * (fp - fm)) / (2 * h)
+    assert_grad_close(g, fd)
+
+
 def test_adjoint_vs_central_differences(engine, oracle_mod):
     b = small_batch(7, hap_len=(30, 120), read_len=(20, 100), qual=(5, 45))
     _, g = engine.adjoint(b, THETA)
