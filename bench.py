@@ -34,7 +34,7 @@ ADJ_FLOP_PER_CELL = 46
 # WRITE_SIZE x 1.000, the factors a 1 GiB device copy in the same run gives
 # (--copy-calibration-mib 1024).  Keyed by (config, scale, semantics,
 # autotuned, haplotype chains).
-MEASURED_TRAFFIC = {(2, 1.0, "schedule", True, 1): 4.333e9, (2, 1.0, "schedule", True, 0): 2.239e10}
+MEASURED_TRAFFIC = {(2, 1.0, "schedule", True, 1): 4.214e9, (2, 1.0, "schedule", True, 0): 2.239e10}
 
 
 def log(msg):
