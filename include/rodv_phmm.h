@@ -263,7 +263,11 @@ int rodv_phmm_get_tuning(const rodv_phmm_ctx *ctx, rodv_phmm_tuning *out);
  * grad[10] = sum_k weight[k] * d log10 L_k / d(gap_open, gap_cont, scale[0..7]),
  * checkpointing every `ckpt_stride`-th anti-diagonal and recomputing the rest
  * (0 = 32).  kernel_ms (may be NULL) receives the summed device time of the
- * adjoint launches.  Reads longer than 378 bases return UNSUPPORTED.
+ * adjoint launches.  Pairs whose FP32 sum underflows run again in FP64 (2^1020
+ * scaling).  Reads longer than 378 bases run in FP64 strips of 252 rows, the
+ * row handed from strip to strip renormalised by a power of two, so their
+ * log10 L and gradient stay finite where a plain FP64 forward underflows; a
+ * read of more than 64 such strips (16,128 bases) returns UNSUPPORTED.
  */
 typedef struct rodv_phmm_params {
     double gap_open;
