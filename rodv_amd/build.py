@@ -1,6 +1,7 @@
 """Builds the engine's shared library (HIP for gfx950) in-tree.
 
     python -m rodv_amd.build        -> rodv_amd/librodv_phmm.so
+                                       rodv_amd/librodv_bam.so (BAM input, host code)
                                        rodv_amd/libvalu_ceiling.so (benchmark only)
 """
 from __future__ import annotations
@@ -14,6 +15,12 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "librodv_phmm.so")
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("phmm_kernels.hip", "phmm_adjoint.hip", "phmm_host.cpp",
                                                    "phmm_schedule.cpp", "phmm_noalloc.hip")]
+# BAM reader/writer over zlib (host code, built with the host compiler)
+BAM_LIB = os.path.join(HERE, "librodv_bam.so")
+BAM_SOURCES = [os.path.join(HERE, "csrc", "bam_io.cpp")]
+CXX = os.environ.get("CXX", "g++")
+BAM_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-fvisibility=hidden",
+             f"-I{os.path.join(ROOT, 'include')}"]
 # FP32 VALU issue-ceiling probe used by bench.py's roofline (not part of the engine)
 CEILING_LIB = os.path.join(HERE, "libvalu_ceiling.so")
 CEILING_SOURCES = [os.path.join(HERE, "csrc", "valu_ceiling.hip")]
@@ -40,10 +47,13 @@ ENGINE_FLAGS = ["-fvisibility=hidden", "-fvisibility-inlines-hidden"]
 
 def build(verbose: bool = False) -> str:
     jobs = []
-    for out, srcs, extra in ((LIB, SOURCES, ENGINE_FLAGS), (CEILING_LIB, CEILING_SOURCES, [])):
-        cmd = [HIPCC, *FLAGS, *extra, *srcs, "-o", out + ".tmp"]
-        jobs.append((out, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
-                                           text=True)))
+    cmds = [(LIB, [HIPCC, *FLAGS, *ENGINE_FLAGS, *SOURCES]),
+            (CEILING_LIB, [HIPCC, *FLAGS, *CEILING_SOURCES]),
+            (BAM_LIB, [CXX, *BAM_FLAGS, *BAM_SOURCES, "-lz",
+                       f"-Wl,--version-script={os.path.join(HERE, 'csrc', 'bam_io.map')}"])]
+    for out, cmd in cmds:
+        jobs.append((out, subprocess.Popen(cmd + ["-o", out + ".tmp"], stdout=subprocess.PIPE,
+                                           stderr=subprocess.PIPE, text=True)))
     failed = []
     for out, proc in jobs:
         _, stderr = proc.communicate()
