@@ -129,6 +129,14 @@ typedef struct rodv_phmm_config {
                                      the device with such items, 2 always, 3
                                      and 4 as 1 and 2 with one read per lane
                                      group, 0 never (one pair per slot)         */
+    int32_t shared_products;      /* 1 (default): reads whose deletion quality of
+                                     each row equals the insertion quality of
+                                     the row below (constant indel qualities)
+                                     take the 16-lane FP32 table kernels that
+                                     compute M(i,j) * mx_{i+1} once for both
+                                     I(i+1,j) and D(i,j+1): 7 FP32 instructions
+                                     per cell instead of 8, the same bits;
+                                     0: never                                  */
 } rodv_phmm_config;
 
 /* Counters of the last execution of a plan. */
@@ -144,6 +152,8 @@ typedef struct rodv_phmm_stats {
     double total_ms;         /* whole execution (profiled)                  */
     uint64_t stacked_pairs;  /* pairs in chained items with two reads per
                                 lane group                                  */
+    uint64_t shared_pairs;   /* pairs scored by the shared-product FP32
+                                kernels (config.shared_products)            */
 } rodv_phmm_stats;
 
 typedef struct rodv_phmm_ctx rodv_phmm_ctx;

@@ -26,8 +26,8 @@ namespace rodv {
 // defined in phmm_kernels.hip
 void *fwd_f32_kernel(int L, int MR, bool inter);
 void *fwd_f64_rescue_kernel(bool inter);
-void *fwd_tab_kernel_ptr(int L, int MR, bool inter);
-void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S);
+void *fwd_tab_kernel_ptr(int L, int MR, bool inter, bool share);
+void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S, bool share);
 uint32_t fwd_tab_bytes(int L, int MR);
 int fwd_f64_rescue_mr();
 int rescue_long_waves();
@@ -229,6 +229,8 @@ struct Bucket {
     bool inter = false;
     bool tab = false;        // table-prior kernel (N-free haplotypes, one strip)
     bool chain = false;      // chained items (fwd_chain_kernel); tab is set too
+    bool share = false;      // shared-product kernel variant: every read's
+                             // rows have my_i == mx_{i+1} (table kernels)
     int stack = 1;           // chained items: reads per group (1, or 2 stacked in its rows)
     uint32_t tab_bytes = 0;  // LDS prior table per group
     void *kern = nullptr;
@@ -757,13 +759,14 @@ void form_chains(std::vector<uint32_t> &mem, Bucket &bk, const uint32_t *ph, con
         if (S == 2) {
             tb = nullptr;
             for (Bucket &c : *stacked)
-                if (c.MR == sg.mr && c.inter == bk.inter) tb = &c;
+                if (c.MR == sg.mr && c.inter == bk.inter && c.share == bk.share) tb = &c;
             if (!tb) {
                 Bucket c;
                 c.L = bk.L;
                 c.MR = sg.mr;
                 c.wpc = bk.wpc;
                 c.inter = bk.inter;
+                c.share = bk.share;
                 c.tab = true;
                 c.chain = true;
                 c.stack = 2;
@@ -985,6 +988,7 @@ void rodv_phmm_config_default(rodv_phmm_config *cfg) {
     cfg->prior_table = 1;
     cfg->use_tuning = 1;
     cfg->haplotype_chains = 1;
+    cfg->shared_products = 1;
 }
 
 static int ctx_create_impl(int device, rodv_phmm_ctx **out) {
@@ -1080,6 +1084,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     if (cfg.simd_width_float > 64 || cfg.simd_width_double > 64) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.haplotype_chains < 0 || cfg.haplotype_chains > 4) return RODV_PHMM_ERR_INVALID_ARG;
     if (cfg.extended_range != 0 && cfg.extended_range != 1) return RODV_PHMM_ERR_INVALID_ARG;
+    if (cfg.shared_products != 0 && cfg.shared_products != 1) return RODV_PHMM_ERR_INVALID_ARG;
     if (!(cfg.max_idle_ratio_float == cfg.max_idle_ratio_float) ||
         !(cfg.max_idle_ratio_double == cfg.max_idle_ratio_double))
         return RODV_PHMM_ERR_INVALID_ARG;
@@ -1148,6 +1153,26 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     for (uint32_t r = 0; r < N; r++)
         for (uint32_t k = 0; k < b->read_lengths[r]; k++)
             read_has_n[r] |= rcode[read_off[r] + k] == kReadN;
+    // reads whose deletion prior of each row is the insertion prior of the
+    // row below (ph2pr[del_q[k]] == ph2pr[ins_q[k + 1]], the masked
+    // qualities of pairhmm_impl.cpp:89-92): the FP32 table kernels then
+    // share one product between the D and I recurrences (same bits)
+    std::vector<uint8_t> read_share(N, 0);
+    if (cfg.shared_products) {
+        const float *ph2pr = tables().ph2pr_f;
+        for (uint32_t r = 0; r < N; r++) {
+            bool ok_r = true;
+            const uint64_t o = read_off[r];
+            for (uint32_t k = 0; k + 1 < b->read_lengths[r] && ok_r; k++) {
+                const float my = ph2pr[qd[o + k] & 127u], mx = ph2pr[qi[o + k + 1] & 127u];
+                uint32_t a32, b32;
+                std::memcpy(&a32, &my, 4);
+                std::memcpy(&b32, &mx, 4);
+                ok_r = a32 == b32;
+            }
+            read_share[r] = ok_r ? 1 : 0;
+        }
+    }
 
     rodv_phmm_plan *p = new (std::nothrow) rodv_phmm_plan();
     if (!p) return RODV_PHMM_ERR_OOM;
@@ -1213,8 +1238,9 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         p->any_inter = !sched_list.empty();
         p->n_regions = (uint32_t)nr;
     }
-    std::vector<int> bucket_of_key(4 * 4 * 17, -1);
+    std::vector<int> bucket_of_key(2 * 4 * 4 * 17, -1);
     std::vector<std::vector<uint32_t>> members;
+    uint64_t n_shared = 0;    // pairs in shared-product buckets
     uint32_t max_long_h = 0;  // hap length of pairs whose FP64 rescue may need strips
     uint64_t n_long = 0;      // such pairs
     for (uint64_t k = 0; k < n_pairs; k++) {
@@ -1236,13 +1262,17 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         shape_for(R, H, cfg, te, tab_ok, &L, &MR, &tab);
         const bool inter = !inter32.empty() && inter32[k];
         const int lidx = L == 16 ? 0 : (L == 32 ? 1 : (L == 64 ? 2 : 3));
-        const int key = (((inter ? 1 : 0) * 2 + (tab ? 1 : 0)) * 4 + lidx) * 17 + MR;
+        // the shared-product variants exist for the 16-lane table kernels
+        const bool share = tab && L == 16 && read_share[pr[k]];
+        n_shared += share ? 1 : 0;
+        const int key = ((((share ? 1 : 0) * 2 + (inter ? 1 : 0)) * 2 + (tab ? 1 : 0)) * 4 + lidx) * 17 + MR;
         if (bucket_of_key[key] < 0) {
             bucket_of_key[key] = (int)p->buckets.size();
             Bucket bk;
             bk.L = L;
             bk.MR = MR;
             bk.tab = tab;
+            bk.share = share;
             bk.inter = inter;
             bk.wpc = te ? te->wpc : 0;
             p->buckets.push_back(bk);
@@ -1270,6 +1300,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             tmpl.MR = b0.MR;
             tmpl.wpc = b0.wpc;
             tmpl.inter = b0.inter;
+            tmpl.share = b0.share;
             const int wpc = cfg.waves_per_cu > 0 ? cfg.waves_per_cu
                                                    : (tmpl.wpc > 0 ? tmpl.wpc : kDefaultWavesPerCU);
             form_chains(members[bi], tmpl, ph.data(), pr.data(), pair_region, b->hap_lengths,
@@ -1279,7 +1310,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         }
         for (Bucket &c : stacked) {
             c.chain_off.push_back((uint32_t)c.chain_hap.size());
-            c.kern = fwd_chain_kernel_ptr(c.L, c.MR, c.inter, 2);
+            c.kern = fwd_chain_kernel_ptr(c.L, c.MR, c.inter, 2, c.share);
             if (!c.kern) return RODV_PHMM_ERR_UNSUPPORTED;
             p->buckets.push_back(std::move(c));
             members.emplace_back();
@@ -1287,13 +1318,15 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         }
         for (size_t bi = 0; bi < nb0; bi++) {
             if (!p->buckets[bi].tab) continue;
-            void *ck = fwd_chain_kernel_ptr(p->buckets[bi].L, p->buckets[bi].MR, p->buckets[bi].inter, 1);
+            void *ck = fwd_chain_kernel_ptr(p->buckets[bi].L, p->buckets[bi].MR, p->buckets[bi].inter, 1,
+                                            p->buckets[bi].share);
             if (!ck) continue;
             Bucket cb;
             cb.L = p->buckets[bi].L;
             cb.MR = p->buckets[bi].MR;
             cb.wpc = p->buckets[bi].wpc;
             cb.inter = p->buckets[bi].inter;
+            cb.share = p->buckets[bi].share;
             cb.tab = true;
             cb.chain = true;
             cb.kern = ck;
@@ -1323,7 +1356,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus * wpc));
             continue;
         }
-        bk.kern = bk.tab ? fwd_tab_kernel_ptr(bk.L, bk.MR, bk.inter)
+        bk.kern = bk.tab ? fwd_tab_kernel_ptr(bk.L, bk.MR, bk.inter, bk.share)
                          : fwd_f32_kernel(bk.L, bk.MR, bk.inter);
         if (!bk.kern) {
             return RODV_PHMM_ERR_UNSUPPORTED;
@@ -1618,6 +1651,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     p->stats.num_pairs = n_pairs;
     p->stats.cells = p->cells;
     p->stats.num_buckets = (uint32_t)p->buckets.size();
+    p->stats.shared_pairs = n_shared;
     for (const Bucket &bk : p->buckets)
         if (bk.stack == 2)
             for (size_t k = 0; k + 1 < bk.chain_off.size(); k++)

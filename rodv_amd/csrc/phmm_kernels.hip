@@ -386,10 +386,19 @@ struct Carry {
 // (which needs `up`, rotated at the end of the previous step), then the
 // rotations for the next step.  LDS results return in issue order, so each
 // wait covers only what the next phase needs.
-template <int L, int MR, bool INTER>
-__device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Priors<float, MR> &P,
-                                         const float (&pr)[MR], const LaneRot<L> &rot,
-                                         float (&pb)[MR], const uint8_t *pnext) {
+// SH (shared products): the read's deletion prior of each row equals the
+// insertion prior of the row below (my_i == mx_{i+1}, as when a read's
+// insertion and deletion qualities are the same constant).  Then the
+// product M(i,j) * mx_{i+1} that I(i+1,j) takes in this step is also the
+// M(i,j) * my_i that D(i,j+1) takes in the next one, the same bits, so it is
+// computed once and kept in Pm: 7 FP32 instructions per cell instead of 8.
+// Pm[s] holds S.M[s] * my[s] (zero where M is zero: identity slots, and
+// every slot after a reset); the lane's last slot computes its own, its row
+// below being the next lane's first.
+template <int L, int MR, bool INTER, bool SH>
+__device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, float (&Pm)[MR],
+                                         const Priors<float, MR> &P, const float (&pr)[MR],
+                                         const LaneRot<L> &rot, float (&pb)[MR], const uint8_t *pnext) {
     // keep the previous step's M and I rotations ahead of this step's LDS
     // reads: the I chain below then waits for them alone
     __builtin_amdgcn_sched_barrier(0);
@@ -410,7 +419,7 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
             t = fmaT(dgD, P.g[s], t);
         }
         nM[s] = t * pr[s];
-        const float v = S.M[s] * P.my[s];
+        const float v = SH ? Pm[s] : S.M[s] * P.my[s];
         nD[s] = fmaT(S.D[s], P.yy[s], v);
         dgM = S.M[s];
         dgI = S.I[s];
@@ -420,6 +429,7 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         const float u = upM * P.mx[s];
+        if (SH && s > 0) Pm[s - 1] = u;  // nM[s-1] * mx[s] == nM[s-1] * my[s-1]
         const float nI = fmaT(upI, P.xx[s], u);
         upM = nM[s];
         upI = nI;
@@ -427,6 +437,7 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
         S.M[s] = nM[s];
         S.D[s] = nD[s];
     }
+    if (SH) Pm[MR - 1] = nM[MR - 1] * P.my[MR - 1];
     C.dD = fmaT(C.dD, C.yyA, C.dM * C.myA);  // the slot above, one column on
     C.dM = C.uM;
     C.dI = C.uI;
@@ -439,7 +450,7 @@ __device__ __forceinline__ void step_tab(State<float, MR> &S, Carry &C, const Pr
 // the group's descriptor; the return value is the index of the wave's next
 // item, fetched from the work counter once the set-up is done so that the
 // atomic's round trip hides behind the sweep.
-template <int L, int MR, bool INTER>
+template <int L, int MR, bool INTER, bool SH>
 __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemDesc &d) {
     constexpr int G = 64 / L;
     constexpr int MRT = TabLayout<MR>::MRT;
@@ -528,11 +539,13 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     if (lane == 0) next = atomicAdd(a.counter, 1u);
 
     State<float, MR> S;
+    float Pm[MR];
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         S.M[s] = 0.0f;
         S.I[s] = 0.0f;
         S.D[s] = realv[s] ? 0.0f : initd;  // identity rows carry K/H
+        Pm[s] = 0.0f;
     }
     const LaneRot<L> rot;
     Carry C;
@@ -566,7 +579,7 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
         for (int u = 0; u < kU; u++) {
             const uint32_t c2 = ld_code(cl + dd + u + 2);
             float pb[MR];
-            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            step_tab<L, MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
 #pragma unroll
@@ -577,7 +590,7 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
     for (; dd <= Dend; dd++) {
         const uint32_t c2 = ld_code(cl + dd + 2);
         float pb[MR];
-        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+        step_tab<L, MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
         if (dd - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
@@ -607,7 +620,7 @@ __device__ uint32_t process_item_tab(const FwdArgs &a, uint8_t *lds, const ItemD
 // item ahead (process_item_tab fetches the next index), and reads each
 // item's descriptors with one load.  One wave per workgroup (the LDS
 // ordering in process_item_tab relies on it).
-template <int L, int MR, bool INTER>
+template <int L, int MR, bool INTER, bool SH>
 __global__ __launch_bounds__(64) void fwd_tab_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int G = 64 / L;
@@ -617,7 +630,7 @@ __global__ __launch_bounds__(64) void fwd_tab_kernel(FwdArgs a) {
     it = __builtin_amdgcn_readfirstlane(it);
     for (uint32_t iter = 0; iter < n && it < n; iter++) {
         const ItemDesc d = a.desc[(uint64_t)it * G + threadIdx.x / L];
-        it = process_item_tab<L, MR, INTER>(a, lds, d);
+        it = process_item_tab<L, MR, INTER, SH>(a, lds, d);
     }
 }
 
@@ -716,7 +729,7 @@ struct GroupRows {
     }
 };
 
-template <int L, int MR, bool INTER, int NS>
+template <int L, int MR, bool INTER, int NS, bool SH>
 __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     constexpr int G = 64 / L;
     constexpr int MRT = TabLayout<MR>::MRT;
@@ -790,11 +803,13 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     if (lane == 0) next = atomicAdd(a.counter, 1u);
 
     State<float, MR> S;
+    float Pm[MR];
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         S.M[s] = 0.0f;
         S.I[s] = 0.0f;
         S.D[s] = realv[s] ? 0.0f : initd0;
+        Pm[s] = 0.0f;
     }
     const LaneRot<L> rot;
     Carry C;
@@ -837,13 +852,14 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
                     S.M[s] = 0.0f;
                     S.I[s] = 0.0f;
                     S.D[s] = d0[s];
+                    Pm[s] = 0.0f;
                 }
                 C.dM = 0.0f;
                 C.dI = 0.0f;
                 C.dD = dD0;
             }
             float pb[MR];
-            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            step_tab<L, MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
 #pragma unroll
@@ -870,7 +886,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
             for (int u = 0; u < kU; u++) {
                 const uint32_t c2 = ld_code(cl + dd + u + 2);
                 float pb[MR];
-                step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+                step_tab<L, MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
                 sm += S.M[MR - 1];
                 si += S.I[MR - 1];
 #pragma unroll
@@ -881,7 +897,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
         for (; dd <= Sn - 2; dd++) {
             const uint32_t c2 = ld_code(cl + dd + 2);
             float pb[MR];
-            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            step_tab<L, MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
 #pragma unroll
@@ -909,12 +925,13 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
                     S.M[s] = 0.0f;
                     S.I[s] = 0.0f;
                     S.D[s] = dres[s];
+                    Pm[s] = 0.0f;
                 }
                 C.dM = 0.0f;
                 C.dI = 0.0f;
                 C.dD = dDres;
             }
-            step_tab<L, MR, INTER>(S, C, P, pin, rot, pout, tlane + c1 * kBlk);
+            step_tab<L, MR, INTER, SH>(S, C, Pm, P, pin, rot, pout, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
             c1 = c2;
@@ -949,7 +966,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
         for (int u = 0; u < kU; u++) {
             const uint32_t c2 = ld_code(cl + dd + u + 2);
             float pb[MR];
-            step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            step_tab<L, MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
 #pragma unroll
@@ -960,7 +977,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
     for (; dd <= Dend; dd++) {
         const uint32_t c2 = ld_code(cl + dd + 2);
         float pb[MR];
-        step_tab<L, MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+        step_tab<L, MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
         if (dd - l - Sk <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
@@ -979,7 +996,7 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
 // Persistent, like fwd_tab_kernel, over chained items.  LDS: the G prior
 // tables, then the two code buffers, which start all zero (prefetches past
 // a haplotype read codes left there, always a valid block 0..3).
-template <int L, int MR, bool INTER, int NS>
+template <int L, int MR, bool INTER, int NS, bool SH>
 __global__ __launch_bounds__(64) void fwd_chain_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int G = 64 / L;
@@ -991,7 +1008,7 @@ __global__ __launch_bounds__(64) void fwd_chain_kernel(FwdArgs a) {
     uint32_t it = 0;
     if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
     it = __builtin_amdgcn_readfirstlane(it);
-    for (uint32_t iter = 0; iter < n && it < n; iter++) it = process_chain<L, MR, INTER, NS>(a, lds, it);
+    for (uint32_t iter = 0; iter < n && it < n; iter++) it = process_chain<L, MR, INTER, NS, SH>(a, lds, it);
 }
 
 template <typename T, int L, int MR, bool INTER, bool RESCUE>
@@ -2066,10 +2083,21 @@ void *fwd_f32_kernel(int L, int MR, bool inter) {
 
 // Table-prior FP32 kernels (haplotypes without N, one strip): 8 lanes per
 // pair with 6..12 rows per lane, 16 lanes with 1..16, 32 lanes with 2..10,
-// 64 lanes with 4, 8, 12, 16.
-void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
+// 64 lanes with 4, 8, 12, 16.  share: the shared-product variant (16 lanes
+// only; nullptr elsewhere).
+void *fwd_tab_kernel_ptr(int L, int MR, bool inter, bool share) {
 #define TK(l, m) \
-    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<l, m, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<l, m, false>);
+    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<l, m, true, false>) : reinterpret_cast<void *>(&fwd_tab_kernel<l, m, false, false>);
+#define TKS(m) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_tab_kernel<16, m, true, true>) : reinterpret_cast<void *>(&fwd_tab_kernel<16, m, false, true>);
+    if (share) {
+        if (L != 16) return nullptr;
+        switch (MR) {
+            TKS(1) TKS(2) TKS(3) TKS(4) TKS(5) TKS(6) TKS(7) TKS(8)
+            TKS(9) TKS(10) TKS(11) TKS(12) TKS(13) TKS(14) TKS(15) TKS(16)
+        default: return nullptr;
+        }
+    }
     if (L == 8) {
         switch (MR) {
             TK(8, 6) TK(8, 7) TK(8, 8) TK(8, 9) TK(8, 10) TK(8, 11) TK(8, 12)
@@ -2096,25 +2124,46 @@ void *fwd_tab_kernel_ptr(int L, int MR, bool inter) {
         }
     }
 #undef TK
+#undef TKS
     return nullptr;
 }
 
-// Chain kernels exist for every table-kernel shape.
-void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S) {
-#define TK2(m) \
-    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<16, m, true, 2>) : reinterpret_cast<void *>(&fwd_chain_kernel<16, m, false, 2>);
+// Chain kernels exist for every table-kernel shape (the shared-product
+// variants for 16 lanes).
+void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S, bool share) {
+#define TK2(m, sh) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<16, m, true, 2, sh>) : reinterpret_cast<void *>(&fwd_chain_kernel<16, m, false, 2, sh>);
     if (S == 2) {  // two reads per group: 16 lanes, 2..16 rows per lane
         if (L != 16) return nullptr;
+        if (share) {
+            switch (MR) {
+                TK2(2, true) TK2(3, true) TK2(4, true) TK2(5, true) TK2(6, true) TK2(7, true) TK2(8, true)
+                TK2(9, true) TK2(10, true) TK2(11, true) TK2(12, true) TK2(13, true) TK2(14, true)
+                TK2(15, true) TK2(16, true)
+            default: return nullptr;
+            }
+        }
         switch (MR) {
-            TK2(2) TK2(3) TK2(4) TK2(5) TK2(6) TK2(7) TK2(8) TK2(9) TK2(10) TK2(11) TK2(12) TK2(13) TK2(14)
-            TK2(15) TK2(16)
+            TK2(2, false) TK2(3, false) TK2(4, false) TK2(5, false) TK2(6, false) TK2(7, false) TK2(8, false)
+            TK2(9, false) TK2(10, false) TK2(11, false) TK2(12, false) TK2(13, false) TK2(14, false)
+            TK2(15, false) TK2(16, false)
         default: return nullptr;
         }
     }
 #undef TK2
     if (S != 1) return nullptr;
 #define TK(l, m) \
-    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<l, m, true, 1>) : reinterpret_cast<void *>(&fwd_chain_kernel<l, m, false, 1>);
+    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<l, m, true, 1, false>) : reinterpret_cast<void *>(&fwd_chain_kernel<l, m, false, 1, false>);
+#define TKS(m) \
+    case m: return inter ? reinterpret_cast<void *>(&fwd_chain_kernel<16, m, true, 1, true>) : reinterpret_cast<void *>(&fwd_chain_kernel<16, m, false, 1, true>);
+    if (share) {
+        if (L != 16) return nullptr;
+        switch (MR) {
+            TKS(1) TKS(2) TKS(3) TKS(4) TKS(5) TKS(6) TKS(7) TKS(8)
+            TKS(9) TKS(10) TKS(11) TKS(12) TKS(13) TKS(14) TKS(15) TKS(16)
+        default: return nullptr;
+        }
+    }
     if (L == 8) {
         switch (MR) {
             TK(8, 6) TK(8, 7) TK(8, 8) TK(8, 9) TK(8, 10) TK(8, 11) TK(8, 12)
@@ -2141,6 +2190,7 @@ void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S) {
         }
     }
 #undef TK
+#undef TKS
     return nullptr;
 }
 uint32_t fwd_tab_bytes(int L, int MR) { return 4u * (uint32_t)L * (uint32_t)((MR + 3) / 4 * 4) * 4u; }

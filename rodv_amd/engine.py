@@ -65,7 +65,7 @@ class Config(C.Structure):
         ("simd_width_float", C.c_uint32), ("simd_width_double", C.c_uint32),
         ("use_graph", C.c_int32), ("waves_per_cu", C.c_int32), ("short_lanes", C.c_int32),
         ("extended_range", C.c_int32), ("prior_table", C.c_int32), ("use_tuning", C.c_int32),
-        ("haplotype_chains", C.c_int32)]
+        ("haplotype_chains", C.c_int32), ("shared_products", C.c_int32)]
 
 
 TUNE_CLASSES = 18
@@ -92,7 +92,7 @@ class Stats(C.Structure):
         ("num_pairs", C.c_uint64), ("cells", C.c_uint64), ("rescued_pairs", C.c_uint64),
         ("rescued_cells", C.c_uint64), ("launches", C.c_uint64), ("num_buckets", C.c_uint32),
         ("fp32_kernel_ms", C.c_double), ("fp64_kernel_ms", C.c_double),
-        ("total_ms", C.c_double), ("stacked_pairs", C.c_uint64)]
+        ("total_ms", C.c_double), ("stacked_pairs", C.c_uint64), ("shared_pairs", C.c_uint64)]
 
 
 class Params(C.Structure):
