@@ -45,7 +45,8 @@ hipError_t launch_desc(const FwdArgs &a, const int32_t *items, ItemDesc *out, ui
 hipError_t launch_regroup(const RegroupArgs &r, hipStream_t st);
 uint32_t rescue_tab_bytes(int MR);
 hipError_t launch_rescue_bin(const FwdArgs &a, int grid, hipStream_t st);
-hipError_t launch_rescue_tab(int part, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
+hipError_t launch_rescue_tab(int part, bool share, int grid, size_t lds_bytes, hipStream_t st,
+                             const FwdArgs &a);
 hipError_t launch_ext(int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 int adj2_max_mr();
 void *adj2_kernel_ptr(int L, int MR, bool rescue);
@@ -339,6 +340,8 @@ struct rodv_phmm_plan {
     uint64_t bnd64_elems = 0;
     int rescue_grid = 0;
     int long_grid = 0;          // workgroups of fwd_rescue_long_kernel (0: no long reads)
+    bool share_all = false;     // every read qualifies for shared products: the
+                                // FP64 rescue table kernels use them
     uint32_t long_counter = 0;  // its work counter
     // its units: every pair whose read is longer than one FP64 strip, the
     // longest pipelines first; slots after the grid's rows in d_bnd64
@@ -1173,6 +1176,8 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             read_share[r] = ok_r ? 1 : 0;
         }
     }
+    const bool share_all = cfg.shared_products && std::all_of(read_share.begin(), read_share.end(),
+                                                              [](uint8_t v) { return v != 0; });
 
     rodv_phmm_plan *p = new (std::nothrow) rodv_phmm_plan();
     if (!p) return RODV_PHMM_ERR_OOM;
@@ -1187,6 +1192,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     p->ctx = ctx;
     p->cfg = cfg;
     p->n_pairs = n_pairs;
+    p->share_all = share_all;
     p->arena = arena;
     p->no_plain = no_plain;
     if (arena) arena->used = arena->need = 0;
@@ -1827,7 +1833,7 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                 t.counter = a.bin_cnt + 2 * kRescueClasses + 1 + part;
                 t.lds_stride = p->rtab_stride;
                 const size_t lds = 2u * (p->rtab_stride + rescue_tab_bytes(part == 0 ? 8 : 4));
-                HIPCHK(launch_rescue_tab(part, p->rtab_grid[part], lds, st, t));
+                HIPCHK(launch_rescue_tab(part, p->share_all, p->rtab_grid[part], lds, st, t));
                 launches++;
             }
             a.rescue_list = p->d_bin + p->bin_base[kRescueClasses];

@@ -1443,10 +1443,11 @@ struct Carry64 {
     double yyA, myA;
 };
 
-template <int MR, bool INTER>
-__device__ __forceinline__ void step_tab64(State<double, MR> &S, Carry64 &C, const Priors<double, MR> &P,
-                                           const double (&pr)[MR], const LaneRot<kRtL> &rot,
-                                           double (&pb)[MR], const uint8_t *pnext) {
+// SH: shared products, as step_tab
+template <int MR, bool INTER, bool SH>
+__device__ __forceinline__ void step_tab64(State<double, MR> &S, Carry64 &C, double (&Pm)[MR],
+                                           const Priors<double, MR> &P, const double (&pr)[MR],
+                                           const LaneRot<kRtL> &rot, double (&pb)[MR], const uint8_t *pnext) {
     load_priors64<MR>(pb, pnext);
     double nM[MR], nD[MR];
     double dgM = C.dM, dgI = C.dI, dgD = C.dD;
@@ -1463,7 +1464,7 @@ __device__ __forceinline__ void step_tab64(State<double, MR> &S, Carry64 &C, con
             t = fmaT(dgD, P.g[s], t);
         }
         nM[s] = t * pr[s];
-        const double v = S.M[s] * P.my[s];
+        const double v = SH ? Pm[s] : S.M[s] * P.my[s];
         nD[s] = fmaT(S.D[s], P.yy[s], v);
         dgM = S.M[s];
         dgI = S.I[s];
@@ -1473,6 +1474,7 @@ __device__ __forceinline__ void step_tab64(State<double, MR> &S, Carry64 &C, con
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         const double u = upM * P.mx[s];
+        if (SH && s > 0) Pm[s - 1] = u;
         const double nI = fmaT(upI, P.xx[s], u);
         upM = nM[s];
         upI = nI;
@@ -1480,6 +1482,7 @@ __device__ __forceinline__ void step_tab64(State<double, MR> &S, Carry64 &C, con
         S.M[s] = nM[s];
         S.D[s] = nD[s];
     }
+    if (SH) Pm[MR - 1] = nM[MR - 1] * P.my[MR - 1];
     C.dD = fmaT(C.dD, C.yyA, C.dM * C.myA);
     C.dM = C.uM;
     C.dI = C.uI;
@@ -1489,7 +1492,7 @@ __device__ __forceinline__ void step_tab64(State<double, MR> &S, Carry64 &C, con
 
 // One item of two pairs (p < 0: an empty slot).  LDS per group: the codes
 // (lds_stride bytes) and the table, 4 blocks of 32 * MRT doubles.
-template <int MR, bool INTER>
+template <int MR, bool INTER, bool SH>
 __device__ void rescue_tab_item(const FwdArgs &a, uint8_t *lds, int p) {
     constexpr int L = kRtL;
     constexpr int G = 64 / L;
@@ -1561,11 +1564,13 @@ __device__ void rescue_tab_item(const FwdArgs &a, uint8_t *lds, int p) {
     wave_lds_sync();
 
     State<double, MR> S;
+    double Pm[MR];
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         S.M[s] = 0.0;
         S.I[s] = 0.0;
         S.D[s] = realv[s] ? 0.0 : initd;
+        Pm[s] = 0.0;
     }
     const LaneRot<L> rot;
     Carry64 C;
@@ -1593,7 +1598,7 @@ __device__ void rescue_tab_item(const FwdArgs &a, uint8_t *lds, int p) {
         for (int u = 0; u < kU64; u++) {
             const uint32_t c2 = ld_code(cl + dd + u + 2);
             double pb[MR];
-            step_tab64<MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+            step_tab64<MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
 #pragma unroll
@@ -1604,7 +1609,7 @@ __device__ void rescue_tab_item(const FwdArgs &a, uint8_t *lds, int p) {
     for (; dd <= Dend; dd++) {
         const uint32_t c2 = ld_code(cl + dd + 2);
         double pb[MR];
-        step_tab64<MR, INTER>(S, C, P, pa, rot, pb, tlane + c1 * kBlk);
+        step_tab64<MR, INTER, SH>(S, C, Pm, P, pa, rot, pb, tlane + c1 * kBlk);
         if (dd - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
@@ -1671,22 +1676,22 @@ __global__ __launch_bounds__(256) void rescue_bin_kernel(FwdArgs a) {
 
 // Items of the table pass: two pairs of one class and arithmetic, classes
 // c in [c_lo, c_hi] with the most rows per lane first, inter before intra.
-template <int Q, int C_HI>
+template <int Q, int C_HI, bool SH>
 __device__ __forceinline__ void rescue_tab_switch(const FwdArgs &a, uint8_t *lds, int c, bool inter,
                                                   uint32_t e, uint32_t ck) {
     if (c == Q) {
         int p = -1;
         if (e < ck) p = (int)a.bin[inter ? a.bin_base[Q] + e : a.bin_base[Q] + a.bin_cap[Q] - 1 - e];
         if (inter)
-            rescue_tab_item<Q + 1, true>(a, lds, p);
+            rescue_tab_item<Q + 1, true, SH>(a, lds, p);
         else
-            rescue_tab_item<Q + 1, false>(a, lds, p);
+            rescue_tab_item<Q + 1, false, SH>(a, lds, p);
     } else if constexpr (Q < C_HI) {
-        rescue_tab_switch<Q + 1, C_HI>(a, lds, c, inter, e, ck);
+        rescue_tab_switch<Q + 1, C_HI, SH>(a, lds, c, inter, e, ck);
     }
 }
 
-template <int C_LO, int C_HI>
+template <int C_LO, int C_HI, bool SH>
 __global__ __launch_bounds__(64) void fwd_rescue_tab_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int NK = 2 * (C_HI - C_LO + 1);
@@ -1718,7 +1723,7 @@ __global__ __launch_bounds__(64) void fwd_rescue_tab_kernel(FwdArgs a) {
             if (k == q) ck = cnt[q];
         const int c = C_HI - k / 2;
         const bool inter = (k & 1) == 0;
-        rescue_tab_switch<C_LO, C_HI>(a, lds, c, inter, 2 * j + threadIdx.x / kRtL, ck);
+        rescue_tab_switch<C_LO, C_HI, SH>(a, lds, c, inter, 2 * j + threadIdx.x / kRtL, ck);
     }
 }
 
@@ -2235,10 +2240,13 @@ hipError_t launch_rescue_bin(const FwdArgs &a, int grid, hipStream_t st) {
     hipLaunchKernelGGL(rescue_bin_kernel, dim3(grid), dim3(256), 0, st, a);
     return hipGetLastError();
 }
-hipError_t launch_rescue_tab(int part, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
+hipError_t launch_rescue_tab(int part, bool share, int grid, size_t lds_bytes, hipStream_t st,
+                             const FwdArgs &a) {
     void *args[] = {const_cast<FwdArgs *>(&a)};
-    void *k = part == 0 ? reinterpret_cast<void *>(&fwd_rescue_tab_kernel<4, 7>)
-                        : reinterpret_cast<void *>(&fwd_rescue_tab_kernel<0, 3>);
+    void *k = share ? (part == 0 ? reinterpret_cast<void *>(&fwd_rescue_tab_kernel<4, 7, true>)
+                                 : reinterpret_cast<void *>(&fwd_rescue_tab_kernel<0, 3, true>))
+                    : (part == 0 ? reinterpret_cast<void *>(&fwd_rescue_tab_kernel<4, 7, false>)
+                                 : reinterpret_cast<void *>(&fwd_rescue_tab_kernel<0, 3, false>));
     return hipLaunchKernel(k, dim3(grid), dim3(64), args, lds_bytes, st);
 }
 
