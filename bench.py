@@ -25,6 +25,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector peak (the rescue pass)
 FLOP_PER_CELL = 12        # 8 mul + 4 add per forward cell (pairhmm_impl.cpp:266-289)
 FP_INSTR_PER_CELL = 8     # non-packed FP32 instructions of the bit-exact FMA split
+FP_INSTR_PER_CELL_SH = 7  # the same with the I and D recurrences sharing M x indel prior
 # adjoint, algorithmic work per cell: 12 forward + 24 reverse + 10 parameter
 # accumulation (the kernel's checkpoint recomputation is not counted)
 ADJ_FLOP_PER_CELL = 46
@@ -34,7 +35,7 @@ ADJ_FLOP_PER_CELL = 46
 # WRITE_SIZE x 1.000, the factors a 1 GiB device copy in the same run gives
 # (--copy-calibration-mib 1024).  Keyed by (config, scale, semantics,
 # autotuned, haplotype chains).
-MEASURED_TRAFFIC = {(2, 1.0, "schedule", True, 1): 4.214e9, (2, 1.0, "schedule", True, 0): 2.239e10}
+MEASURED_TRAFFIC = {(2, 1.0, "schedule", True, 1): 4.230e9, (2, 1.0, "schedule", True, 0): 2.239e10}
 
 
 def log(msg):
@@ -228,8 +229,9 @@ def dropin_latency(reps=20):
 def valu_ceiling():
     """FP32 issue ceiling of this device (rodv_amd/libvalu_ceiling.so): lane
     operations per second of independent v_fma_f32, v_mul_f32 and of the
-    forward cell's own 8-instruction mix, at 8 waves per SIMD; the last one
-    divided by 8 is the cells/s a bit-exact non-packed forward can reach."""
+    forward cell's own 8-instruction mix and its 7-instruction shared-product
+    mix, at 8 waves per SIMD; those divided by 8 and 7 are the cells/s a
+    bit-exact non-packed forward can reach."""
     import ctypes
     path = os.path.join(ROOT, "rodv_amd", "libvalu_ceiling.so")
     if not os.path.exists(path):
@@ -240,7 +242,7 @@ def valu_ceiling():
                                  ctypes.POINTER(ctypes.c_double)]
     out = {}
     for kind, name in ((0, "fma_lane_ops_per_s"), (1, "mul_lane_ops_per_s"),
-                       (2, "cell_mix_lane_ops_per_s")):
+                       (2, "cell_mix_lane_ops_per_s"), (3, "cell_mix_shared_lane_ops_per_s")):
         v = ctypes.c_double(0.0)
         rc = lib.valu_ceiling(0, kind, 8, 3, ctypes.byref(v))
         if rc:
@@ -248,7 +250,9 @@ def valu_ceiling():
             return None
         out[name] = round(v.value, 1)
     out["cells_per_s"] = round(out["cell_mix_lane_ops_per_s"] / FP_INSTR_PER_CELL, 1)
+    out["cells_per_s_shared"] = round(out["cell_mix_shared_lane_ops_per_s"] / FP_INSTR_PER_CELL_SH, 1)
     out["tflops_at_12_flop_per_cell"] = round(out["cells_per_s"] * FLOP_PER_CELL / 1e12, 2)
+    out["tflops_at_12_flop_per_cell_shared"] = round(out["cells_per_s_shared"] * FLOP_PER_CELL / 1e12, 2)
     return out
 
 
@@ -267,6 +271,9 @@ def main():
                     help="chained items: 1 where a length bucket is large enough, short reads "
                          "two per lane group where that packs better (default), 2 always, "
                          "3 and 4 as 1 and 2 with one read per lane group, 0 one pair per slot")
+    ap.add_argument("--shared-products", type=int, choices=(0, 1), default=1,
+                    help="reads whose indel priors allow it take the shared-product kernels "
+                         "(7 FP32 instructions per cell, the same bits)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="tune pairs per wavefront / strip width / waves per CU on a 4%% sample "
                          "of the workload's distribution before timing")
@@ -305,7 +312,8 @@ def main():
     if args.autotune:
         t0 = time.time()
         sample = synth.config(args.config, scale=args.scale * 0.04)
-        tn = eng.autotune(sample, default_config(prior_table=args.prior_table))
+        tn = eng.autotune(sample, default_config(prior_table=args.prior_table,
+                                                 shared_products=args.shared_products))
         tuning = {"sample_pairs": sample.num_pairs(), "seconds": round(time.time() - t0, 1),
                   "classes": [{"max_read_len": e["max_read_len"],
                                "pairs_per_wave": e["pairs_per_wave"],
@@ -323,7 +331,8 @@ def main():
     run_cfg = default_config(use_graph=1, prior_table=args.prior_table,
                              waves_per_cu=args.waves_per_cu, semantics=sem,
                              haplotype_chains=args.haplotype_chains,
-                             extended_range=args.extended_range)
+                             extended_range=args.extended_range,
+                             shared_products=args.shared_products)
     plan = eng.plan(batch, run_cfg)
     log(f"plan + upload {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
@@ -368,7 +377,7 @@ def main():
     rescued_cells = int(st["rescued_cells"])
     log(f"fp32 kernels {fp32_ms:.2f} ms, fp64 rescue {fp64_ms:.2f} ms, rescued {rescued} "
         f"({rescued_cells:.3e} cells), {int(st['stacked_pairs'])} pairs in two-read chained items, "
-        f"{int(st['num_buckets'])} FP32 buckets")
+        f"{int(st['shared_pairs'])} in shared-product kernels, {int(st['num_buckets'])} FP32 buckets")
     parity = None
     if args.oracle_regions > 0:
         t0 = time.time()
@@ -428,7 +437,11 @@ def main():
     if ceiling:
         cps = cells / (fp32_ms / 1e3)
         ceiling["forward_cells_per_s"] = round(cps, 1)
-        ceiling["frac_of_cell_mix_ceiling"] = round(cps / ceiling["cells_per_s"], 4)
+        # the workload's mix: shared-product pairs at 7 instructions per cell
+        fs = float(st["shared_pairs"]) / max(1, batch.num_pairs())
+        mix = 1.0 / (fs / ceiling["cells_per_s_shared"] + (1.0 - fs) / ceiling["cells_per_s"])
+        ceiling["workload_cells_per_s"] = round(mix, 1)
+        ceiling["frac_of_cell_mix_ceiling"] = round(cps / mix, 4)
         log(f"VALU ceiling: {ceiling}")
     traffic = MEASURED_TRAFFIC.get((args.config, args.scale, args.semantics, bool(args.autotune),
                                     args.haplotype_chains)) \
@@ -458,6 +471,7 @@ def main():
         "autotune": tuning,
         "parity": parity,
         "stacked_pairs": int(st["stacked_pairs"]),
+        "shared_pairs": int(st["shared_pairs"]),
         "rescued_pairs": rescued,
         "rescued_cells": rescued_cells,
         "rescued_fraction": round(rescued / max(1, batch.num_pairs()), 6),
