@@ -268,6 +268,7 @@ struct Bucket {
     // and `units` (strips 1.. of the others, from that slot) skips it
     std::vector<LongUnit> units1;
     LongUnit *d_units1 = nullptr;
+    void *kern1 = nullptr;  // units1's kernel: kSubMR rows per lane in the bucket's layout
     uint32_t counter1 = 0;  // work counter of the units1 launch
 };
 
@@ -1409,14 +1410,25 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                 bk.grid_cap = bk.grid;
                 // batched: strip 0 of every pair first (units1), then the
                 // other strips split into units, so that a long pair's units
-                // run on several CUs at once
+                // run on several CUs at once.  units1 covers strip 0's rows
+                // as MR / kSubMR sub-strips of kSubMR rows per lane in the
+                // same layout, pipelined over the workgroup's waves (a
+                // shorter step: the lone pipeline of the longest pair's
+                // first strip sets this launch's time)
+                constexpr int kSubMR = 4;
+                const int sub = bk.MR % kSubMR == 0 ? bk.MR / kSubMR : 1;
+                bk.kern1 = sub > 1 ? fwd_long_kernel_ptr(kSubMR, bk.inter) : lk;
                 for (int32_t q : bk.items) {
                     if (q < 0) continue;
-                    const int nstr = strips_for(b->read_lengths[pr[q]], 64, bk.MR);
+                    const uint32_t R = b->read_lengths[pr[q]];
+                    const int nstr = strips_for(R, 64, bk.MR);
                     if (nstr < 1 || nstr > 0xffff) return RODV_PHMM_ERR_INVALID_ARG;
+                    // sub-strips of strip 0: all of them, or those up to row R
+                    const int off = (bk.MR - (int)(R % (uint32_t)bk.MR)) % bk.MR;
+                    const int n1 = sub > 1 ? std::min(sub, ((int)R - 1 + off) / (63 * kSubMR) + 1) : 1;
                     LongUnit u1;
                     u1.pair = q;
-                    u1.strips = 0u | 1u << 16;
+                    u1.strips = 0u | (uint32_t)n1 << 16;
                     u1.xin = kNoSlot;
                     u1.xout = nstr > 1 ? bk.n_xslots++ : kNoSlot;
                     bk.units1.push_back(u1);
@@ -1759,8 +1771,9 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
                     a1.n_items = (uint32_t)bk.units1.size();
                     a1.counter = p->d_counters + bk.counter1;
                     a1.dead_mode = 1;
+                    a1.off_mr = (uint32_t)bk.MR;
                     const size_t lds1 = (size_t)(bk.lds_stride + bk.tab_bytes);
-                    HIPCHK(launch_fwd_block(bk.kern, bk.grid, bk.block, lds1, bs, a1));
+                    HIPCHK(launch_fwd_block(bk.kern1, bk.grid, bk.block, lds1, bs, a1));
                     launches++;
                     if (bk.units.empty()) continue;
                     a.units = bk.d_units;

@@ -120,6 +120,12 @@ struct FwdArgs {
     // finds the pairs whose last row of strip 0 is all zero (kFlagDead);
     // 2 = the other strips, skipping those pairs; 0 = one launch
     uint32_t dead_mode;
+    // long-read kernels: the rows per lane whose layout (leading identity
+    // rows, off = (off_mr - R % off_mr) % off_mr) the strips follow; 0 = the
+    // kernel's own.  The FP32 pass's first launch runs the first strip of
+    // the 16-row layout as sub-strips of 4 rows per lane in that layout, so
+    // that its last row is the row the second launch continues from.
+    uint32_t off_mr;
     // FP64 rescue in classes.  Class c < kRescueClasses: reads of the 32-lane
     // table pass with c + 1 rows per lane; its bin_cap[c] entries from
     // bin[bin_base[c]] hold its inter pairs from the front (bin_cnt[c] of

@@ -1126,7 +1126,10 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
     const uint64_t ro64 = a.read_off[r];
     const uint64_t ro = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(ro64 >> 32)) << 32) |
                         (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)ro64);
-    const int off = (MR - R % MR) % MR;
+    // (off_mr: another rows-per-lane layout whose rows these strips cover, a
+    // multiple of MR: row R still ends its lane, (R + off) % MR == 0)
+    const int omr = a.off_mr ? (int)a.off_mr : MR;
+    const int off = (omr - R % omr) % omr;
     const int vR = R - 1 + off;
     const int nstr = vR / C + 1;
     const int lR = (vR % C) / MR;
@@ -1165,8 +1168,9 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
     bool inject = false, store = false, last = false;
     int Dend = 0, d = 1;
     int cr = w;  // this wave's current strip, s0 + cr
-    // FP32 strip 0 run alone (dead_mode 1): the bits of the last row's
-    // entries at columns 1..H that the storing lane wrote, ORed
+    // the FP32 pass's first launch (dead_mode 1, the first strip of the
+    // pair's layout): the bits of the unit's last row's entries at columns
+    // 1..H that the storing lane wrote, ORed
     const bool detect = !RESCUE && a.dead_mode == 1;
     uint32_t orbits = 0;
     for (int t = 0; t < nblk; t++) {
@@ -1264,8 +1268,8 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
             }
             if (t == st0 + nb - 1) {
                 if constexpr (!RESCUE) {
-                    if (detect && !last) {
-                        // strip 0's last row all zero at columns 1..H: every
+                    if (detect && !last && cr == nu - 1) {
+                        // the unit's last row all zero at columns 1..H: every
                         // later row, and the FP32 sum, are exactly zero -- the
                         // reference's result, which rescues the pair; the
                         // other strips are not run
@@ -1301,7 +1305,16 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
             }
             pub = bnd_out == xb_out && store && xb_out;
         }
-        __threadfence();  // this block's boundary stores before the barrier
+        // this block's boundary stores before the barrier: the rows handed to
+        // the next wave of this workgroup need a workgroup-scope release; the
+        // row a unit's last strip hands to a unit on another workgroup (pub)
+        // an agent-scope one, which writes back this XCD's L2 and is taken by
+        // that wave alone, and only where the consumer runs in the same
+        // launch (the first launch's units are read by the next launch)
+        if (pub && a.dead_mode != 1)
+            __threadfence();
+        else
+            __threadfence_block();
         if (pub && l == L - 2)  // the storing lane: the blocks of the strip written so far
             __hip_atomic_store(a.prog + xout, (uint32_t)(t - st0 + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
