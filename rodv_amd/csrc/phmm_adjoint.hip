@@ -124,8 +124,9 @@ __device__ __forceinline__ void adj_step(const State<T, MR> &Sb, const State<T, 
         const T sd = (s == MR - 1) ? seed : T(0);
         const T oldT = Tb[s];
         const T Mb = fmaT(P.my[s], Db[s], fmaT(b_mx, bI, fmaT(b_mm, bT, sd)));
-        const T Ibn = fmaT(b_xx, bI, fmaT(b_g, bT, sd));
-        const T Dbn = fmaT(P.yy[s], Db[s], b_g * bT);
+        const T gT = b_g * bT;  // fma(b_g, bT, 0) where no seed enters: the same bits
+        const T Ibn = fmaT(b_xx, bI, s == MR - 1 ? fmaT(b_g, bT, sd) : gT);
+        const T Dbn = fmaT(P.yy[s], Db[s], gT);
         // forward values around cell (i, j)
         const T dgM = s == 0 ? dgM0 : Sb.M[s - 1];
         const T dgI = s == 0 ? dgI0 : Sb.I[s - 1];
@@ -218,10 +219,20 @@ struct URows {
     uint32_t rpos[MR];
 };
 
-// forward step of the uniform-transition model (the intra FMA order)
+// forward step of the uniform-transition model (the intra FMA order).  Every
+// row's mx and my are delta, so the product upM * delta that row s's I takes
+// is row s-1's M times delta, which row s-1's D takes in the next step: it
+// is computed once and kept in Pm (Pm[s] == S.M[s] * delta between steps;
+// the lane's last slot computes its own).
+template <typename T, int MR>
+__device__ __forceinline__ void shared_products(const State<T, MR> &S, T dl, T (&Pm)[MR]) {
+#pragma unroll
+    for (int s = 0; s < MR; s++) Pm[s] = S.M[s] * dl;
+}
+
 template <typename T, int L, int MR>
-__device__ __forceinline__ void ustep(State<T, MR> &S, const URows<T, MR> &P, const UTrans<T> &u, T yyA,
-                                      uint32_t hc, const LaneXfer<L, 1> &rot, bool lane0) {
+__device__ __forceinline__ void ustep(State<T, MR> &S, T (&Pm)[MR], const URows<T, MR> &P, const UTrans<T> &u,
+                                      T yyA, uint32_t hc, const LaneXfer<L, 1> &rot, bool lane0) {
     T upM0 = rot(S.M[MR - 1]), upI0 = rot(S.I[MR - 1]);
     upM0 = lane0 ? T(0) : upM0;
     upI0 = lane0 ? T(0) : upI0;
@@ -234,8 +245,10 @@ __device__ __forceinline__ void ustep(State<T, MR> &S, const URows<T, MR> &P, co
         t = fmaT(dgI, u.g, t);
         t = fmaT(dgD, u.g, t);
         const T nM = t * pr;
-        const T nI = fmaT(upI, u.gm, upM * u.dl);
-        const T nD = fmaT(S.D[s], P.yy[s], S.M[s] * u.dl);
+        const T up_dl = upM * u.dl;
+        const T nI = fmaT(upI, u.gm, up_dl);
+        const T nD = fmaT(S.D[s], P.yy[s], Pm[s]);
+        if (s > 0) Pm[s - 1] = up_dl;
         dgM = S.M[s];
         dgI = S.I[s];
         dgD = S.D[s];
@@ -245,6 +258,7 @@ __device__ __forceinline__ void ustep(State<T, MR> &S, const URows<T, MR> &P, co
         S.I[s] = nI;
         S.D[s] = nD;
     }
+    Pm[MR - 1] = S.M[MR - 1] * u.dl;
     S.Dp = fmaT(S.Dp, yyA, S.Mp * u.dl);
     S.Mp = upM0;
     S.Ip = upI0;
@@ -270,8 +284,9 @@ __device__ __forceinline__ void adj_step_u(const State<T, MR> &Sb, const State<T
         const T sd = (s == MR - 1) ? seed : T(0);
         const T oldT = Tb[s];
         const T Mb = fmaT(u.dl, Db[s], fmaT(b_mx, bI, fmaT(b_mm, bT, sd)));
-        const T Ibn = fmaT(b_xx, bI, fmaT(b_g, bT, sd));
-        const T Dbn = fmaT(P.yy[s], Db[s], b_g * bT);
+        const T gT = b_g * bT;  // fma(b_g, bT, 0) where no seed enters: the same bits
+        const T Ibn = fmaT(b_xx, bI, s == MR - 1 ? fmaT(b_g, bT, sd) : gT);
+        const T Dbn = fmaT(P.yy[s], Db[s], gT);
         const T dgM = s == 0 ? dgM0 : Sb.M[s - 1];
         const T dgI = s == 0 ? dgI0 : Sb.I[s - 1];
         const T dgD = s == 0 ? dgD0 : Sb.D[s - 1];
@@ -410,10 +425,12 @@ __device__ void adj2_item(const AdjArgs &a, uint8_t *lds, int p) {
     const int nseg = (D4 + K1 - 1) / K1;
     T *ck = reinterpret_cast<T *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
     State<T, MR> S = S0;
+    T Pm[MR];
+    shared_products(S, u.dl, Pm);
     T sm = T(0), si = T(0);
     put_state(ck, NV, 0, S);
     for (int d = 1; d <= D4; d++) {
-        ustep<T, L, MR>(S, P, u, yyA, cp[d], up, lane0);
+        ustep<T, L, MR>(S, Pm, P, u, yyA, cp[d], up, lane0);
         if (d <= Dend && d - l <= H) {
             sm += S.M[MR - 1];
             si += S.I[MR - 1];
@@ -444,16 +461,17 @@ __device__ void adj2_item(const AdjArgs &a, uint8_t *lds, int p) {
 #pragma unroll
     for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gg[s] = ge[s] = T(0);
     T Tb0p = T(0), gd = T(0), gdT = T(0);
-    auto window4 = [&](const State<T, MR> &W0, int dw) {
+    // Pw: the shared products of W0 (advanced through the window)
+    auto window4 = [&](const State<T, MR> &W0, int dw, T (&Pw)[MR]) {
         State<T, MR> W1, W2, W3, W4;
         W1 = W0;
-        ustep<T, L, MR>(W1, P, u, yyA, cp[dw + 1], up, lane0);
+        ustep<T, L, MR>(W1, Pw, P, u, yyA, cp[dw + 1], up, lane0);
         W2 = W1;
-        ustep<T, L, MR>(W2, P, u, yyA, cp[dw + 2], up, lane0);
+        ustep<T, L, MR>(W2, Pw, P, u, yyA, cp[dw + 2], up, lane0);
         W3 = W2;
-        ustep<T, L, MR>(W3, P, u, yyA, cp[dw + 3], up, lane0);
+        ustep<T, L, MR>(W3, Pw, P, u, yyA, cp[dw + 3], up, lane0);
         W4 = W3;
-        ustep<T, L, MR>(W4, P, u, yyA, cp[dw + 4], up, lane0);
+        ustep<T, L, MR>(W4, Pw, P, u, yyA, cp[dw + 4], up, lane0);
         auto seed_at = [&](int d) {
             const int j = d - l;
             return (l == lR && j >= 1 && j <= H) ? sigma : T(0);
@@ -472,11 +490,13 @@ __device__ void adj2_item(const AdjArgs &a, uint8_t *lds, int p) {
         const int nsteps = min(K1, D4 - d0);  // a multiple of 4
         const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
         get_state(ck, NV, k, S);
+        shared_products(S, u.dl, Pm);
         __syncthreads();
         for (int w = 0; w < nwin; w++) {
             put_state(l2, NV, w, S);
             const int n = min(2 * kW, nsteps - w * 2 * kW);
-            for (int v = 0; v < n; v++) ustep<T, L, MR>(S, P, u, yyA, cp[d0 + w * 2 * kW + v + 1], up, lane0);
+            for (int v = 0; v < n; v++)
+                ustep<T, L, MR>(S, Pm, P, u, yyA, cp[d0 + w * 2 * kW + v + 1], up, lane0);
         }
         __syncthreads();
         for (int w = nwin - 1; w >= 0; w--) {
@@ -487,13 +507,15 @@ __device__ void adj2_item(const AdjArgs &a, uint8_t *lds, int p) {
             if (n == 2 * kW) {
                 State<T, MR> X;
                 get_state(l2, NV, w, X);
+                shared_products(X, u.dl, Pm);
 #pragma unroll
-                for (int v = 1; v <= kW; v++) ustep<T, L, MR>(X, P, u, yyA, cp[dw + v], up, lane0);
-                window4(X, dw + kW);
+                for (int v = 1; v <= kW; v++) ustep<T, L, MR>(X, Pm, P, u, yyA, cp[dw + v], up, lane0);
+                window4(X, dw + kW, Pm);
             }
             State<T, MR> W0;
             get_state(l2, NV, w, W0);
-            window4(W0, dw);
+            shared_products(W0, u.dl, Pm);
+            window4(W0, dw, Pm);
         }
     }
 
