@@ -42,7 +42,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
 
 
 # the engine exports its C ABI and schedule_pairhmm only
-ENGINE_FLAGS = ["-fvisibility=hidden", "-fvisibility-inlines-hidden"]
+ENGINE_FLAGS = ["-fvisibility=hidden", "-fvisibility-inlines-hidden",
+                f"-Wl,--version-script={os.path.join(HERE, 'csrc', 'phmm.map')}"]
 
 
 def build(verbose: bool = False) -> str:
