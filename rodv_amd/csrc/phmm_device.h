@@ -100,15 +100,16 @@ struct Priors {
 // previous step's up; D runs along the row, so the lane recomputes it with
 // that row's own instruction, fma(D, yy, M * my), from the row's yy and my
 // (yyA, myA, set once per strip) -- the same bits.  Two rotations per step.
-template <typename T, int L, int MR, bool INTER>
-__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, T yyA, T myA,
-                                       uint32_t hc, const LaneRot<L> &rot) {
+// `prior(s)` gives slot s's emission prior at this step's column.
+template <typename T, int L, int MR, bool INTER, typename PriorFn>
+__device__ __forceinline__ void step_with(State<T, MR> &S, const Priors<T, MR> &P, T yyA, T myA,
+                                          const LaneRot<L> &rot, PriorFn prior) {
     const T upM0 = rot(S.M[MR - 1]), upI0 = rot(S.I[MR - 1]);
     T dgM = S.Mp, dgI = S.Ip, dgD = S.Dp;
     T upM = upM0, upI = upI0;
 #pragma unroll
     for (int s = 0; s < MR; s++) {
-        T pr = select_prior(hc, P.rpos[s], P.X[s], P.Y[s]);
+        T pr = prior(s);
         T t;
         if (INTER) {
             t = dgI * P.g[s];
@@ -136,6 +137,15 @@ __device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, T 
     S.Dp = fmaT(S.Dp, yyA, S.Mp * myA);
     S.Mp = upM0;
     S.Ip = upI0;
+}
+
+// The step with each slot's prior selected from the one-hot haplotype code
+// hc of the lane's column.
+template <typename T, int L, int MR, bool INTER>
+__device__ __forceinline__ void step(State<T, MR> &S, const Priors<T, MR> &P, T yyA, T myA,
+                                       uint32_t hc, const LaneRot<L> &rot) {
+    step_with<T, L, MR, INTER>(S, P, yyA, myA, rot,
+                               [&](int s) { return select_prior(hc, P.rpos[s], P.X[s], P.Y[s]); });
 }
 
 }  // namespace dev

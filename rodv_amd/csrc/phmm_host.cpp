@@ -33,8 +33,10 @@ int fwd_f64_rescue_mr();
 int rescue_long_waves();
 int rescue_long_rows();
 int rescue_long_mr();
-hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
-void *fwd_long_kernel_ptr(int MR, bool inter);
+hipError_t launch_rescue_long(bool route, bool tab, int grid, size_t lds_bytes, hipStream_t st,
+                              const FwdArgs &a);
+uint32_t long_tab_bytes(int MR, bool fp64);
+void *fwd_long_kernel_ptr(int MR, bool inter, bool tab);
 hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t st);
@@ -343,6 +345,8 @@ struct rodv_phmm_plan {
     int long_grid = 0;          // workgroups of fwd_rescue_long_kernel (0: no long reads)
     bool share_all = false;     // every read qualifies for shared products: the
                                 // FP64 rescue table kernels use them
+    bool long_tab = false;      // every haplotype N-free: the long-read kernels
+                                // read their priors from LDS tables
     uint32_t long_counter = 0;  // its work counter
     // its units: every pair whose read is longer than one FP64 strip, the
     // longest pipelines first; slots after the grid's rows in d_bnd64
@@ -1194,6 +1198,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     p->cfg = cfg;
     p->n_pairs = n_pairs;
     p->share_all = share_all;
+    p->long_tab = std::none_of(hap_has_n.begin(), hap_has_n.end(), [](uint8_t v) { return v != 0; });
     p->arena = arena;
     p->no_plain = no_plain;
     if (arena) arena->used = arena->need = 0;
@@ -1401,9 +1406,11 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             bk.bnd_elems = 3ull * (bk.max_h + 3ull * bk.L + 2);
             // reads longer than one strip: their strips pipelined over the
             // waves of a workgroup, one workgroup per CU
-            void *lk = bk.L == 64 ? fwd_long_kernel_ptr(bk.MR, bk.inter) : nullptr;
+            void *lk = bk.L == 64 ? fwd_long_kernel_ptr(bk.MR, bk.inter, p->long_tab) : nullptr;
             if (lk) {
                 bk.kern = lk;
+                // the waves' prior tables (covers the sub-strip launch's too)
+                bk.tab_bytes = p->long_tab ? long_tab_bytes(bk.MR, false) : 0u;
                 bk.block = 64 * rescue_long_waves();
                 bk.nbnd = (uint32_t)rescue_long_waves() + 1;
                 bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus));
@@ -1417,7 +1424,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                 // first strip sets this launch's time)
                 constexpr int kSubMR = 4;
                 const int sub = bk.MR % kSubMR == 0 ? bk.MR / kSubMR : 1;
-                bk.kern1 = sub > 1 ? fwd_long_kernel_ptr(kSubMR, bk.inter) : lk;
+                bk.kern1 = sub > 1 ? fwd_long_kernel_ptr(kSubMR, bk.inter, p->long_tab) : lk;
                 for (int32_t q : bk.items) {
                     if (q < 0) continue;
                     const uint32_t R = b->read_lengths[pr[q]];
@@ -1862,7 +1869,8 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
             a.fault = p->d_counters + p->fault_counter;
             a.spin_limit = p->spin_limit;
             a.xbnd = p->d_bnd64 + (size_t)p->long_grid * (rescue_long_waves() + 1) * p->bnd64_elems;
-            HIPCHK(launch_rescue_long(p->any_inter, p->long_grid, p->rescue_lds, st, a));
+            const size_t llds = (size_t)p->rescue_lds + (p->long_tab ? long_tab_bytes(rescue_long_mr(), true) : 0u);
+            HIPCHK(launch_rescue_long(p->any_inter, p->long_tab, p->long_grid, llds, st, a));
             launches++;
         }
     }

@@ -1111,11 +1111,41 @@ __device__ __forceinline__ double lane_value(double v, int k) {
                             __builtin_amdgcn_readlane(__double2loint(v), k));
 }
 
-template <typename T, int MR, bool INTER, bool RESCUE>
+// Long-read prior tables (TAB): per wave, the emission priors of its strip's
+// rows for each haplotype base, [4 bases][chunks][64 lanes][16 bytes], a
+// lane's MR rows in LT::E-element 16-byte chunks.
+template <typename T, int MR>
+struct LongTab {
+    static constexpr int E = 16 / (int)sizeof(T);               // elements per chunk
+    static constexpr int NC = (MR + E - 1) / E;                 // chunks per lane
+    static constexpr uint32_t kBlk = 64u * (uint32_t)NC * 16u;  // bytes per base
+    static constexpr uint32_t kWave = 4u * kBlk;
+};
+
+template <typename T, int MR>
+__device__ __forceinline__ void long_priors(T (&pr)[MR], const uint8_t *p) {
+    using LT = LongTab<T, MR>;
+    const uint4 *tp = reinterpret_cast<const uint4 *>(p);
+#pragma unroll
+    for (int q = 0; q < LT::NC; q++) {
+        const uint4 v = tp[q * 64];
+        const T *e = reinterpret_cast<const T *>(&v);
+#pragma unroll
+        for (int k = 0; k < LT::E; k++)
+            if (q * LT::E + k < MR) pr[q * LT::E + k] = e[k];
+    }
+}
+
+// TAB: the haplotype has no N, and each step reads its slots' priors from
+// the wave's table in LDS (one 16-byte read per chunk, the next column's
+// issued a step ahead) instead of selecting them per cell from the one-hot
+// code (2 VALU instructions per cell in FP32, 4 in FP64).
+template <typename T, int MR, bool INTER, bool RESCUE, bool TAB>
 __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1, uint32_t xin,
                           uint32_t xout) {
     constexpr int L = 64;
     constexpr int C = (L - 1) * MR;
+    using LT = LongTab<T, MR>;
     static_assert(kLB >= L - 1, "a block covers the boundary row's lag");
     const int l = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -1136,18 +1166,30 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
     const T initd = Tab<T>::initd(a, h);
 
     // one-hot codes of the haplotype, shared by the waves: column j at
-    // lds[L - 1 + j]
+    // lds[L - 1 + j] (TAB: the base's block 0..3 instead)
     for (uint32_t k = threadIdx.x * 16u; k < a.lds_stride; k += 64u * kLW * 16u)
         *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
     __syncthreads();
     {
         const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
         const int nch = (H + 15) >> 4;
-        for (int c = threadIdx.x; c < nch; c += 64 * kLW)
-            *reinterpret_cast<uint4 *>(lds + L + 16 * c) = src[c];
+        for (int c = threadIdx.x; c < nch; c += 64 * kLW) {
+            uint4 v = src[c];
+            if constexpr (TAB) {  // one-hot 1, 2, 4, 8 -> 0..3 per byte
+                v.x = ((v.x >> 1) & 0x7f7f7f7fu) - ((v.x >> 3) & 0x1f1f1f1fu);
+                v.y = ((v.y >> 1) & 0x7f7f7f7fu) - ((v.y >> 3) & 0x1f1f1f1fu);
+                v.z = ((v.z >> 1) & 0x7f7f7f7fu) - ((v.z >> 3) & 0x1f1f1f1fu);
+                v.w = ((v.w >> 1) & 0x7f7f7f7fu) - ((v.w >> 3) & 0x1f1f1f1fu);
+            }
+            *reinterpret_cast<uint4 *>(lds + L + 16 * c) = v;
+        }
     }
     __syncthreads();
     const uint8_t *cp = lds + (L - 1) - l;
+    // this wave's prior table (TAB), after the codes; lane l's chunks
+    uint8_t *tabl = lds + a.lds_stride + (uint32_t)w * LT::kWave + (uint32_t)l * 16u;
+    T pa[MR];        // TAB: the priors of the current step's column
+    uint32_t c1 = 0; // TAB: the code of the next step's column
 
     if (s1 > nstr) s1 = nstr;
     const int nu = s1 - s0;  // strips of this unit
@@ -1192,6 +1234,19 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
                     S.M[s] = T(0);
                     S.I[s] = T(0);
                     S.D[s] = real ? T(0) : initd;
+                }
+                if constexpr (TAB) {
+                    // the wave's table for this strip (its previous strip's
+                    // reads of it have all returned: their values were used)
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+#pragma unroll
+                        for (int s = 0; s < MR; s++)
+                            reinterpret_cast<T *>(tabl + c * LT::kBlk + (s / LT::E) * (64u * 16u))[s % LT::E] =
+                                P.rpos[s] == (uint32_t)c ? P.X[s] : P.Y[s];
+                    wave_lds_sync();
+                    long_priors<T, MR>(pa, tabl + ld_code(cp + 1) * LT::kBlk);
+                    c1 = ld_code(cp + 2);
                 }
                 S.Mp = T(0);
                 S.Ip = T(0);
@@ -1239,7 +1294,17 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
             }
             if (!xlive && cr == 0) fm = T(__builtin_nan(""));
             for (; d <= dlim; d++) {
-                step<T, L, MR, INTER>(S, P, yyA, myA, cp[d], rot);
+                if constexpr (TAB) {
+                    const uint32_t c2 = ld_code(cp + d + 2);
+                    T pb[MR];
+                    long_priors<T, MR>(pb, tabl + c1 * LT::kBlk);
+                    step_with<T, L, MR, INTER>(S, P, yyA, myA, rot, [&](int s) { return pa[s]; });
+#pragma unroll
+                    for (int s2 = 0; s2 < MR; s2++) pa[s2] = pb[s2];
+                    c1 = c2;
+                } else {
+                    step<T, L, MR, INTER>(S, P, yyA, myA, cp[d], rot);
+                }
                 if (last && d - l <= H) {
                     sm += S.M[MR - 1];
                     si += S.I[MR - 1];
@@ -1326,7 +1391,7 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
 // leaves them), one unit per workgroup at a time; a unit whose pair the
 // FP32 pass did not reject is skipped, like every other unit of its pair.
 // ROUTE: per pair inter or intra FP64 as the regroup decided.
-template <bool ROUTE>
+template <bool ROUTE, bool TAB>
 __global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t s_it;
@@ -1342,16 +1407,16 @@ __global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
         if (!(fl & 0x01u)) continue;  // RESCUED
         const int s0 = (int)(u.strips & 0xffffu), s1 = (int)(u.strips >> 16);
         if (__builtin_amdgcn_readfirstlane((ROUTE && (fl & kFlagGrouped64)) ? 1 : 0))
-            long_pair<double, kLongRescueMR, true, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
+            long_pair<double, kLongRescueMR, true, true, TAB>(a, lds, u.pair, s0, s1, u.xin, u.xout);
         else
-            long_pair<double, kLongRescueMR, false, true>(a, lds, u.pair, s0, s1, u.xin, u.xout);
+            long_pair<double, kLongRescueMR, false, true, TAB>(a, lds, u.pair, s0, s1, u.xin, u.xout);
     }
 }
 
 // The FP32 pass of a 64-lane select-prior bucket whose reads need more than
 // one strip, the pairs' strips pipelined over the workgroup's waves as
 // above: over a.units when given, else one whole pair per item.
-template <int MR, bool INTER>
+template <int MR, bool INTER, bool TAB>
 __global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t s_it;
@@ -1379,7 +1444,7 @@ __global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
         } else {
             p = a.items[it];
         }
-        if (p >= 0) long_pair<float, MR, INTER, false>(a, lds, p, s0, s1, xin, xout);
+        if (p >= 0) long_pair<float, MR, INTER, false, TAB>(a, lds, p, s0, s1, xin, xout);
     }
 }
 
@@ -2222,14 +2287,25 @@ void *fwd_f64_rescue_kernel(bool route) {
 
 // FP64 rescue of the reads longer than one strip (workgroups of kLW waves).
 int rescue_long_waves() { return kLW; }
-void *fwd_long_kernel_ptr(int MR, bool inter) {
+// tab: the LDS prior-table variant (every haplotype of the plan N-free)
+void *fwd_long_kernel_ptr(int MR, bool inter, bool tab) {
+#define LK(m)                                                                                \
+    case m:                                                                                  \
+        if (tab)                                                                             \
+            return inter ? reinterpret_cast<void *>(&fwd_long_kernel<m, true, true>)        \
+                         : reinterpret_cast<void *>(&fwd_long_kernel<m, false, true>);      \
+        return inter ? reinterpret_cast<void *>(&fwd_long_kernel<m, true, false>)           \
+                     : reinterpret_cast<void *>(&fwd_long_kernel<m, false, false>);
     switch (MR) {
-    case 4: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<4, true>) : reinterpret_cast<void *>(&fwd_long_kernel<4, false>);
-    case 8: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<8, true>) : reinterpret_cast<void *>(&fwd_long_kernel<8, false>);
-    case 12: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<12, true>) : reinterpret_cast<void *>(&fwd_long_kernel<12, false>);
-    case 16: return inter ? reinterpret_cast<void *>(&fwd_long_kernel<16, true>) : reinterpret_cast<void *>(&fwd_long_kernel<16, false>);
+        LK(4) LK(8) LK(12) LK(16)
     default: return nullptr;
     }
+#undef LK
+}
+// LDS of the prior tables of a long-read workgroup (added to its code buffer)
+uint32_t long_tab_bytes(int MR, bool fp64) {
+    const uint32_t e = fp64 ? 2u : 4u;
+    return (uint32_t)kLW * 4u * 64u * (((uint32_t)MR + e - 1) / e) * 16u;
 }
 hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
     void *args[] = {const_cast<FwdArgs *>(&a)};
@@ -2237,9 +2313,12 @@ hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, h
 }
 int rescue_long_rows() { return kRescueStrip; }
 int rescue_long_mr() { return kLongRescueMR; }
-hipError_t launch_rescue_long(bool route, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
-    void *kern = route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true>)
-                       : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false>);
+hipError_t launch_rescue_long(bool route, bool tab, int grid, size_t lds_bytes, hipStream_t st,
+                              const FwdArgs &a) {
+    void *kern = tab ? (route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true, true>)
+                              : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false, true>))
+                     : (route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true, false>)
+                              : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false, false>));
     void *args[] = {const_cast<FwdArgs *>(&a)};
     return hipLaunchKernel(kern, dim3(grid), dim3(64 * kLW), args, lds_bytes, st);
 }

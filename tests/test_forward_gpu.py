@@ -354,3 +354,39 @@ def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
         np.testing.assert_array_equal(got[1], ref[1])
         np.testing.assert_array_equal(got[0], ref[0])
     plan.close()
+
+
+def test_long_reads_haplotype_with_n(engine, oracle_mod):
+    # Long reads over a haplotype with N: the plan then keeps the long-read
+    # kernels' per-cell prior select (the LDS prior tables need N-free
+    # haplotypes); batched (units, the sub-strip first launch) and plain
+    # launches against the oracle and each other, FP32 results and rescues.
+    rng = np.random.default_rng(29)
+    lens = [300, 1100, 2300]
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    h0 = acgt[rng.integers(0, 4, 2600)].copy()
+    h0[rng.choice(2600, 12, replace=False)] = ord("N")
+    h1 = acgt[rng.integers(0, 4, 2000)]
+    reads = []
+    for n in lens:
+        st = int(rng.integers(0, 2600 - n + 1))
+        r = h0[st:st + n].copy()
+        r[r == ord("N")] = ord("A")
+        reads.append(r.tobytes())
+    q = [rng.integers(20, 41, n) for n in lens]
+    ins = [np.full(n, 45) for n in lens]
+    gcp = [np.full(n, 10) for n in lens]
+    b = Batch.from_lists([h0.tobytes(), h1.tobytes()], reads, q, ins, ins, gcp)
+    out, fl, _ = check_against_oracle(oracle_mod, engine, b)
+    n = len(lens)
+    assert not (fl[:n] & FLAG_RESCUED).any()
+    assert (fl[n:][np.array(lens) > 378] & FLAG_RESCUED).all()
+    plan = engine.plan(b, default_config(semantics=0))
+    plan.execute_plain()
+    ref = plan.read(raw=True)
+    plan.execute()
+    got = plan.read(raw=True)
+    plan.close()
+    np.testing.assert_array_equal(got[2].view(np.uint32), ref[2].view(np.uint32))
+    np.testing.assert_array_equal(got[1], ref[1])
+    np.testing.assert_array_equal(got[0], ref[0])
