@@ -71,7 +71,7 @@ template <> struct PTab<double> {
 template <typename T, int MR>
 __device__ __forceinline__ void put_state(T *base, int nvec, int k, const State<T, MR> &S) {
     // element v of state k of this lane at base[(k*nvec + v)*64 + lane]
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     T *p = base + (size_t)k * nvec * kL + lane;
 #pragma unroll
     for (int s = 0; s < MR; s++) {
@@ -86,7 +86,7 @@ __device__ __forceinline__ void put_state(T *base, int nvec, int k, const State<
 
 template <typename T, int MR>
 __device__ __forceinline__ void get_state(const T *base, int nvec, int k, State<T, MR> &S) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const T *p = base + (size_t)k * nvec * kL + lane;
 #pragma unroll
     for (int s = 0; s < MR; s++) {
@@ -621,7 +621,7 @@ __device__ __forceinline__ void long_setup(const AdjArgs &a, int cs, int nstr, i
                                            double initd, Priors<double, MR> &P, AdjRows<double, MR> &A,
                                            State<double, MR> &S0) {
     constexpr int C = (kL - 1) * MR;
-    const int l = threadIdx.x;
+    const int l = threadIdx.x & 63;
 #pragma unroll
     for (int s = 0; s < MR; s++) {
         const int i = cs * C + l * MR + s - off + 1;
@@ -666,11 +666,63 @@ __device__ __forceinline__ void long_setup(const AdjArgs &a, int cs, int nstr, i
     A.nb_xx = wrap ? 0.0 : (below ? a.gamma : xx);
 }
 
-template <int MR>
+// Waves of a long-read workgroup: one per SIMD.  Pass A runs in wave 0.
+// Pass B deals the strips to the waves, last first: wave w takes strips
+// nstr-1-w, nstr-1-w-W, ...  A strip's reverse reads the adjoints the strip
+// below hands up, which that strip writes column by column as its own
+// reverse proceeds from the last column.  Before staging the columns of a
+// checkpoint segment, a wave waits until the strip below has published (in
+// LDS: prog[cs + 1], the lowest step it has finished) every column the
+// segment reads; its forward recomputation (pass 1) waits for nothing.
+// The strips add their gradient sums into the per-lane totals in the order
+// one wave would (the strip below first, waiting for its prog = 0), so the
+// results are the same bits for any number of waves.  The handed-up rows go
+// through W + 1 HBM rows per workgroup: strip cs writes row cs % (W + 1),
+// which strip cs + W + 1 wrote last and strip cs + W, the previous strip of
+// the same wave, read last.  Every wait is bounded (a.spin_limit polls);
+// one that runs out counts in *a.fault, the strip's handed-up adjoints
+// become NaN and its wave waits no more.
+constexpr int kLongWaves = 4;
+constexpr int kProgNone = 0x7fffffff;
+
+// LDS of a long-read workgroup: hap codes (lds_stride bytes) | per wave:
+// level-2 states, feeder inputs of a segment, adjoints from below of a
+// segment | per-lane gradient totals (10 x 64) | scaled sum | the strips'
+// shifts | the strips' progress | the item taken
+__host__ __device__ constexpr uint64_t long_wave_elems(int K1, int MR) {
+    return (uint64_t)((K1 + 7) / 8) * (3 * MR + 3) * kL + 3ull * K1 + 2ull * (K1 + 1);
+}
+__host__ __device__ constexpr uint64_t long_lds_bytes(uint32_t lds_stride, int K1, int MR, int W) {
+    return lds_stride + ((uint64_t)W * long_wave_elems(K1, MR) + 10ull * kL + 1) * 8 + 8ull * kMaxLongStrips + 16;
+}
+
+// Waits until the published progress *c is <= need; false if `limit` polls
+// (of about half a microsecond each) did not see it.  Wave-uniform.
+__device__ __forceinline__ bool wait_progress(const int *c, int need, uint32_t limit) {
+    for (uint32_t k = 0; k < limit; k++) {
+        const int v = uni(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (v <= need) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the handed-up rows after the count
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(16);
+    }
+    return false;
+}
+__device__ __forceinline__ void publish_progress(int *c, int v) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // the handed-up rows before the count
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// LDS written by other lanes of this wave, before it is read (one wave's
+// LDS operations complete in order)
+__device__ __forceinline__ void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+template <int MR, int W>
 __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     using T = double;
-    const int lane = threadIdx.x;
-    const int l = lane;
+    const int l = threadIdx.x & 63;
+    const int lane = l;
+    const int w = uni((int)(threadIdx.x >> 6));
     constexpr int NV = 3 * MR + 3;
     constexpr int C = (kL - 1) * MR;
     const int p = uni(p_in);
@@ -686,22 +738,23 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     const T initd = a.initd_d[h];
     const int K1 = a.ckpt;
 
-    // LDS: hap codes | level-2 states | feeder inputs of a segment | adjoints
-    // from below of a segment | the strips' shifts
-    T *l2 = reinterpret_cast<T *>(lds + a.lds_stride);
+    T *wbase = reinterpret_cast<T *>(lds + a.lds_stride);
+    T *l2 = wbase + (size_t)w * long_wave_elems(K1, MR);
     T *fst = l2 + (size_t)((K1 + 7) / 8) * NV * kL;
     T *bst = fst + 3 * K1;
-    T *tot = bst + 2 * (K1 + 1);  // per-lane gradient totals, tot[t * 64 + lane]
-    int *shs = reinterpret_cast<int *>(tot + 10 * kL);
-    __syncthreads();
-    for (uint32_t k = lane * 16; k < a.lds_stride; k += 64 * 16)
+    T *tot = wbase + (size_t)W * long_wave_elems(K1, MR);  // per-lane totals, tot[t * 64 + lane]
+    T *ssum = tot + 10 * kL;
+    int *shs = reinterpret_cast<int *>(ssum + 1);
+    int *prog = shs + kMaxLongStrips;
+    __syncthreads();  // the previous item's LDS reads are done
+    for (uint32_t k = threadIdx.x * 16; k < a.lds_stride; k += 64 * W * 16)
         *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
     {
         const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
         const int nch = (H + 15) >> 4;
         __syncthreads();
         // one-hot codes with bit 4 set: a read N matches every column
-        for (int c = l; c < nch; c += kL) {
+        for (int c = threadIdx.x; c < nch; c += kL * W) {
             const uint4 v = src[c];
             *reinterpret_cast<uint4 *>(lds + kL + 16 * c) =
                 make_uint4(v.x | 0x10101010u, v.y | 0x10101010u, v.z | 0x10101010u, v.w | 0x10101010u);
@@ -712,83 +765,87 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     const LaneXfer<kL, 1> up;
     const LaneXfer<kL, -1> down;
     T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)blockIdx.x * a.lbnd_stride;  // row of strip cs at cs * lrow
-    T *AB = reinterpret_cast<T *>(a.lab) + (uint64_t)blockIdx.x * 2 * a.lab_elems;
-    T *ck = reinterpret_cast<T *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
+    T *AB = reinterpret_cast<T *>(a.lab) + (uint64_t)blockIdx.x * (W + 1) * a.lab_elems;
+    T *ck = reinterpret_cast<T *>(a.ck) + ((uint64_t)blockIdx.x * W + w) * a.ck_elems;
 
-    // ---- A: forward over the strips
+    // ---- A: forward over the strips (wave 0)
     int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
-    T sm = T(0), si = T(0);
-    if (lane == 0) shs[0] = 0;
-    for (int cs = 0; cs < nstr; cs++) {
-        Priors<T, MR> P;
-        AdjRows<T, MR> A;
-        State<T, MR> S;
-        long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S);
-        __syncthreads();
-        const bool last = cs == nstr - 1;
-        const int sh = shs[cs];
-        const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
-        T *bout = B + (uint64_t)cs * a.lrow;
-        if (cs > 0) {  // column 1 of the row above (column j at 3 * (j + L))
-            const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
-                    bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
-            if (l == kL - 1) {
-                S.M[MR - 1] = bm;
-                S.I[MR - 1] = bi;
-                S.D[MR - 1] = bd;
-            }
-        }
-        // a strip that is not the last stores its last row up to column H + 69,
-        // past every column the recomputations' feeders read (D4 + 1)
-        const int Dend = last ? H + lR : H + 2 * kL + 3;
-        T bmax = T(0);
-        sm = T(0);
-        si = T(0);
-        for (int d0 = 1; d0 <= Dend; d0 += kL) {
-            const int dlim = d0 + kL - 1 < Dend ? d0 + kL - 1 : Dend;
-            T fm = T(0), fi = T(0), fd = T(0);
-            if (cs > 0 && d0 + l <= dlim) {
-                const T *in = bin + 3 * (d0 + l + 1 + kL);
-                fm = ldexp(ld_ag(in), sh);
-                fi = ldexp(ld_ag(in + 1), sh);
-                fd = ldexp(ld_ag(in + 2), sh);
-            }
-            for (int d = d0; d <= dlim; d++) {
-                step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
-                if (last && d - l <= H) {
-                    sm += S.M[MR - 1];
-                    si += S.I[MR - 1];
+    if (w == 0) {
+        T sm = T(0), si = T(0);
+        if (lane == 0) shs[0] = 0;
+        for (int cs = 0; cs < nstr; cs++) {
+            Priors<T, MR> P;
+            AdjRows<T, MR> A;
+            State<T, MR> S;
+            long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S);
+            wave_sync();
+            const bool last = cs == nstr - 1;
+            const int sh = shs[cs];
+            const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
+            T *bout = B + (uint64_t)cs * a.lrow;
+            if (cs > 0) {  // column 1 of the row above (column j at 3 * (j + L))
+                const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
+                        bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
+                if (l == kL - 1) {
+                    S.M[MR - 1] = bm;
+                    S.I[MR - 1] = bi;
+                    S.D[MR - 1] = bd;
                 }
-                if (!last && l == kL - 2) {  // column d - (L-2) at 3 * (d + 2)
-                    T *o = bout + 3 * (d + 2);
-                    st_ag(o, S.M[MR - 1]);
-                    st_ag(o + 1, S.I[MR - 1]);
-                    st_ag(o + 2, S.D[MR - 1]);
-                    bmax = fmax(bmax, fmax(S.M[MR - 1], fmax(S.I[MR - 1], S.D[MR - 1])));
+            }
+            // a strip that is not the last stores its last row up to column
+            // H + 69, past every column the recomputations' feeders read (D4 + 1)
+            const int Dend = last ? H + lR : H + 2 * kL + 3;
+            T bmax = T(0);
+            sm = T(0);
+            si = T(0);
+            for (int d0 = 1; d0 <= Dend; d0 += kL) {
+                const int dlim = d0 + kL - 1 < Dend ? d0 + kL - 1 : Dend;
+                T fm = T(0), fi = T(0), fd = T(0);
+                if (cs > 0 && d0 + l <= dlim) {
+                    const T *in = bin + 3 * (d0 + l + 1 + kL);
+                    fm = ldexp(ld_ag(in), sh);
+                    fi = ldexp(ld_ag(in + 1), sh);
+                    fd = ldexp(ld_ag(in + 2), sh);
                 }
-                if (cs > 0) {  // column d + 1 of the row above
-                    const int k = d - d0;
-                    const T bm = lane_value(fm, k), bi = lane_value(fi, k), bd = lane_value(fd, k);
-                    if (l == kL - 1) {
-                        S.M[MR - 1] = bm;
-                        S.I[MR - 1] = bi;
-                        S.D[MR - 1] = bd;
+                for (int d = d0; d <= dlim; d++) {
+                    step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
+                    if (last && d - l <= H) {
+                        sm += S.M[MR - 1];
+                        si += S.I[MR - 1];
+                    }
+                    if (!last && l == kL - 2) {  // column d - (L-2) at 3 * (d + 2)
+                        T *o = bout + 3 * (d + 2);
+                        st_ag(o, S.M[MR - 1]);
+                        st_ag(o + 1, S.I[MR - 1]);
+                        st_ag(o + 2, S.D[MR - 1]);
+                        bmax = fmax(bmax, fmax(S.M[MR - 1], fmax(S.I[MR - 1], S.D[MR - 1])));
+                    }
+                    if (cs > 0) {  // column d + 1 of the row above
+                        const int k = d - d0;
+                        const T bm = lane_value(fm, k), bi = lane_value(fi, k), bd = lane_value(fd, k);
+                        if (l == kL - 1) {
+                            S.M[MR - 1] = bm;
+                            S.I[MR - 1] = bi;
+                            S.D[MR - 1] = bd;
+                        }
                     }
                 }
             }
+            if (!last) {
+                __threadfence();  // the row's stores before the next strip's loads
+                const T mx = lane_value(bmax, kL - 2);
+                const int nsh = mx > T(0) ? kLongTop - ilogb(mx) : 0;
+                if (lane == 0) shs[cs + 1] = nsh;
+                E += nsh;
+            }
         }
-        if (!last) {
-            __threadfence();  // the row's stores before the next strip's loads
-            const T mx = lane_value(bmax, kL - 2);
-            const int nsh = mx > T(0) ? kLongTop - ilogb(mx) : 0;
-            if (lane == 0) shs[cs + 1] = nsh;
-            E += nsh;
-        }
+        const T Ssum = lane_value(sm + si, lR);  // pairhmm_impl.cpp:434-437
+        if (lane == 0) *ssum = Ssum;
     }
     __syncthreads();
-    const T Ssum = lane_value(sm + si, lR);  // pairhmm_impl.cpp:434-437
-    if (!(Ssum > T(0))) {                    // underflow even renormalised: no gradient
-        if (lane == 0) {
+    const T Ssum = *ssum;
+    if (!(Ssum > T(0))) {  // underflow even renormalised: no gradient
+        if (threadIdx.x == 0) {
             a.out_log10[p] = -INFINITY;
             for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)p + t] = 0.0;
             a.flags[p] = 0x03;
@@ -796,11 +853,13 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         return;
     }
     const T sigma = ldexp(1.0, -ilogb(Ssum));
+    for (int k = threadIdx.x; k < 10 * kL; k += kL * W) tot[k] = T(0);
+    for (int k = threadIdx.x; k < kMaxLongStrips; k += kL * W) prog[k] = kProgNone;
+    __syncthreads();
 
-    // ---- B: strips last to first
-#pragma unroll
-    for (int t = 0; t < 10; t++) tot[t * kL + lane] = T(0);
-    for (int cs = nstr - 1; cs >= 0; cs--) {
+    // ---- B: strips last to first, dealt to the waves
+    bool live = true;  // this wave's waits have not run out
+    for (int cs = nstr - 1 - w; cs >= 0; cs -= W) {
         Priors<T, MR> P;
         AdjRows<T, MR> A;
         State<T, MR> S0;
@@ -809,8 +868,8 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         const int sh = shs[cs];
         const T fb = last ? T(0) : ldexp(1.0, shs[cs + 1]);  // adjoints from below to this strip's scale
         const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
-        const T *ab_in = AB + (uint64_t)((cs + 1) & 1) * a.lab_elems;
-        T *ab_out = AB + (uint64_t)(cs & 1) * a.lab_elems;
+        const T *ab_in = AB + (uint64_t)((cs + 1) % (W + 1)) * a.lab_elems;
+        T *ab_out = AB + (uint64_t)(cs % (W + 1)) * a.lab_elems;
         if (cs > 0) {
             const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
                     bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
@@ -827,13 +886,13 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         const int nseg = (D4 + K1 - 1) / K1;
         // feeder inputs of the segment after step d0s: column d0s + 2 + m at fst[3m]
         auto stage_feeder = [&](int d0s) {
-            __syncthreads();
+            wave_sync();
             if (cs > 0)
                 for (int k = l; k < 3 * K1; k += kL) {
                     const int c = d0s + 2 + k / 3;
                     fst[k] = ldexp(ld_ag(bin + 3 * (c + kL) + k % 3), sh);
                 }
-            __syncthreads();
+            wave_sync();
         };
         // a forward step, then the feeder's injection of column d + 1
         auto fstep = [&](State<T, MR> &S, int d, int d0s) {
@@ -906,27 +965,36 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
             const int nsteps = min(K1, D4 - d0s);
             const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
             stage_feeder(d0s);
-            if (!last)
+            if (!last) {
+                // columns d0s - 61 .. d0s - 61 + K1 of the rows handed up,
+                // once the strip below has finished their steps
+                const int lo = max(1, d0s - 61), hi = min(H, d0s - 61 + K1);
+                if (live && lo <= hi) {
+                    live = wait_progress(prog + cs + 1, lo, a.spin_limit);
+                    if (!live && l == 0) atomicAdd(a.fault, 1u);
+                }
+                const T poison = live ? T(1) : T(__builtin_nan(""));
                 for (int m = l; m <= K1; m += kL) {
                     const int c = d0s - 61 + m;
                     const bool in = c >= 1 && c <= H;  // beyond: no adjoint
-                    bst[2 * m] = in ? ld_ag(ab_in + 2 * c) * fb : T(0);
-                    bst[2 * m + 1] = in ? ld_ag(ab_in + 2 * c + 1) * fb : T(0);
+                    bst[2 * m] = in ? ld_ag(ab_in + 2 * c) * fb * poison : T(0);
+                    bst[2 * m + 1] = in ? ld_ag(ab_in + 2 * c + 1) * fb * poison : T(0);
                 }
-            __syncthreads();
+            }
+            wave_sync();
             State<T, MR> S;
             get_state(ck, NV, k, S);
-            for (int w = 0; w < nwin; w++) {
-                put_state(l2, NV, w, S);
-                const int n = min(2 * kW, nsteps - w * 2 * kW);
-                for (int u = 0; u < n; u++) fstep(S, d0s + w * 2 * kW + u + 1, d0s);
+            for (int wi = 0; wi < nwin; wi++) {
+                put_state(l2, NV, wi, S);
+                const int n = min(2 * kW, nsteps - wi * 2 * kW);
+                for (int u = 0; u < n; u++) fstep(S, d0s + wi * 2 * kW + u + 1, d0s);
             }
-            __syncthreads();
-            for (int w = nwin - 1; w >= 0; w--) {
-                const int dw = d0s + w * 2 * kW;
-                const int n = min(2 * kW, nsteps - w * 2 * kW);
+            wave_sync();
+            for (int wi = nwin - 1; wi >= 0; wi--) {
+                const int dw = d0s + wi * 2 * kW;
+                const int n = min(2 * kW, nsteps - wi * 2 * kW);
                 State<T, MR> W0;
-                get_state(l2, NV, w, W0);
+                get_state(l2, NV, wi, W0);
                 if (n == 2 * kW) {
                     State<T, MR> X = W0;
 #pragma unroll
@@ -935,8 +1003,14 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
                 }
                 window4(W0, dw, d0s);
             }
+            // steps d0s + 1 .. of this strip done: their handed-up columns
+            if (cs > 0) publish_progress(prog + cs, d0s + 1);
         }
-        // the strip's rows into the per-lane totals
+        // the strip's rows into the per-lane totals, after the strip below's
+        if (!last && live) {
+            live = wait_progress(prog + cs + 1, 0, a.spin_limit);
+            if (!live && l == 0) atomicAdd(a.fault, 1u);
+        }
 #pragma unroll
         for (int s = 0; s < MR; s++) {
             const int i = cs * C + l * MR + s - off + 1;
@@ -948,8 +1022,10 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
             const T c = rl ? ge[s] * a.wq[qq] : T(0);
             tot[(2 + bin_q) * kL + lane] += c;
         }
-        __threadfence();  // the handed-up adjoints before the strip above loads them
+        publish_progress(prog + cs, 0);
     }
+    __syncthreads();  // every strip's totals
+    if (w != 0) return;
     double acc[10];
 #pragma unroll
     for (int t = 0; t < 10; t++) {
@@ -969,16 +1045,19 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     }
 }
 
-__global__ __launch_bounds__(64, 1) void adj_long_kernel(AdjArgs a) {
+template <int W>
+__global__ __launch_bounds__(64 * W, 1) void adj_long_kernel(AdjArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t n = a.n_items;
-    // a wave takes at most n items: the bound makes every wave exit
+    int *slot = reinterpret_cast<int *>(lds + long_lds_bytes(a.lds_stride, a.ckpt, kLongMR, W) - 16);
+    // a workgroup takes at most n items: the bound makes every wave exit
     for (uint32_t iter = 0; iter <= n; iter++) {
-        uint32_t it = 0;
-        if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
-        it = __builtin_amdgcn_readfirstlane(it);
+        __syncthreads();  // every wave has read the slot
+        if (threadIdx.x == 0) *slot = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint32_t it = (uint32_t)uni(*slot);
         if (it >= n) break;
-        adj_long_item<kLongMR>(a, lds, a.items[it]);
+        adj_long_item<kLongMR, W>(a, lds, a.items[it]);
     }
 }
 
@@ -1001,10 +1080,17 @@ void *adj2_kernel_ptr(int L, int MR, bool rescue) {
 
 int adj_long_mr() { return kLongMR; }
 int adj_long_max_strips() { return kMaxLongStrips; }
+// waves per workgroup of the long-read kernel: 1 or adj_long_waves()
+int adj_long_waves() { return kLongWaves; }
+size_t adj_long_lds(uint32_t lds_stride, int K1, int waves) {
+    return (size_t)long_lds_bytes(lds_stride, K1, kLongMR, waves);
+}
 
-hipError_t launch_adj_long(int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a) {
+hipError_t launch_adj_long(int waves, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a) {
     void *args[] = {const_cast<AdjArgs *>(&a)};
-    return hipLaunchKernel(reinterpret_cast<void *>(&adj_long_kernel), dim3(grid), dim3(64), args, lds_bytes, st);
+    void *k = waves == 1 ? reinterpret_cast<void *>(&adj_long_kernel<1>)
+                         : reinterpret_cast<void *>(&adj_long_kernel<kLongWaves>);
+    return hipLaunchKernel(k, dim3(grid), dim3(64 * (waves == 1 ? 1 : kLongWaves)), args, lds_bytes, st);
 }
 
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a) {

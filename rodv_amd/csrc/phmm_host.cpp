@@ -55,7 +55,9 @@ void *adj2_kernel_ptr(int L, int MR, bool rescue);
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
 int adj_long_mr();
 int adj_long_max_strips();
-hipError_t launch_adj_long(int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
+int adj_long_waves();
+size_t adj_long_lds(uint32_t lds_stride, int K1, int waves);
+hipError_t launch_adj_long(int waves, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
 }  // namespace rodv
 
 using namespace rodv;
@@ -151,10 +153,18 @@ bool fail_alloc_hook() {
 
 // Polls of one hand-over wait of the long-read kernels: 2^20 (about a
 // second; the waits take microseconds to milliseconds).  Tests force the
-// bound to run out with RODV_PHMM_SPIN_LIMIT=0, read when a plan is created.
+// bound to run out with RODV_PHMM_SPIN_LIMIT=0, read when a plan is created
+// and when an adjoint runs.
 uint32_t spin_limit_hook() {
     const char *e = std::getenv("RODV_PHMM_SPIN_LIMIT");
     return e && e[0] ? (uint32_t)std::strtoul(e, nullptr, 10) : (1u << 20);
+}
+
+// Waves per workgroup of the long-read adjoint: tests compare the one-wave
+// kernel (RODV_PHMM_ADJ_LONG_WAVES=1) with the default, bit for bit.
+int adj_long_waves_hook() {
+    const char *e = std::getenv("RODV_PHMM_ADJ_LONG_WAVES");
+    return e && e[0] ? (int)std::strtol(e, nullptr, 10) : 0;
 }
 
 template <typename T>
@@ -2456,11 +2466,16 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     if (!rc) rc = dev_alloc(&d_grad, 10 * n);
     if (!rc) rc = dev_alloc(&d_flags, n);
     if (!rc) rc = dev_alloc(&d_resc, n);
-    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)nbk + 1);  // + the long-read kernel's
+    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)nbk + 2);  // + the long-read kernel's and its faults
     if (!rc) rc = dev_alloc(reinterpret_cast<double **>(&d_ck), (size_t)grid * ck_elems);
     // long reads: per block, every strip's last row but the last strip's,
-    // two rows of handed-up adjoints and the level-1 checkpoints of a strip
-    const int lgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(longs.size(), (uint64_t)ctx->num_cus * 2));
+    // waves + 1 rows of handed-up adjoints and per wave the level-1
+    // checkpoints of a strip.  A workgroup of adj_long_waves() waves takes
+    // the LDS of a CU; one wave (RODV_PHMM_ADJ_LONG_WAVES=1, for tests) two
+    // workgroups per CU.
+    const int lwaves = adj_long_waves_hook() == 1 ? 1 : adj_long_waves();
+    const int lgrid = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>(longs.size(), (uint64_t)ctx->num_cus * (lwaves == 1 ? 2 : 1)));
     // columns of a row: up to H + 2L + 4 read by the forward, + a checkpoint
     // segment (<= 256 steps) staged past the last step
     const uint64_t lcols = (uint64_t)max_h_long + 4 * 64 + 8 + 256;
@@ -2469,10 +2484,7 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     const uint64_t lnv = 3ull * lmr + 3;
     int lK1 = ckpt_stride;
     const uint32_t lstride = lds_stride_for(max_h_long, 64);
-    auto long_lds = [&](int K1) {
-        return (size_t)lstride + (size_t)((K1 + 7) / 8) * lnv * 64 * 8 + 3ull * K1 * 8 + 2ull * (K1 + 1) * 8 +
-               10ull * 64 * 8 + 4ull * adj_long_max_strips();
-    };
+    auto long_lds = [&](int K1) { return adj_long_lds(lstride, K1, lwaves); };
     while (lK1 > 4 && long_lds(lK1) > (size_t)maxlds) lK1 /= 2;
     lK1 = lK1 / 4 * 4;
     const uint64_t ld4max = ((uint64_t)max_h_long + 64 + 3) / 4 * 4;
@@ -2481,20 +2493,20 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         std::vector<int32_t> li(longs.size());
         for (size_t k = 0; k < longs.size(); k++) li[k] = longs[k].second;
         if (!rc) rc = upload(&d_litems, li.data(), li.size());
-        if (!rc) rc = dev_alloc(&d_lck, (size_t)lgrid * lck_elems);
+        if (!rc) rc = dev_alloc(&d_lck, (size_t)lgrid * lwaves * lck_elems);
         if (!rc) rc = dev_alloc(&d_lbnd, (size_t)lgrid * lbnd_stride);
-        if (!rc) rc = dev_alloc(&d_lab, (size_t)lgrid * 2 * lab_elems);
+        if (!rc) rc = dev_alloc(&d_lab, (size_t)lgrid * (lwaves + 1) * lab_elems);
         // rows are written before they are read; zeros keep the columns a
         // strip never writes finite
         if (!rc && (hipMemset(d_lbnd, 0, (size_t)lgrid * lbnd_stride * 8) != hipSuccess ||
-                    hipMemset(d_lab, 0, (size_t)lgrid * 2 * lab_elems * 8) != hipSuccess))
+                    hipMemset(d_lab, 0, (size_t)lgrid * (lwaves + 1) * lab_elems * 8) != hipSuccess))
             rc = RODV_PHMM_ERR_DEVICE;
     }
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (!rc && (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess))
         rc = RODV_PHMM_ERR_DEVICE;
-    if (!rc && launch_reset(d_cnt, 3 * (uint32_t)nbk + 1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && launch_reset(d_cnt, 3 * (uint32_t)nbk + 2, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     if (!rc && hipEventRecord(ev0, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     for (size_t bi = 0; bi < nbk && !rc; bi++) {
         const int L = shapes[bi].first, mr = shapes[bi].second, G = 64 / L;
@@ -2593,7 +2605,9 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         a.lrow = lrow;
         a.lab = d_lab;
         a.lab_elems = lab_elems;
-        if (launch_adj_long(lgrid, long_lds(lK1), st, a) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+        a.spin_limit = spin_limit_hook();
+        a.fault = d_cnt + 3 * nbk + 1;
+        if (launch_adj_long(lwaves, lgrid, long_lds(lK1), st, a) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     }
     std::vector<double> out(n), g(10 * n);
     if (!rc && hipEventRecord(ev1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
@@ -2609,6 +2623,10 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         rc = RODV_PHMM_ERR_DEVICE;
     if (!rc && hipMemcpy(g.data(), d_grad, 10 * n * 8, hipMemcpyDeviceToHost) != hipSuccess)
         rc = RODV_PHMM_ERR_DEVICE;
+    uint32_t lfaults = 0;  // hand-over waits of the long-read kernel that ran out
+    if (!rc && hipMemcpy(&lfaults, d_cnt + 3 * nbk + 1, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && lfaults) rc = RODV_PHMM_ERR_DEVICE;
     if (!rc) {
         for (int k = 0; k < 10; k++) grad[k] = 0.0;
         for (uint64_t k = 0; k < n; k++) {

@@ -222,13 +222,17 @@ struct AdjArgs {
     int ckpt;             // level-1 checkpoint stride in steps (multiple of 4)
     // reads longer than one strip (adj_long_kernel), per block: the last row
     // of every strip but the last (lrow doubles each, lbnd_stride per
-    // block), and two rows of the adjoints a strip hands to the strip above
-    // (lab_elems doubles each)
+    // block), and waves + 1 rows of the adjoints a strip hands to the strip
+    // above (lab_elems doubles each); ck_elems per wave
     void *lbnd;
     uint64_t lbnd_stride;
     uint64_t lrow;
     void *lab;
     uint64_t lab_elems;
+    // hand-over waits between the long-read kernel's waves: polls before a
+    // wait gives up, and the count of those that did
+    uint32_t spin_limit;
+    uint32_t *fault;
 };
 
 }  // namespace rodv

@@ -2,6 +2,7 @@
 differences of the FP64 forward (h = 1e-3), checkpoint-stride invariance and
 the torch.autograd wrapper."""
 import dataclasses
+import os
 
 import numpy as np
 import pytest
@@ -187,3 +188,51 @@ def test_adjoint_long_and_short_reads_together(engine, oracle_mod):
     o_out, o_g = oracle_mod.param_batch(b, THETA, weights=w, nthreads=8)
     assert np.max(np.abs(out - o_out)) <= 1e-4
     assert_grad_close(g, o_g)
+
+
+def _adjoint_env(engine, b, env, **kw):
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return engine.adjoint(b, THETA, **kw)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+def test_adjoint_long_reads_waves_same_bits(engine):
+    # the long-read kernel deals a pair's strips to the 4 waves of its
+    # workgroup, which hand adjoints up through published progress and add
+    # their sums in strip order: the bits of one wave taking the strips in
+    # sequence (the default kernel is checked against the oracle above).
+    # Reads of 2 (fewer strips than waves) to 30 strips of 252 rows, one
+    # and two reads per haplotype.
+    rng = np.random.default_rng(77)
+    from rodv_amd.batch import Batch
+    regs = [synth.region(rng, [R + 300, R + 90], [R, R - 7], unrelated=(1,) if R > 900 else (), qual=q)
+            for R, q in ((379, 30), (505, 12), (760, 25), (1009, 35))]
+    b = Batch.concat(regs + [long_pairs(6)])
+    w = rng.normal(size=b.num_pairs())
+    one = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 1}, weights=w)
+    four = engine.adjoint(b, THETA, weights=w)
+    np.testing.assert_array_equal(four[0].view(np.uint64), one[0].view(np.uint64))
+    np.testing.assert_array_equal(four[1].view(np.uint64), one[1].view(np.uint64))
+    assert np.all(np.isfinite(four[0]))
+
+
+def test_adjoint_long_wait_bound_is_an_error(engine):
+    # the waits between the waves are bounded: forced to run out from the
+    # host (no poll), every kernel still ends, the call reports a device
+    # error, and the next adjoint on the same context is exact again
+    from rodv_amd.engine import ERR_DEVICE, PhmmError
+    b = long_pairs(2)
+    with pytest.raises(PhmmError) as e:
+        _adjoint_env(engine, b, {"RODV_PHMM_SPIN_LIMIT": 0})
+    assert e.value.code == ERR_DEVICE
+    ref = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 1})
+    out, g = engine.adjoint(b, THETA)
+    np.testing.assert_array_equal(out, ref[0])
+    np.testing.assert_array_equal(g, ref[1])
