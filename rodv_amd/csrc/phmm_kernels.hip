@@ -1293,7 +1293,11 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
                 fd = ld_agent(in + 2);
             }
             if (!xlive && cr == 0) fm = T(__builtin_nan(""));
-            for (; d <= dlim; d++) {
+            // one step; two per iteration, so that the registers of the
+            // state and the priors alternate instead of being copied back
+            // at the end of every step (the loop holds convergent readlanes,
+            // which the compiler does not unroll with a remainder)
+            auto one_step = [&](const int d) {
                 if constexpr (TAB) {
                     const uint32_t c2 = ld_code(cp + d + 2);
                     T pb[MR];
@@ -1330,7 +1334,12 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
                         S.D[MR - 1] = bd;
                     }
                 }
+            };
+            for (; d + 1 <= dlim; d += 2) {
+                one_step(d);
+                one_step(d + 1);
             }
+            if (d <= dlim) one_step(d++);
             if (t == st0 + nb - 1) {
                 if constexpr (!RESCUE) {
                     if (detect && !last && cr == nu - 1) {
