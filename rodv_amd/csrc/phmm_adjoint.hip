@@ -811,7 +811,9 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
                     fi = ldexp(ld_ag(in + 1), sh);
                     fd = ldexp(ld_ag(in + 2), sh);
                 }
-                for (int d = d0; d <= dlim; d++) {
+                // one step; two per iteration, so that the state's registers
+                // alternate instead of being copied back after every step
+                auto one_step = [&](const int d) {
                     step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
                     if (last && d - l <= H) {
                         sm += S.M[MR - 1];
@@ -833,7 +835,13 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
                             S.D[MR - 1] = bd;
                         }
                     }
+                };
+                int d = d0;
+                for (; d + 1 <= dlim; d += 2) {
+                    one_step(d);
+                    one_step(d + 1);
                 }
+                if (d <= dlim) one_step(d);
             }
             if (!last) {
                 __threadfence();  // the row's stores before the next strip's loads
@@ -919,10 +927,11 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
                 const int d0s = k * K1;
                 stage_feeder(d0s);
                 const int n = min(K1, D4 - d0s);
-                for (int u = 1; u <= n; u++) {
+                for (int u = 1; u <= n; u += 2) {  // n is a multiple of 4
                     fstep(S, d0s + u, d0s);
-                    if (u == K1 && d0s + u < D4) put_state(ck, NV, k + 1, S);
+                    fstep(S, d0s + u + 1, d0s);
                 }
+                if (n == K1 && d0s + K1 < D4) put_state(ck, NV, k + 1, S);
             }
         }
         T Tb[MR], Ib[MR], Db[MR], gd[MR], gdT[MR], gg[MR], ge[MR];
