@@ -30,12 +30,13 @@ void *fwd_tab_kernel_ptr(int L, int MR, bool inter, bool share);
 void *fwd_chain_kernel_ptr(int L, int MR, bool inter, int S, bool share);
 uint32_t fwd_tab_bytes(int L, int MR);
 int fwd_f64_rescue_mr();
+int fwd_long_waves();
 int rescue_long_waves();
 int rescue_long_rows();
 int rescue_long_mr();
-hipError_t launch_rescue_long(bool route, bool tab, int grid, size_t lds_bytes, hipStream_t st,
+hipError_t launch_rescue_long(bool route, bool tab, int waves, int grid, size_t lds_bytes, hipStream_t st,
                               const FwdArgs &a);
-uint32_t long_tab_bytes(int MR, bool fp64);
+uint32_t long_tab_bytes(int MR, bool fp64, int waves);
 void *fwd_long_kernel_ptr(int MR, bool inter, bool tab);
 hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
 hipError_t launch_fwd(void *kern, int grid, size_t lds_bytes, hipStream_t st, const FwdArgs &a);
@@ -158,6 +159,14 @@ bool fail_alloc_hook() {
 uint32_t spin_limit_hook() {
     const char *e = std::getenv("RODV_PHMM_SPIN_LIMIT");
     return e && e[0] ? (uint32_t)std::strtoul(e, nullptr, 10) : (1u << 20);
+}
+
+// Waves per workgroup of the FP64 long-read rescue: tests compare the
+// 8-wave units (RODV_PHMM_RESCUE_LONG_WAVES=8) with the default 4, bit for
+// bit; read when a plan is created.
+int rescue_long_waves_hook() {
+    const char *e = std::getenv("RODV_PHMM_RESCUE_LONG_WAVES");
+    return e && e[0] ? (int)std::strtol(e, nullptr, 10) : 0;
 }
 
 // Waves per workgroup of the long-read adjoint: tests compare the one-wave
@@ -353,6 +362,7 @@ struct rodv_phmm_plan {
     uint64_t bnd64_elems = 0;
     int rescue_grid = 0;
     int long_grid = 0;          // workgroups of fwd_rescue_long_kernel (0: no long reads)
+    int rescue64_waves = 4;     // their waves (rescue_long_waves(), or fwd_long_waves() for tests)
     bool share_all = false;     // every read qualifies for shared products: the
                                 // FP64 rescue table kernels use them
     bool long_tab = false;      // every haplotype N-free: the long-read kernels
@@ -823,15 +833,14 @@ int strips_for(uint32_t R, int L, int MR) {
     return ((int)R - 1 + off) / C + 1;
 }
 
-// A pair's units for the long-read kernels: runs of at most
-// rescue_long_waves() strips (one per wave) from strip s_begin, each but the
-// last handing its boundary row to the next through a new slot; the first
-// reads the row above strip s_begin from slot xin (kNoSlot: none).  False if
-// the strip count does not fit the units' 16-bit fields.
-bool append_units(std::vector<LongUnit> &units, uint32_t &nslots, int32_t pair, int nstr,
+// A pair's units for the long-read kernels: runs of at most W strips (one
+// per wave of a W-wave workgroup) from strip s_begin, each but the last
+// handing its boundary row to the next through a new slot; the first reads
+// the row above strip s_begin from slot xin (kNoSlot: none).  False if the
+// strip count does not fit the units' 16-bit fields.
+bool append_units(std::vector<LongUnit> &units, uint32_t &nslots, int32_t pair, int nstr, int W,
                   int s_begin = 0, uint32_t xin = kNoSlot) {
     if (nstr < 1 || nstr > 0xffff) return false;
-    const int W = rescue_long_waves();
     for (int s0 = s_begin; s0 < nstr; s0 += W) {
         const int s1 = std::min(nstr, s0 + W);
         LongUnit u;
@@ -1208,7 +1217,19 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     p->cfg = cfg;
     p->n_pairs = n_pairs;
     p->share_all = share_all;
-    p->long_tab = std::none_of(hap_has_n.begin(), hap_has_n.end(), [](uint8_t v) { return v != 0; });
+    p->rescue64_waves = rescue_long_waves_hook() == fwd_long_waves() ? fwd_long_waves() : rescue_long_waves();
+    {
+        // the long-read kernels' LDS prior tables: every haplotype N-free,
+        // and the codes of the longest haplotype and the tables fit
+        uint32_t mh = 0;
+        for (uint32_t h = 0; h < M; h++) mh = std::max(mh, b->hap_lengths[h]);
+        int maxlds = 65536;
+        (void)hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device);
+        const uint32_t tabs = std::max(long_tab_bytes(16, false, fwd_long_waves()),
+                                       long_tab_bytes(rescue_long_mr(), true, p->rescue64_waves));
+        p->long_tab = std::none_of(hap_has_n.begin(), hap_has_n.end(), [](uint8_t v) { return v != 0; }) &&
+                      (uint64_t)((lds_stride_for(mh, 64) + 255u) & ~255u) + tabs <= (uint64_t)maxlds;
+    }
     p->arena = arena;
     p->no_plain = no_plain;
     if (arena) arena->used = arena->need = 0;
@@ -1420,9 +1441,9 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
             if (lk) {
                 bk.kern = lk;
                 // the waves' prior tables (covers the sub-strip launch's too)
-                bk.tab_bytes = p->long_tab ? long_tab_bytes(bk.MR, false) : 0u;
-                bk.block = 64 * rescue_long_waves();
-                bk.nbnd = (uint32_t)rescue_long_waves() + 1;
+                bk.tab_bytes = p->long_tab ? long_tab_bytes(bk.MR, false, fwd_long_waves()) : 0u;
+                bk.block = 64 * fwd_long_waves();
+                bk.nbnd = (uint32_t)fwd_long_waves() + 1;
                 bk.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bk.n_items, (uint64_t)ctx->num_cus));
                 bk.grid_cap = bk.grid;
                 // batched: strip 0 of every pair first (units1), then the
@@ -1449,7 +1470,7 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
                     u1.xin = kNoSlot;
                     u1.xout = nstr > 1 ? bk.n_xslots++ : kNoSlot;
                     bk.units1.push_back(u1);
-                    if (nstr > 1 && !append_units(bk.units, bk.n_xslots, q, nstr, 1, u1.xout))
+                    if (nstr > 1 && !append_units(bk.units, bk.n_xslots, q, nstr, fwd_long_waves(), 1, u1.xout))
                         return RODV_PHMM_ERR_INVALID_ARG;
                 }
                 bk.grid = (int)std::max<uint64_t>(
@@ -1613,9 +1634,9 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
     for (uint32_t h = 0; h < M; h++) max_h_all = std::max(max_h_all, b->hap_lengths[h]);
     p->rescue_lds = lds_stride_for(max_h_all, 64);
     if (max_long_h) {
-        // long reads: workgroups of rescue_long_waves() (8) waves, one per CU
-        // (2 waves per SIMD; the kernel's VGPRs allow 3, so a second would
-        // not fit), each with rescue_long_waves() + 1 boundary-row buffers.
+        // long reads: workgroups of rescue64_waves waves (4: two per CU; 8:
+        // one), 2 waves per SIMD (the kernel's VGPRs allow no more), each
+        // with rescue64_waves + 1 boundary-row buffers.
         // Units of every pair that may need them, longest pipeline first:
         // blocks of a strip plus the two-block lag of each further strip
         std::vector<std::pair<uint64_t, uint32_t>> lp;
@@ -1632,12 +1653,15 @@ static int plan_create_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *b,
         });
         for (const auto &e : lp)
             if (!rc && !append_units(p->long_units, p->n_x64, (int32_t)e.second,
-                                     strips_for(b->read_lengths[pr[e.second]], 64, rescue_long_mr())))
+                                     strips_for(b->read_lengths[pr[e.second]], 64, rescue_long_mr()),
+                                     p->rescue64_waves))
                 rc = RODV_PHMM_ERR_INVALID_ARG;
         p->prog64_base = p->n_counters;
         p->n_counters += p->n_x64;
-        p->long_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>(p->long_units.size(), (uint64_t)ctx->num_cus));
-        const size_t nring = (size_t)p->long_grid * (size_t)(rescue_long_waves() + 1);
+        p->long_grid = (int)std::max<uint64_t>(
+            1, std::min<uint64_t>(p->long_units.size(),
+                                  (uint64_t)ctx->num_cus * (uint64_t)(fwd_long_waves() / p->rescue64_waves)));
+        const size_t nring = (size_t)p->long_grid * (size_t)(p->rescue64_waves + 1);
         p->bnd64_elems = 3ull * (max_long_h + 3ull * 64 + 2);
         if (!rc) rc = plan_alloc(p, &p->d_bnd64, (nring + p->n_x64) * p->bnd64_elems);
         if (!rc) rc = plan_fill(p, p->d_bnd64, 0, nring * p->bnd64_elems * sizeof(double));
@@ -1878,9 +1902,10 @@ int enqueue(rodv_phmm_plan *p, hipStream_t st, int mode, hipEvent_t *ev) {
             a.prog = p->d_counters + p->prog64_base;
             a.fault = p->d_counters + p->fault_counter;
             a.spin_limit = p->spin_limit;
-            a.xbnd = p->d_bnd64 + (size_t)p->long_grid * (rescue_long_waves() + 1) * p->bnd64_elems;
-            const size_t llds = (size_t)p->rescue_lds + (p->long_tab ? long_tab_bytes(rescue_long_mr(), true) : 0u);
-            HIPCHK(launch_rescue_long(p->any_inter, p->long_tab, p->long_grid, llds, st, a));
+            a.xbnd = p->d_bnd64 + (size_t)p->long_grid * (p->rescue64_waves + 1) * p->bnd64_elems;
+            const size_t llds = (size_t)p->rescue_lds +
+                                (p->long_tab ? long_tab_bytes(rescue_long_mr(), true, p->rescue64_waves) : 0u);
+            HIPCHK(launch_rescue_long(p->any_inter, p->long_tab, p->rescue64_waves, p->long_grid, llds, st, a));
             launches++;
         }
     }

@@ -1083,6 +1083,9 @@ __device__ __forceinline__ void rescue_pair(const FwdArgs &a, uint8_t *lds, int 
 // strip and from there into the pair's result, and the wait is not
 // repeated.
 constexpr int kLW = 8;   // waves per workgroup (2 per SIMD; 4 was 1.5x slower)
+// waves per workgroup of the FP64 long-read rescue: units of 4 strips, two
+// workgroups per CU (a pair's last unit leaves fewer waves idle)
+constexpr int kRescueLW = 4;
 // rows per lane of the FP64 long-read rescue (strips of 63 * 7 = 441 rows)
 constexpr int kLongRescueMR = 7;
 constexpr int kLB = 64;  // steps per block (>= L - 1)
@@ -1140,7 +1143,7 @@ __device__ __forceinline__ void long_priors(T (&pr)[MR], const uint8_t *p) {
 // the wave's table in LDS (one 16-byte read per chunk, the next column's
 // issued a step ahead) instead of selecting them per cell from the one-hot
 // code (2 VALU instructions per cell in FP32, 4 in FP64).
-template <typename T, int MR, bool INTER, bool RESCUE, bool TAB>
+template <typename T, int MR, bool INTER, bool RESCUE, bool TAB, int W>
 __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1, uint32_t xin,
                           uint32_t xout) {
     constexpr int L = 64;
@@ -1167,13 +1170,13 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
 
     // one-hot codes of the haplotype, shared by the waves: column j at
     // lds[L - 1 + j] (TAB: the base's block 0..3 instead)
-    for (uint32_t k = threadIdx.x * 16u; k < a.lds_stride; k += 64u * kLW * 16u)
+    for (uint32_t k = threadIdx.x * 16u; k < a.lds_stride; k += 64u * W * 16u)
         *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
     __syncthreads();
     {
         const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
         const int nch = (H + 15) >> 4;
-        for (int c = threadIdx.x; c < nch; c += 64 * kLW) {
+        for (int c = threadIdx.x; c < nch; c += 64 * W) {
             uint4 v = src[c];
             if constexpr (TAB) {  // one-hot 1, 2, 4, 8 -> 0..3 per byte
                 v.x = ((v.x >> 1) & 0x7f7f7f7fu) - ((v.x >> 3) & 0x1f1f1f1fu);
@@ -1194,9 +1197,9 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
     if (s1 > nstr) s1 = nstr;
     const int nu = s1 - s0;  // strips of this unit
     const int nb = (H + 2 * L - 1 + kLB - 1) / kLB;  // blocks of a strip (steps 1 .. H + 2L - 1)
-    const int P0 = nb > 2 * kLW ? nb : 2 * kLW;
-    const int nblk = ((nu - 1) / kLW) * P0 + 2 * ((nu - 1) % kLW) + nb;
-    T *bbase = reinterpret_cast<T *>(a.bnd) + (uint64_t)blockIdx.x * (kLW + 1) * a.bnd_elems;
+    const int P0 = nb > 2 * W ? nb : 2 * W;
+    const int nblk = ((nu - 1) / W) * P0 + 2 * ((nu - 1) % W) + nb;
+    T *bbase = reinterpret_cast<T *>(a.bnd) + (uint64_t)blockIdx.x * (W + 1) * a.bnd_elems;
     T *xb_in = xin != kNoSlot ? reinterpret_cast<T *>(a.xbnd) + (uint64_t)xin * a.bnd_elems : nullptr;
     T *xb_out = xout != kNoSlot ? reinterpret_cast<T *>(a.xbnd) + (uint64_t)xout * a.bnd_elems : nullptr;
     bool xlive = true;  // the waits for the unit above have not run out
@@ -1216,7 +1219,7 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
     const bool detect = !RESCUE && a.dead_mode == 1;
     uint32_t orbits = 0;
     for (int t = 0; t < nblk; t++) {
-        const int st0 = (cr / kLW) * P0 + 2 * (cr % kLW);
+        const int st0 = (cr / W) * P0 + 2 * (cr % W);
         bool pub = false;  // this block of the unit's last strip goes to slot xout
         if (cr < nu && t >= st0 && t < st0 + nb) {
             const int cs = s0 + cr;
@@ -1262,9 +1265,9 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
                 last = cs == nstr - 1;
                 inject = cs > 0;
                 store = !last;
-                bnd_out = cr == nu - 1 ? xb_out : bbase + (uint64_t)(cr % (kLW + 1)) * a.bnd_elems;
+                bnd_out = cr == nu - 1 ? xb_out : bbase + (uint64_t)(cr % (W + 1)) * a.bnd_elems;
                 bnd_in = cr == 0 ? xb_in  // strip cs - 1's
-                                 : bbase + (uint64_t)((cr + kLW) % (kLW + 1)) * a.bnd_elems;
+                                 : bbase + (uint64_t)((cr + W) % (W + 1)) * a.bnd_elems;
                 if (inject) {
                     T bm = ld_agent(bnd_in + 3 * (1 + L)), bi = ld_agent(bnd_in + 3 * (1 + L) + 1),
                       bd = ld_agent(bnd_in + 3 * (1 + L) + 2);
@@ -1375,7 +1378,7 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
                         a.flags[p] = fl;
                     }
                 }
-                cr += kLW;
+                cr += W;
             }
             pub = bnd_out == xb_out && store && xb_out;
         }
@@ -1400,8 +1403,8 @@ __device__ void long_pair(const FwdArgs &a, uint8_t *lds, int p, int s0, int s1,
 // leaves them), one unit per workgroup at a time; a unit whose pair the
 // FP32 pass did not reject is skipped, like every other unit of its pair.
 // ROUTE: per pair inter or intra FP64 as the regroup decided.
-template <bool ROUTE, bool TAB>
-__global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
+template <bool ROUTE, bool TAB, int W>
+__global__ __launch_bounds__(64 * W, kLW / W) void fwd_rescue_long_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t s_it;
     const uint32_t n = a.n_items;
@@ -1416,9 +1419,9 @@ __global__ __launch_bounds__(64 * kLW) void fwd_rescue_long_kernel(FwdArgs a) {
         if (!(fl & 0x01u)) continue;  // RESCUED
         const int s0 = (int)(u.strips & 0xffffu), s1 = (int)(u.strips >> 16);
         if (__builtin_amdgcn_readfirstlane((ROUTE && (fl & kFlagGrouped64)) ? 1 : 0))
-            long_pair<double, kLongRescueMR, true, true, TAB>(a, lds, u.pair, s0, s1, u.xin, u.xout);
+            long_pair<double, kLongRescueMR, true, true, TAB, W>(a, lds, u.pair, s0, s1, u.xin, u.xout);
         else
-            long_pair<double, kLongRescueMR, false, true, TAB>(a, lds, u.pair, s0, s1, u.xin, u.xout);
+            long_pair<double, kLongRescueMR, false, true, TAB, W>(a, lds, u.pair, s0, s1, u.xin, u.xout);
     }
 }
 
@@ -1453,7 +1456,7 @@ __global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
         } else {
             p = a.items[it];
         }
-        if (p >= 0) long_pair<float, MR, INTER, false, TAB>(a, lds, p, s0, s1, xin, xout);
+        if (p >= 0) long_pair<float, MR, INTER, false, TAB, kLW>(a, lds, p, s0, s1, xin, xout);
     }
 }
 
@@ -2294,8 +2297,10 @@ void *fwd_f64_rescue_kernel(bool route) {
                  : reinterpret_cast<void *>(&fwd_rescue_kernel);
 }
 
-// FP64 rescue of the reads longer than one strip (workgroups of kLW waves).
-int rescue_long_waves() { return kLW; }
+// Waves per workgroup of the FP32 long-read kernel, and those the FP64
+// rescue of the reads longer than one strip may use (kLW or kRescueLW).
+int fwd_long_waves() { return kLW; }
+int rescue_long_waves() { return kRescueLW; }
 // tab: the LDS prior-table variant (every haplotype of the plan N-free)
 void *fwd_long_kernel_ptr(int MR, bool inter, bool tab) {
 #define LK(m)                                                                                \
@@ -2311,10 +2316,11 @@ void *fwd_long_kernel_ptr(int MR, bool inter, bool tab) {
     }
 #undef LK
 }
-// LDS of the prior tables of a long-read workgroup (added to its code buffer)
-uint32_t long_tab_bytes(int MR, bool fp64) {
+// LDS of the prior tables of a long-read workgroup of `waves` waves (added
+// to its code buffer)
+uint32_t long_tab_bytes(int MR, bool fp64, int waves) {
     const uint32_t e = fp64 ? 2u : 4u;
-    return (uint32_t)kLW * 4u * 64u * (((uint32_t)MR + e - 1) / e) * 16u;
+    return (uint32_t)waves * 4u * 64u * (((uint32_t)MR + e - 1) / e) * 16u;
 }
 hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, hipStream_t st, const FwdArgs &a) {
     void *args[] = {const_cast<FwdArgs *>(&a)};
@@ -2322,14 +2328,19 @@ hipError_t launch_fwd_block(void *kern, int grid, int block, size_t lds_bytes, h
 }
 int rescue_long_rows() { return kRescueStrip; }
 int rescue_long_mr() { return kLongRescueMR; }
-hipError_t launch_rescue_long(bool route, bool tab, int grid, size_t lds_bytes, hipStream_t st,
+template <int W>
+void *rescue_long_kernel(bool route, bool tab) {
+    return tab ? (route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true, true, W>)
+                        : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false, true, W>))
+               : (route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true, false, W>)
+                        : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false, false, W>));
+}
+// waves: kLW or kRescueLW
+hipError_t launch_rescue_long(bool route, bool tab, int waves, int grid, size_t lds_bytes, hipStream_t st,
                               const FwdArgs &a) {
-    void *kern = tab ? (route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true, true>)
-                              : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false, true>))
-                     : (route ? reinterpret_cast<void *>(&fwd_rescue_long_kernel<true, false>)
-                              : reinterpret_cast<void *>(&fwd_rescue_long_kernel<false, false>));
+    void *kern = waves == kRescueLW ? rescue_long_kernel<kRescueLW>(route, tab) : rescue_long_kernel<kLW>(route, tab);
     void *args[] = {const_cast<FwdArgs *>(&a)};
-    return hipLaunchKernel(kern, dim3(grid), dim3(64 * kLW), args, lds_bytes, st);
+    return hipLaunchKernel(kern, dim3(grid), dim3(64 * waves), args, lds_bytes, st);
 }
 
 // FP64 rescue in classes: the binning pass, and the table pass over classes

@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-4  # |d log10 L| bound of this path
 FP64_STRIP = 63 * 7  # rows of an FP64 strip of the long-read rescue (7 rows per lane)
-UNIT = 8 * FP64_STRIP  # rows of a unit of 8 strips
+UNIT = 4 * FP64_STRIP  # rows of an FP64 unit of 4 strips
+UNIT8 = 8 * FP64_STRIP  # with 8-wave units (RODV_PHMM_RESCUE_LONG_WAVES=8)
 
 
 def run(engine, batch, **cfg):
@@ -285,7 +286,7 @@ def test_long_read_wait_bound_is_an_error(engine, oracle_mod):
     # follows another proceeds at once, with the NaN-filled hand-over rows of
     # a fresh plan), the plan's read reports a device error after copying the
     # results, the pairs whose FP64 rescue runs in several units (reads of
-    # more than 3,528 rows: 8 strips of 441) are NaN and the others are not,
+    # more than 1,764 rows: 4 strips of 441) are NaN and the others are not,
     # and the same context scores normally afterwards.
     from rodv_amd.engine import ERR_DEVICE, _ptr
     rng = np.random.default_rng(31)
@@ -313,16 +314,16 @@ def test_long_read_wait_bound_is_an_error(engine, oracle_mod):
 
 def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
     # Reads longer than one strip: batched launches split each pair into
-    # units of at most 8 strips that run on several workgroups at once and
-    # hand boundary rows over through HBM slots (NaN until written); plain
-    # launches run every pair whole in one workgroup.  Read lengths sit at
-    # the unit boundaries: FP64 strips of 441 rows (3,528 / 3,529 / 7,056 /
-    # 7,057 rows: 8 / 9 / 16 / 17 strips), FP32 strips of 1,008 rows at 16
-    # rows per lane (8,064 / 8,065).  Reads are taken from haplotype 0 with
-    # rare substitutions (FP32 results), and are unrelated to haplotype 1
-    # (rescued in FP64).
+    # units (8 strips in FP32, 4 in FP64) that run on several workgroups at
+    # once and hand boundary rows over through HBM slots (NaN until
+    # written); plain launches run every FP32 pair whole in one workgroup.
+    # Read lengths sit at the unit boundaries: FP64 strips of 441 rows
+    # (1,764 / 1,765 / 3,528 / 3,529 / 7,056 / 7,057 rows: 4 / 5 / 8 / 9 / 16
+    # / 17 strips), FP32 strips of 1,008 rows at 16 rows per lane (8,064 /
+    # 8,065).  Reads are taken from haplotype 0 with rare substitutions (FP32
+    # results), and are unrelated to haplotype 1 (rescued in FP64).
     rng = np.random.default_rng(23)
-    lens = [379, 1100, UNIT, UNIT + 1, 2 * UNIT, 2 * UNIT + 1, 8064, 8065]
+    lens = [379, 1100, UNIT, UNIT + 1, UNIT8, UNIT8 + 1, 2 * UNIT8, 2 * UNIT8 + 1, 8064, 8065]
     acgt = np.frombuffer(b"ACGT", np.uint8)
     h0 = acgt[rng.integers(0, 4, 8400)]
     h1 = acgt[rng.integers(0, 4, 3000)]
@@ -354,6 +355,39 @@ def test_long_read_units_match_whole_pairs_and_oracle(engine, oracle_mod):
         np.testing.assert_array_equal(got[1], ref[1])
         np.testing.assert_array_equal(got[0], ref[0])
     plan.close()
+
+
+def test_long_read_fp64_units_of_4_and_8_strips(engine):
+    # the FP64 long-read rescue in units of 4 strips (two 4-wave workgroups
+    # per CU, the default) and of 8 (one 8-wave workgroup per CU): the same
+    # strips with the same boundary rows, so the same bits; reads of 2 to 17
+    # FP64 strips, all rescued (unrelated to the haplotype), with and without
+    # the LDS prior tables (a haplotype with N)
+    rng = np.random.default_rng(37)
+    lens = [500, UNIT, UNIT + 1, 2 * UNIT + 300, UNIT8 + 1, 2 * UNIT8 + 1]
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    for with_n in (False, True):
+        hap = acgt[rng.integers(0, 4, 2500)].copy()
+        if with_n:
+            hap[rng.choice(2500, 5, replace=False)] = ord("N")
+        reads = [acgt[rng.integers(0, 4, n)].tobytes() for n in lens]
+        q = [rng.integers(20, 41, n) for n in lens]
+        gcp = [np.full(n, 10) for n in lens]
+        b = Batch.from_lists([hap.tobytes()], reads, q, q, q, gcp)
+        got = {}
+        for w in ("4", "8"):
+            os.environ["RODV_PHMM_RESCUE_LONG_WAVES"] = w
+            try:
+                plan = engine.plan(b, default_config(semantics=0))
+            finally:
+                del os.environ["RODV_PHMM_RESCUE_LONG_WAVES"]
+            plan.execute()
+            got[w] = plan.read(raw=True)
+            plan.close()
+        assert (got["4"][1] & FLAG_RESCUED).all()
+        np.testing.assert_array_equal(got["4"][3].view(np.uint64), got["8"][3].view(np.uint64))
+        np.testing.assert_array_equal(got["4"][1], got["8"][1])
+        np.testing.assert_array_equal(got["4"][0].view(np.uint64), got["8"][0].view(np.uint64))
 
 
 def test_long_reads_haplotype_with_n(engine, oracle_mod):

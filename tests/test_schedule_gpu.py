@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-4
 ROUTE_INTER32, ROUTE_INTER64 = 1, 2
 FP64_STRIP = 63 * 7  # rows of an FP64 strip of the long-read rescue (7 rows per lane)
-UNIT = 8 * FP64_STRIP  # rows of a unit of 8 strips
+UNIT = 4 * FP64_STRIP  # rows of an FP64 unit of 4 strips
 
 
 def check_schedule(engine, oracle_mod, batch, wf=16, wd=8, idle_f=0.1, idle_d=0.1,
@@ -216,7 +216,7 @@ def test_long_reads_in_inter_fp64_groups(engine, oracle_mod):
     # Reads longer than the one-wave FP64 rescue (378 rows), unrelated to their
     # haplotypes: the FP32 pass rejects them, the regroup puts windows of
     # them in inter FP64 groups, and the long-read rescue runs them with the
-    # inter arithmetic as units of at most 8 strips (reads over 3,528 rows
+    # inter arithmetic as units of at most 4 strips (reads over 1,764 rows
     # take more than one unit); batched and plain launches alike.
     rng = np.random.default_rng(29)
     regions = [
