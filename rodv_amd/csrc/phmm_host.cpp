@@ -2587,7 +2587,11 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
             a.counter = d_cnt + 3 * bi + 1 + pass;
             a.n_items_dev = pass == 0 ? nullptr : a.rescue_count;
             const size_t lds = codes + (size_t)((K1 + 7) / 8) * nv * 64 * esz;
-            if (launch_adj(adj2_kernel_ptr(L, mr, pass == 1), grid, lds, st, a) != hipSuccess)
+            // the FP64 instance holds one wave per SIMD (up to 512
+            // registers): four workgroups per CU are all that run at once,
+            // and the pass over an empty list costs a dispatch of the grid
+            const int g = pass == 0 ? grid : std::min(grid, ctx->num_cus * 4);
+            if (launch_adj(adj2_kernel_ptr(L, mr, pass == 1), g, lds, st, a) != hipSuccess)
                 rc = RODV_PHMM_ERR_DEVICE;
         }
     }
