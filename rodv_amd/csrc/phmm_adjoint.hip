@@ -587,12 +587,12 @@ __global__ __launch_bounds__(64, RESCUE ? 1 : 2) void adj2_kernel(AdjArgs a) {
 
 // ------------------------------------------------------------ long reads
 // Reads longer than one strip of 63 * MR rows, in FP64 (their FP32 sums
-// underflow), one pair per wave, strips in sequence:
+// underflow), one pair per workgroup, its strips over the waves:
 //  A. forward over every strip, storing each strip's last row in HBM; the
-//     row a strip hands to the next is scaled by a power of two that takes
-//     its largest entry to 2^kLongTop (the shift is kept per strip), so the
-//     scaled entries stay in range however long the read is (as the
-//     extended-range forward does);
+//     row a strip hands to the next is scaled by a power of two (kept per
+//     strip) that takes the largest entry of the row four strips back, in
+//     the same scale, to 2^kLongTop, so the scaled entries stay in range
+//     however long the read is (as the extended-range forward does);
 //  B. strips last to first: recompute the strip's forward from its input
 //     row with the two levels of checkpoints of the one-strip kernel, then
 //     its reverse sweep.  Below lane 62's last row lies the next strip's
@@ -670,7 +670,9 @@ __device__ __forceinline__ void long_setup(const AdjArgs &a, int cs, int nstr, i
     A.nb_xx = wrap ? 0.0 : (below ? a.gamma : xx);
 }
 
-// Waves of a long-read workgroup: one per SIMD.  Pass A runs in wave 0.
+// Waves of a long-read workgroup: one per SIMD.  Pass A deals the strips to
+// the waves in order (wave w takes strips w, w + W, ...), each strip's
+// feeder waiting for the strip above to publish the columns of each block.
 // Pass B deals the strips to the waves, last first: wave w takes strips
 // nstr-1-w, nstr-1-w-W, ...  A strip's reverse reads the adjoints the strip
 // below hands up, which that strip writes column by column as its own
@@ -688,16 +690,33 @@ __device__ __forceinline__ void long_setup(const AdjArgs &a, int cs, int nstr, i
 // become NaN and its wave waits no more.
 constexpr int kLongWaves = 4;
 constexpr int kProgNone = 0x7fffffff;
+constexpr int kNoMax = -0x40000000;  // a strip's last row is all zero
+// Pass A's shifts: strip cs's input row (the last row of strip cs - 1) is
+// scaled by 2^shs[cs], chosen from the largest entry of strip
+// cs - kShiftLag's last row as the shifts since bring it to strip cs - 1's
+// scale: 2^kLongTop if the three strips between change nothing.  Strip
+// cs - kShiftLag ran on the same wave (kShiftLag a multiple of the waves),
+// so its row is complete when strip cs starts, and 4 strips run at once
+// instead of one after another.  A row's total over columns and states grows
+// by less than 1 % over a strip (the transitions out of a state sum to at
+// most 1, emissions are at most 1) and lies between the row's largest entry
+// and 3H times it, so scaled entries stay below 2^(kLongTop + 2 log2(3H) +
+// 1) < 2^1023; below, they sit about three strips' decay under
+// 2^kLongTop.  Powers of two scale exactly, so the results are those of any
+// other choice of shifts except in subnormal entries.
+constexpr int kShiftLag = 4;
+constexpr int kSafeLog = -500;  // below: pass A again with a shift per strip
 
 // LDS of a long-read workgroup: hap codes (lds_stride bytes) | per wave:
 // level-2 states, feeder inputs of a segment, adjoints from below of a
 // segment | per-lane gradient totals (10 x 64) | scaled sum | the strips'
-// shifts | the strips' progress | the item taken
+// shifts | the strips' reverse progress | their forward progress | the
+// exponents of their last rows' largest entries | the item taken
 __host__ __device__ constexpr uint64_t long_wave_elems(int K1, int MR) {
     return (uint64_t)((K1 + 7) / 8) * (3 * MR + 3) * kL + 3ull * K1 + 2ull * (K1 + 1);
 }
 __host__ __device__ constexpr uint64_t long_lds_bytes(uint32_t lds_stride, int K1, int MR, int W) {
-    return lds_stride + ((uint64_t)W * long_wave_elems(K1, MR) + 10ull * kL + 1) * 8 + 8ull * kMaxLongStrips + 16;
+    return lds_stride + ((uint64_t)W * long_wave_elems(K1, MR) + 10ull * kL + 1) * 8 + 16ull * kMaxLongStrips + 16;
 }
 
 // Waits until the published progress *c is <= need; false if `limit` polls
@@ -707,6 +726,19 @@ __device__ __forceinline__ bool wait_progress(const int *c, int need, uint32_t l
         const int v = uni(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
         if (v <= need) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the handed-up rows after the count
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(16);
+    }
+    return false;
+}
+// Waits until the published progress *c is >= need (pass A's forward
+// progress); as wait_progress.
+__device__ __forceinline__ bool wait_progress_ge(const int *c, int need, uint32_t limit) {
+    for (uint32_t k = 0; k < limit; k++) {
+        const int v = uni(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        if (v >= need) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the row's stores after the count
             return true;
         }
         __builtin_amdgcn_s_sleep(16);
@@ -750,7 +782,13 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     T *ssum = tot + 10 * kL;
     int *shs = reinterpret_cast<int *>(ssum + 1);
     int *prog = shs + kMaxLongStrips;
+    int *progA = prog + kMaxLongStrips;
+    int *mlog = progA + kMaxLongStrips;
     __syncthreads();  // the previous item's LDS reads are done
+    for (int k = threadIdx.x; k < kMaxLongStrips; k += kL * W) {
+        progA[k] = -1;
+        mlog[k] = kNoMax;
+    }
     for (uint32_t k = threadIdx.x * 16; k < a.lds_stride; k += 64 * W * 16)
         *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
     {
@@ -772,90 +810,134 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     T *AB = reinterpret_cast<T *>(a.lab) + (uint64_t)blockIdx.x * (W + 1) * a.lab_elems;
     T *ck = reinterpret_cast<T *>(a.ck) + ((uint64_t)blockIdx.x * W + w) * a.ck_elems;
 
-    // ---- A: forward over the strips (wave 0)
-    int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
-    if (w == 0) {
-        T sm = T(0), si = T(0);
-        if (lane == 0) shs[0] = 0;
-        for (int cs = 0; cs < nstr; cs++) {
-            Priors<T, MR> P;
-            AdjRows<T, MR> A;
-            State<T, MR> S;
-            long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S);
-            wave_sync();
-            const bool last = cs == nstr - 1;
-            const int sh = shs[cs];
-            const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
-            T *bout = B + (uint64_t)cs * a.lrow;
-            if (cs > 0) {  // column 1 of the row above (column j at 3 * (j + L))
-                const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
-                        bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
-                if (l == kL - 1) {
-                    S.M[MR - 1] = bm;
-                    S.I[MR - 1] = bi;
-                    S.D[MR - 1] = bd;
-                }
+    // ---- A: forward over the strips from strip `first` in steps of
+    // `stride`, the shifts taken `lag` strips back; a strip's feeder waits
+    // for the strip above to publish the columns of each block (LDS
+    // progress, bounded waits)
+    static_assert(kShiftLag % W == 0, "strip cs - kShiftLag ran on this wave");
+    bool liveA = true;  // this wave's waits have not run out
+    auto pass_a = [&](const int first, const int stride, const int lag) {
+    for (int cs = first; cs < nstr; cs += stride) {
+        Priors<T, MR> P;
+        AdjRows<T, MR> A;
+        State<T, MR> S;
+        long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S);
+        const bool last = cs == nstr - 1;
+        const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
+        T *bout = B + (uint64_t)cs * a.lrow;
+        int sh = 0;
+        if (cs > 0) {
+            // the row above up to column 1 (written at the strip above's step 63)
+            if (liveA) {
+                liveA = wait_progress_ge(progA + cs - 1, 63, a.spin_limit);
+                if (!liveA && l == 0) atomicAdd(a.fault, 1u);
             }
-            // a strip that is not the last stores its last row up to column
-            // H + 69, past every column the recomputations' feeders read (D4 + 1)
-            const int Dend = last ? H + lR : H + 2 * kL + 3;
-            T bmax = T(0);
-            sm = T(0);
-            si = T(0);
-            for (int d0 = 1; d0 <= Dend; d0 += kL) {
-                const int dlim = d0 + kL - 1 < Dend ? d0 + kL - 1 : Dend;
-                T fm = T(0), fi = T(0), fd = T(0);
-                if (cs > 0 && d0 + l <= dlim) {
-                    const T *in = bin + 3 * (d0 + l + 1 + kL);
-                    fm = ldexp(ld_ag(in), sh);
-                    fi = ldexp(ld_ag(in + 1), sh);
-                    fd = ldexp(ld_ag(in + 2), sh);
-                }
-                // one step; two per iteration, so that the state's registers
-                // alternate instead of being copied back after every step
-                auto one_step = [&](const int d) {
-                    step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
-                    if (last && d - l <= H) {
-                        sm += S.M[MR - 1];
-                        si += S.I[MR - 1];
-                    }
-                    if (!last && l == kL - 2) {  // column d - (L-2) at 3 * (d + 2)
-                        T *o = bout + 3 * (d + 2);
-                        st_ag(o, S.M[MR - 1]);
-                        st_ag(o + 1, S.I[MR - 1]);
-                        st_ag(o + 2, S.D[MR - 1]);
-                        bmax = fmax(bmax, fmax(S.M[MR - 1], fmax(S.I[MR - 1], S.D[MR - 1])));
-                    }
-                    if (cs > 0) {  // column d + 1 of the row above
-                        const int k = d - d0;
-                        const T bm = lane_value(fm, k), bi = lane_value(fi, k), bd = lane_value(fd, k);
-                        if (l == kL - 1) {
-                            S.M[MR - 1] = bm;
-                            S.I[MR - 1] = bi;
-                            S.D[MR - 1] = bd;
-                        }
-                    }
-                };
-                int d = d0;
-                for (; d + 1 <= dlim; d += 2) {
-                    one_step(d);
-                    one_step(d + 1);
-                }
-                if (d <= dlim) one_step(d);
+            // the shift: strip cs - kShiftLag's last row, brought to the scale
+            // of the row above through the shifts since (each published with
+            // its strip's first progress), goes to 2^kLongTop
+            const int j = cs - lag;
+            if (j >= 0 && mlog[j] != kNoMax) {
+                int since = 0;
+                for (int i = j + 1; i < cs; i++) since += shs[i];
+                sh = kLongTop - mlog[j] - since;
             }
-            if (!last) {
-                __threadfence();  // the row's stores before the next strip's loads
-                const T mx = lane_value(bmax, kL - 2);
-                const int nsh = mx > T(0) ? kLongTop - ilogb(mx) : 0;
-                if (lane == 0) shs[cs + 1] = nsh;
-                E += nsh;
+            if (lane == 0) shs[cs] = sh;
+            // column 1 of the row above (column j at 3 * (j + L))
+            const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
+                    bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
+            if (l == kL - 1) {
+                S.M[MR - 1] = bm;
+                S.I[MR - 1] = bi;
+                S.D[MR - 1] = bd;
             }
+        } else if (lane == 0) {
+            shs[0] = 0;
         }
-        const T Ssum = lane_value(sm + si, lR);  // pairhmm_impl.cpp:434-437
-        if (lane == 0) *ssum = Ssum;
+        // a strip that is not the last stores its last row up to column
+        // H + 69, past every column the recomputations' feeders read (D4 + 1)
+        const int Dend = last ? H + lR : H + 2 * kL + 3;
+        T bmax = T(0), sm = T(0), si = T(0);
+        for (int d0 = 1; d0 <= Dend; d0 += kL) {
+            const int dlim = d0 + kL - 1 < Dend ? d0 + kL - 1 : Dend;
+            // the block's feeder columns d0 + 1 .. dlim + 1, written by the
+            // strip above up to its step dlim + 63
+            if (cs > 0 && liveA) {
+                liveA = wait_progress_ge(progA + cs - 1, dlim + 63, a.spin_limit);
+                if (!liveA && l == 0) atomicAdd(a.fault, 1u);
+            }
+            T fm = T(0), fi = T(0), fd = T(0);
+            if (cs > 0 && d0 + l <= dlim) {
+                const T *in = bin + 3 * (d0 + l + 1 + kL);
+                fm = ldexp(ld_ag(in), sh);
+                fi = ldexp(ld_ag(in + 1), sh);
+                fd = ldexp(ld_ag(in + 2), sh);
+            }
+            if (!liveA) fm = T(__builtin_nan(""));
+            // one step; two per iteration, so that the state's registers
+            // alternate instead of being copied back after every step
+            auto one_step = [&](const int d) {
+                step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
+                if (last && d - l <= H) {
+                    sm += S.M[MR - 1];
+                    si += S.I[MR - 1];
+                }
+                if (!last && l == kL - 2) {  // column d - (L-2) at 3 * (d + 2)
+                    T *o = bout + 3 * (d + 2);
+                    st_ag(o, S.M[MR - 1]);
+                    st_ag(o + 1, S.I[MR - 1]);
+                    st_ag(o + 2, S.D[MR - 1]);
+                    bmax = fmax(bmax, fmax(S.M[MR - 1], fmax(S.I[MR - 1], S.D[MR - 1])));
+                }
+                if (cs > 0) {  // column d + 1 of the row above
+                    const int k = d - d0;
+                    const T bm = lane_value(fm, k), bi = lane_value(fi, k), bd = lane_value(fd, k);
+                    if (l == kL - 1) {
+                        S.M[MR - 1] = bm;
+                        S.I[MR - 1] = bi;
+                        S.D[MR - 1] = bd;
+                    }
+                }
+            };
+            int d = d0;
+            for (; d + 1 <= dlim; d += 2) {
+                one_step(d);
+                one_step(d + 1);
+            }
+            if (d <= dlim) one_step(d);
+            if (!last) publish_progress(progA + cs, dlim);  // the row's columns of this block
+        }
+        if (!last) {
+            const T mx = lane_value(bmax, kL - 2);
+            if (lane == 0) mlog[cs] = mx > T(0) ? ilogb(mx) : kNoMax;
+        } else {
+            const T Ssum = lane_value(sm + si, lR);  // pairhmm_impl.cpp:434-437
+            if (lane == 0) *ssum = Ssum;
+        }
+        publish_progress(progA + cs, kProgNone);  // the whole row
+    }
+    };
+    // all waves, 4 strips at a time; where a strip's row fell more than
+    // 2^1460 below 2^kLongTop (the three strips without a fresh shift lost
+    // more than the range holds: reads unrelated to their haplotype), or the
+    // sum underflowed, wave 0 runs the strips again one after another with
+    // each shift from the row just above (every strip's row at 2^kLongTop)
+    pass_a(w, W, kShiftLag);
+    __syncthreads();
+    bool again = !(*ssum > T(0)) || ilogb(*ssum) < kSafeLog;
+    for (int k = 0; k + 1 < nstr; k++) again = again || mlog[k] < kSafeLog;
+    if (again) {
+        __syncthreads();  // every wave has read the rows' exponents
+        for (int k = threadIdx.x; k < kMaxLongStrips; k += kL * W) {
+            progA[k] = -1;
+            mlog[k] = kNoMax;
+        }
+        __syncthreads();
+        if (w == 0) pass_a(0, 1, 1);
     }
     __syncthreads();
     const T Ssum = *ssum;
+    int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
+    for (int k = 1; k < nstr; k++) E += shs[k];
     if (!(Ssum > T(0))) {  // underflow even renormalised: no gradient
         if (threadIdx.x == 0) {
             a.out_log10[p] = -INFINITY;
