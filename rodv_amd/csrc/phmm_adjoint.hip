@@ -753,70 +753,94 @@ __device__ __forceinline__ void publish_progress(int *c, int v) {
 // LDS operations complete in order)
 __device__ __forceinline__ void wave_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
 
-template <int MR, int W>
-__device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
+// A long pair's per-item scalars (wave-uniform).
+struct LongQ {
+    int p;
+    uint32_t h;
+    int R, H, off, nstr, lR;
+    uint64_t ro;
+    double initd;
+};
+template <int MR>
+__device__ __forceinline__ LongQ long_q(const AdjArgs &a, int p_in) {
+    constexpr int C = (kL - 1) * MR;
+    LongQ q;
+    q.p = uni(p_in);
+    q.h = (uint32_t)uni((int)a.pair_hap[q.p]);
+    const uint32_t r = (uint32_t)uni((int)a.pair_read[q.p]);
+    q.R = uni((int)a.read_len[r]);
+    q.H = uni((int)a.hap_len[q.h]);
+    const uint64_t ro64 = a.read_off[r];
+    q.ro = ((uint64_t)(uint32_t)uni((int)(ro64 >> 32)) << 32) | (uint64_t)(uint32_t)uni((int)(uint32_t)ro64);
+    q.off = (MR - q.R % MR) % MR;
+    const int vR = q.R - 1 + q.off;
+    q.nstr = vR / C + 1;  // <= kMaxLongStrips (checked by the host)
+    q.lR = (vR % C) / MR;
+    q.initd = a.initd_d[q.h];
+    return q;
+}
+
+// The haplotype's codes in LDS, column j at lds[L - 1 + j], staged by the
+// workgroup's nt threads (one-hot, bit 4 set: a read N matches every
+// column); begins and ends with a workgroup barrier.
+__device__ __forceinline__ void long_codes(const AdjArgs &a, uint8_t *lds, const LongQ &q, int nt) {
+    __syncthreads();  // the previous item's LDS reads are done
+    for (uint32_t k = threadIdx.x * 16; k < a.lds_stride; k += (uint32_t)nt * 16)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[q.h]);
+    const int nch = (q.H + 15) >> 4;
+    __syncthreads();
+    for (int c = threadIdx.x; c < nch; c += nt) {
+        const uint4 v = src[c];
+        *reinterpret_cast<uint4 *>(lds + kL + 16 * c) =
+            make_uint4(v.x | 0x10101010u, v.y | 0x10101010u, v.z | 0x10101010u, v.w | 0x10101010u);
+    }
+    __syncthreads();
+}
+
+// Pass B's codes, two per byte: column j at nibble L - 1 + j (low nibble
+// first), the one-hot codes (N = 0xf) without the bit that makes a read N
+// match, which long_strip_b puts back on every nonzero code (the zeros past
+// the haplotype stay zero, as in the byte codes); by one wave, between
+// workgroup barriers.  Half the LDS of the byte codes, so that four pass-B
+// waves fit a CU.
+__host__ __device__ constexpr uint32_t packed_stride(uint32_t lds_stride) { return ((lds_stride + 1) / 2 + 15) & ~15u; }
+__device__ __forceinline__ uint32_t pack_nibbles(uint32_t x) {  // 4 codes < 16 -> 2 bytes
+    return (x & 0xfu) | ((x >> 4) & 0xf0u) | ((x >> 8) & 0xf00u) | ((x >> 12) & 0xf000u);
+}
+__device__ __forceinline__ void long_codes_packed(const AdjArgs &a, uint8_t *lds, const LongQ &q) {
+    __syncthreads();  // the previous item's LDS reads are done
+    const uint32_t ps = packed_stride(a.lds_stride);
+    for (uint32_t k = threadIdx.x * 16; k < ps; k += 64 * 16)
+        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[q.h]);
+    const int nch = (q.H + 15) >> 4;
+    __syncthreads();
+    for (int c = threadIdx.x; c < nch; c += 64) {  // columns 16c + 1 .. 16c + 16 at bytes 32 + 8c ..
+        const uint4 v = src[c];
+        *reinterpret_cast<uint2 *>(lds + kL / 2 + 8 * c) =
+            make_uint2(pack_nibbles(v.x) | pack_nibbles(v.y) << 16, pack_nibbles(v.z) | pack_nibbles(v.w) << 16);
+    }
+    __syncthreads();
+}
+
+// Pass A over the strips from strip `first` in steps of `stride`, the
+// shifts taken `lag` strips back; a strip's feeder waits for the strip
+// above to publish the columns of each block (LDS progress, bounded waits).
+// Each strip's last row to B (row cs at cs * lrow), its shift to shs[cs],
+// the exponent of its largest entry to mlog[cs]; the last strip's sum to
+// *ssum.
+template <int MR>
+__device__ void long_pass_a(const AdjArgs &a, const LongQ &q, const uint8_t *cp, double *B, int *shs, int *progA,
+                            int *mlog, double *ssum, const int first, const int stride, const int lag,
+                            bool &liveA) {
     using T = double;
     const int l = threadIdx.x & 63;
     const int lane = l;
-    const int w = uni((int)(threadIdx.x >> 6));
-    constexpr int NV = 3 * MR + 3;
-    constexpr int C = (kL - 1) * MR;
-    const int p = uni(p_in);
-    const uint32_t h = (uint32_t)uni((int)a.pair_hap[p]), r = (uint32_t)uni((int)a.pair_read[p]);
-    const int R = uni((int)a.read_len[r]), H = uni((int)a.hap_len[h]);
-    const uint64_t ro64 = a.read_off[r];
-    const uint64_t ro = ((uint64_t)(uint32_t)uni((int)(ro64 >> 32)) << 32) |
-                        (uint64_t)(uint32_t)uni((int)(uint32_t)ro64);
-    const int off = (MR - R % MR) % MR;
-    const int vR = R - 1 + off;
-    const int nstr = vR / C + 1;  // <= kMaxLongStrips (checked by the host)
-    const int lR = (vR % C) / MR;
-    const T initd = a.initd_d[h];
-    const int K1 = a.ckpt;
-
-    T *wbase = reinterpret_cast<T *>(lds + a.lds_stride);
-    T *l2 = wbase + (size_t)w * long_wave_elems(K1, MR);
-    T *fst = l2 + (size_t)((K1 + 7) / 8) * NV * kL;
-    T *bst = fst + 3 * K1;
-    T *tot = wbase + (size_t)W * long_wave_elems(K1, MR);  // per-lane totals, tot[t * 64 + lane]
-    T *ssum = tot + 10 * kL;
-    int *shs = reinterpret_cast<int *>(ssum + 1);
-    int *prog = shs + kMaxLongStrips;
-    int *progA = prog + kMaxLongStrips;
-    int *mlog = progA + kMaxLongStrips;
-    __syncthreads();  // the previous item's LDS reads are done
-    for (int k = threadIdx.x; k < kMaxLongStrips; k += kL * W) {
-        progA[k] = -1;
-        mlog[k] = kNoMax;
-    }
-    for (uint32_t k = threadIdx.x * 16; k < a.lds_stride; k += 64 * W * 16)
-        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[h]);
-        const int nch = (H + 15) >> 4;
-        __syncthreads();
-        // one-hot codes with bit 4 set: a read N matches every column
-        for (int c = threadIdx.x; c < nch; c += kL * W) {
-            const uint4 v = src[c];
-            *reinterpret_cast<uint4 *>(lds + kL + 16 * c) =
-                make_uint4(v.x | 0x10101010u, v.y | 0x10101010u, v.z | 0x10101010u, v.w | 0x10101010u);
-        }
-    }
-    __syncthreads();
-    const uint8_t *cp = lds + (kL - 1) - l;
+    const int R = q.R, H = q.H, off = q.off, nstr = q.nstr, lR = q.lR;
+    const uint64_t ro = q.ro;
+    const T initd = q.initd;
     const LaneXfer<kL, 1> up;
-    const LaneXfer<kL, -1> down;
-    T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)blockIdx.x * a.lbnd_stride;  // row of strip cs at cs * lrow
-    T *AB = reinterpret_cast<T *>(a.lab) + (uint64_t)blockIdx.x * (W + 1) * a.lab_elems;
-    T *ck = reinterpret_cast<T *>(a.ck) + ((uint64_t)blockIdx.x * W + w) * a.ck_elems;
-
-    // ---- A: forward over the strips from strip `first` in steps of
-    // `stride`, the shifts taken `lag` strips back; a strip's feeder waits
-    // for the strip above to publish the columns of each block (LDS
-    // progress, bounded waits)
-    static_assert(kShiftLag % W == 0, "strip cs - kShiftLag ran on this wave");
-    bool liveA = true;  // this wave's waits have not run out
-    auto pass_a = [&](const int first, const int stride, const int lag) {
     for (int cs = first; cs < nstr; cs += stride) {
         Priors<T, MR> P;
         AdjRows<T, MR> A;
@@ -915,16 +939,29 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
         }
         publish_progress(progA + cs, kProgNone);  // the whole row
     }
-    };
-    // all waves, 4 strips at a time; where a strip's row fell more than
-    // 2^1460 below 2^kLongTop (the three strips without a fresh shift lost
-    // more than the range holds: reads unrelated to their haplotype), or the
-    // sum underflowed, wave 0 runs the strips again one after another with
-    // each shift from the row just above (every strip's row at 2^kLongTop)
-    pass_a(w, W, kShiftLag);
+}
+
+// Pass A by the W waves of the workgroup, 4 strips at a time, and where a
+// strip's row fell more than 2^1460 below 2^kLongTop (the three strips
+// without a fresh shift lost more than the range holds: reads unrelated to
+// their haplotype) or the sum underflowed, again by wave 0 with each shift
+// from the row just above (every strip's row at 2^kLongTop).  Ends with a
+// workgroup barrier.
+template <int MR, int W>
+__device__ void long_forward(const AdjArgs &a, const LongQ &q, const uint8_t *cp, double *B, int *shs, int *progA,
+                             int *mlog, double *ssum) {
+    static_assert(kShiftLag % W == 0, "strip cs - kShiftLag ran on this wave");
+    const int w = uni((int)(threadIdx.x >> 6));
+    for (int k = threadIdx.x; k < kMaxLongStrips; k += kL * W) {
+        progA[k] = -1;
+        mlog[k] = kNoMax;
+    }
     __syncthreads();
-    bool again = !(*ssum > T(0)) || ilogb(*ssum) < kSafeLog;
-    for (int k = 0; k + 1 < nstr; k++) again = again || mlog[k] < kSafeLog;
+    bool liveA = true;  // this wave's waits have not run out
+    long_pass_a<MR>(a, q, cp, B, shs, progA, mlog, ssum, w, W, kShiftLag, liveA);
+    __syncthreads();
+    bool again = !(*ssum > 0.0) || ilogb(*ssum) < kSafeLog;
+    for (int k = 0; k + 1 < q.nstr; k++) again = again || mlog[k] < kSafeLog;
     if (again) {
         __syncthreads();  // every wave has read the rows' exponents
         for (int k = threadIdx.x; k < kMaxLongStrips; k += kL * W) {
@@ -932,9 +969,270 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
             mlog[k] = kNoMax;
         }
         __syncthreads();
-        if (w == 0) pass_a(0, 1, 1);
+        if (w == 0) long_pass_a<MR>(a, q, cp, B, shs, progA, mlog, ssum, 0, 1, 1, liveA);
     }
     __syncthreads();
+}
+
+// Progress of the strips of one pair spread over several workgroups
+// (HBM counters, agent scope): as wait_progress / publish_progress.
+__device__ __forceinline__ bool wait_progress_x(const int *c, int need, uint32_t limit) {
+    for (uint32_t k = 0; k < limit; k++) {
+        const int v = uni(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (v <= need) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the handed-up rows after the count
+            return true;
+        }
+        __builtin_amdgcn_s_sleep(16);
+    }
+    return false;
+}
+__device__ __forceinline__ void publish_progress_x(int *c, int v) {
+    __threadfence();  // the handed-up rows before the count
+    if ((threadIdx.x & 63) == 0) __hip_atomic_store(c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Pass B of strip cs: its forward recomputed from its input row B[cs - 1]
+// (scaled by 2^sh) with level-1 checkpoints in ck, then segment by segment
+// from the last the level-2 states in l2 and the reverse sweep, the
+// adjoints from below read from ab_in (scaled by 2^sh_below) once the strip
+// below has published their steps in *prog_below, this strip's first row's
+// adjoints written to ab_out and published in *prog_self.  XWG: the pair's
+// strips run on several workgroups (HBM progress), else on the waves of
+// one workgroup (LDS progress).  The rows' gradient sums are left in gd,
+// gdT, gg and ge.
+template <int MR, bool XWG>
+__device__ void long_strip_b(const AdjArgs &a, const LongQ &q, const uint8_t *cp, const int cs, const double *B,
+                             const int sh, const int sh_below, const double sigma, const double *ab_in,
+                             double *ab_out, const int *prog_below, int *prog_self, double *ck, double *l2,
+                             double *fst, double *bst, bool &live, double (&gd)[MR], double (&gdT)[MR],
+                             double (&gg)[MR], double (&ge)[MR]) {
+    using T = double;
+    constexpr int NV = 3 * MR + 3;
+    const int l = threadIdx.x & 63;
+    const int R = q.R, H = q.H, off = q.off, nstr = q.nstr, lR = q.lR;
+    const uint64_t ro = q.ro;
+    const T initd = q.initd;
+    const int K1 = a.ckpt;
+    const LaneXfer<kL, 1> up;
+    const LaneXfer<kL, -1> down;
+    // the code this lane's rows meet at step d: XWG, cp is the base of the
+    // packed codes (long_codes_packed), else the lane's byte codes
+    auto code = [&](int d) -> uint32_t {
+        if constexpr (XWG) {
+            const int n = (kL - 1) - l + d;
+            const uint32_t nib = ((uint32_t)cp[n >> 1] >> ((n & 1) << 2)) & 0xfu;
+            return nib | ((nib + 15u) & 0x10u);
+        } else {
+            return cp[d];
+        }
+    };
+    Priors<T, MR> P;
+    AdjRows<T, MR> A;
+    State<T, MR> S0;
+    long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S0);
+    const bool last = cs == nstr - 1;
+    const T fb = last ? T(0) : ldexp(1.0, sh_below);  // adjoints from below to this strip's scale
+    const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
+    if (cs > 0) {
+        const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
+                bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
+        if (l == kL - 1) {
+            S0.M[MR - 1] = bm;
+            S0.I[MR - 1] = bi;
+            S0.D[MR - 1] = bd;
+        }
+    }
+    // the reverse covers every step at which a lane of read rows is at a
+    // column 1..H: up to lane lR (last strip) or lane L-2
+    const int Dn = last ? H + lR : H + kL - 2;
+    const int D4 = (Dn + kW - 1) / kW * kW;
+    const int nseg = (D4 + K1 - 1) / K1;
+    // feeder inputs of the segment after step d0s: column d0s + 2 + m at fst[3m]
+    auto stage_feeder = [&](int d0s) {
+        wave_sync();
+        if (cs > 0)
+            for (int k = l; k < 3 * K1; k += kL) {
+                const int c = d0s + 2 + k / 3;
+                fst[k] = ldexp(ld_ag(bin + 3 * (c + kL) + k % 3), sh);
+            }
+        wave_sync();
+    };
+    // a forward step, then the feeder's injection of column d + 1
+    auto fstep = [&](State<T, MR> &S, int d, int d0s) {
+        step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, code(d), up);
+        if (cs > 0) {
+            const T *f = fst + 3 * (d - d0s - 1);
+            const T bm = f[0], bi = f[1], bd = f[2];
+            if (l == kL - 1) {
+                S.M[MR - 1] = bm;
+                S.I[MR - 1] = bi;
+                S.D[MR - 1] = bd;
+            }
+        }
+    };
+    // pass 1: the strip's forward with level-1 checkpoints
+    {
+        State<T, MR> S = S0;
+        put_state(ck, NV, 0, S);
+        for (int k = 0; k < nseg; k++) {
+            const int d0s = k * K1;
+            stage_feeder(d0s);
+            const int n = min(K1, D4 - d0s);
+            for (int u = 1; u <= n; u += 2) {  // n is a multiple of 4
+                fstep(S, d0s + u, d0s);
+                fstep(S, d0s + u + 1, d0s);
+            }
+            if (n == K1 && d0s + K1 < D4) put_state(ck, NV, k + 1, S);
+        }
+    }
+    T Tb[MR], Ib[MR], Db[MR];
+#pragma unroll
+    for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gdT[s] = gg[s] = ge[s] = T(0);
+    T Tb0p = T(0);
+    // one reverse step: lane L-1 first presents the adjoints of the row
+    // below lane L-2's last row (bst[2m] = Tb, bst[2m + 1] = Ib of column
+    // d0s - 61 + m); lane 0 then hands its first row's adjoints up
+    auto rstep = [&](const State<T, MR> &Sb, const State<T, MR> &Sa, int d, int d0s) {
+        if (!last) {
+            const T tT = bst[2 * (d - d0s)];          // Tb at column d - 61 = j + 1
+            const T iI = bst[2 * (d - d0s) - 1];      // Ib at column d - 62 = j
+            if (l == kL - 1) {
+                Tb0p = tT;
+                Ib[0] = iI;
+            }
+        }
+        const int j = d - l;
+        const T seed = (last && l == lR && j >= 1 && j <= H) ? sigma : T(0);
+        adj_step<T, MR>(Sb, Sa, P, A, code(d), seed, Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
+        if (cs > 0 && l == 0) {
+            st_ag(ab_out + 2 * d, Tb[0]);
+            st_ag(ab_out + 2 * d + 1, Ib[0]);
+        }
+    };
+    auto window4 = [&](const State<T, MR> &W0, int dw, int d0s) {
+        State<T, MR> W1 = W0;
+        fstep(W1, dw + 1, d0s);
+        State<T, MR> W2 = W1;
+        fstep(W2, dw + 2, d0s);
+        State<T, MR> W3 = W2;
+        fstep(W3, dw + 3, d0s);
+        State<T, MR> W4 = W3;
+        fstep(W4, dw + 4, d0s);
+        rstep(W3, W4, dw + 4, d0s);
+        rstep(W2, W3, dw + 3, d0s);
+        rstep(W1, W2, dw + 2, d0s);
+        rstep(W0, W1, dw + 1, d0s);
+    };
+    // passes 2 and 3, segment by segment from the last
+    for (int k = nseg - 1; k >= 0; k--) {
+        const int d0s = k * K1;
+        const int nsteps = min(K1, D4 - d0s);
+        const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
+        stage_feeder(d0s);
+        if (!last) {
+            // columns d0s - 61 .. d0s - 61 + K1 of the rows handed up,
+            // once the strip below has finished their steps
+            const int lo = max(1, d0s - 61), hi = min(H, d0s - 61 + K1);
+            if (live && lo <= hi) {
+                live = XWG ? wait_progress_x(prog_below, lo, a.spin_limit)
+                           : wait_progress(prog_below, lo, a.spin_limit);
+                if (!live && l == 0) atomicAdd(a.fault, 1u);
+            }
+            const T poison = live ? T(1) : T(__builtin_nan(""));
+            for (int m = l; m <= K1; m += kL) {
+                const int c = d0s - 61 + m;
+                const bool in = c >= 1 && c <= H;  // beyond: no adjoint
+                bst[2 * m] = in ? ld_ag(ab_in + 2 * c) * fb * poison : T(0);
+                bst[2 * m + 1] = in ? ld_ag(ab_in + 2 * c + 1) * fb * poison : T(0);
+            }
+        }
+        wave_sync();
+        State<T, MR> S;
+        get_state(ck, NV, k, S);
+        // the window starts; the last window's steps are its own pass 3's
+        for (int wi = 0; wi < nwin; wi++) {
+            put_state(l2, NV, wi, S);
+            if (wi == nwin - 1) break;
+            for (int u = 0; u < 2 * kW; u++) fstep(S, d0s + wi * 2 * kW + u + 1, d0s);
+        }
+        wave_sync();
+        for (int wi = nwin - 1; wi >= 0; wi--) {
+            const int dw = d0s + wi * 2 * kW;
+            const int n = min(2 * kW, nsteps - wi * 2 * kW);
+            State<T, MR> W0;
+            get_state(l2, NV, wi, W0);
+            if (n == 2 * kW) {
+                State<T, MR> X = W0;
+#pragma unroll
+                for (int u = 1; u <= kW; u++) fstep(X, dw + u, d0s);
+                window4(X, dw + kW, d0s);
+            }
+            window4(W0, dw, d0s);
+        }
+        // steps d0s + 1 .. of this strip done: their handed-up columns
+        if (cs > 0) {
+            if constexpr (XWG)
+                publish_progress_x(prog_self, d0s + 1);
+            else
+                publish_progress(prog_self, d0s + 1);
+        }
+    }
+}
+
+// A pair's outputs (one wave): the per-lane totals v summed over the lanes,
+// log10 L and d log10 L / d theta written by lane 0.
+__device__ __forceinline__ void long_outputs(const AdjArgs &a, int p, double (&v)[10], double sigma, double Ssum,
+                                             int E) {
+    double acc[10];
+#pragma unroll
+    for (int t = 0; t < 10; t++) {
+        double x = v[t];
+        for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+        acc[t] = x;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const double sS = sigma * Ssum;
+        const double ln10 = 2.302585092994045684;
+        a.out_log10[p] = log10(Ssum) - (1020.0 + (double)E) * 0.30102999566398119521;
+        double *g = a.out_grad + 10 * (uint64_t)p;
+        g[0] = -a.delta * acc[0] / (10.0 * sS);
+        g[1] = -a.gamma * acc[1] / (10.0 * sS);
+        for (int b = 0; b < 8; b++) g[2 + b] = acc[2 + b] / (sS * ln10);
+        a.flags[p] = 0x01;
+    }
+}
+
+template <int MR, int W>
+__device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
+    using T = double;
+    const int l = threadIdx.x & 63;
+    const int lane = l;
+    const int w = uni((int)(threadIdx.x >> 6));
+    constexpr int C = (kL - 1) * MR;
+    const LongQ q = long_q<MR>(a, p_in);
+    const int p = q.p, R = q.R, off = q.off, nstr = q.nstr;
+    const uint64_t ro = q.ro;
+    const int K1 = a.ckpt;
+
+    T *wbase = reinterpret_cast<T *>(lds + a.lds_stride);
+    T *l2 = wbase + (size_t)w * long_wave_elems(K1, MR);
+    T *fst = l2 + (size_t)((K1 + 7) / 8) * (3 * MR + 3) * kL;
+    T *bst = fst + 3 * K1;
+    T *tot = wbase + (size_t)W * long_wave_elems(K1, MR);  // per-lane totals, tot[t * 64 + lane]
+    T *ssum = tot + 10 * kL;
+    int *shs = reinterpret_cast<int *>(ssum + 1);
+    int *prog = shs + kMaxLongStrips;
+    int *progA = prog + kMaxLongStrips;
+    int *mlog = progA + kMaxLongStrips;
+    long_codes(a, lds, q, 64 * W);
+    const uint8_t *cp = lds + (kL - 1) - l;
+    T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)blockIdx.x * a.lbnd_stride;  // row of strip cs at cs * lrow
+    T *AB = reinterpret_cast<T *>(a.lab) + (uint64_t)blockIdx.x * (W + 1) * a.lab_elems;
+    T *ck = reinterpret_cast<T *>(a.ck) + ((uint64_t)blockIdx.x * W + w) * a.ck_elems;
+
+    // ---- A
+    long_forward<MR, W>(a, q, cp, B, shs, progA, mlog, ssum);
     const T Ssum = *ssum;
     int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
     for (int k = 1; k < nstr; k++) E += shs[k];
@@ -954,154 +1252,12 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     // ---- B: strips last to first, dealt to the waves
     bool live = true;  // this wave's waits have not run out
     for (int cs = nstr - 1 - w; cs >= 0; cs -= W) {
-        Priors<T, MR> P;
-        AdjRows<T, MR> A;
-        State<T, MR> S0;
-        long_setup<MR>(a, cs, nstr, R, off, ro, initd, P, A, S0);
         const bool last = cs == nstr - 1;
-        const int sh = shs[cs];
-        const T fb = last ? T(0) : ldexp(1.0, shs[cs + 1]);  // adjoints from below to this strip's scale
-        const T *bin = B + (uint64_t)(cs > 0 ? cs - 1 : 0) * a.lrow;
-        const T *ab_in = AB + (uint64_t)((cs + 1) % (W + 1)) * a.lab_elems;
-        T *ab_out = AB + (uint64_t)(cs % (W + 1)) * a.lab_elems;
-        if (cs > 0) {
-            const T bm = ldexp(ld_ag(bin + 3 * (1 + kL)), sh), bi = ldexp(ld_ag(bin + 3 * (1 + kL) + 1), sh),
-                    bd = ldexp(ld_ag(bin + 3 * (1 + kL) + 2), sh);
-            if (l == kL - 1) {
-                S0.M[MR - 1] = bm;
-                S0.I[MR - 1] = bi;
-                S0.D[MR - 1] = bd;
-            }
-        }
-        // the reverse covers every step at which a lane of read rows is at a
-        // column 1..H: up to lane lR (last strip) or lane L-2
-        const int Dn = last ? H + lR : H + kL - 2;
-        const int D4 = (Dn + kW - 1) / kW * kW;
-        const int nseg = (D4 + K1 - 1) / K1;
-        // feeder inputs of the segment after step d0s: column d0s + 2 + m at fst[3m]
-        auto stage_feeder = [&](int d0s) {
-            wave_sync();
-            if (cs > 0)
-                for (int k = l; k < 3 * K1; k += kL) {
-                    const int c = d0s + 2 + k / 3;
-                    fst[k] = ldexp(ld_ag(bin + 3 * (c + kL) + k % 3), sh);
-                }
-            wave_sync();
-        };
-        // a forward step, then the feeder's injection of column d + 1
-        auto fstep = [&](State<T, MR> &S, int d, int d0s) {
-            step<T, kL, MR, false>(S, P, A.ab_yy, A.ab_my, cp[d], up);
-            if (cs > 0) {
-                const T *f = fst + 3 * (d - d0s - 1);
-                const T bm = f[0], bi = f[1], bd = f[2];
-                if (l == kL - 1) {
-                    S.M[MR - 1] = bm;
-                    S.I[MR - 1] = bi;
-                    S.D[MR - 1] = bd;
-                }
-            }
-        };
-        // pass 1: the strip's forward with level-1 checkpoints
-        {
-            State<T, MR> S = S0;
-            put_state(ck, NV, 0, S);
-            for (int k = 0; k < nseg; k++) {
-                const int d0s = k * K1;
-                stage_feeder(d0s);
-                const int n = min(K1, D4 - d0s);
-                for (int u = 1; u <= n; u += 2) {  // n is a multiple of 4
-                    fstep(S, d0s + u, d0s);
-                    fstep(S, d0s + u + 1, d0s);
-                }
-                if (n == K1 && d0s + K1 < D4) put_state(ck, NV, k + 1, S);
-            }
-        }
-        T Tb[MR], Ib[MR], Db[MR], gd[MR], gdT[MR], gg[MR], ge[MR];
-#pragma unroll
-        for (int s = 0; s < MR; s++) Tb[s] = Ib[s] = Db[s] = gd[s] = gdT[s] = gg[s] = ge[s] = T(0);
-        T Tb0p = T(0);
-        // one reverse step: lane L-1 first presents the adjoints of the row
-        // below lane L-2's last row (bst[2m] = Tb, bst[2m + 1] = Ib of column
-        // d0s - 61 + m); lane 0 then hands its first row's adjoints up
-        auto rstep = [&](const State<T, MR> &Sb, const State<T, MR> &Sa, int d, int d0s) {
-            if (!last) {
-                const T tT = bst[2 * (d - d0s)];          // Tb at column d - 61 = j + 1
-                const T iI = bst[2 * (d - d0s) - 1];      // Ib at column d - 62 = j
-                if (l == kL - 1) {
-                    Tb0p = tT;
-                    Ib[0] = iI;
-                }
-            }
-            const int j = d - l;
-            const T seed = (last && l == lR && j >= 1 && j <= H) ? sigma : T(0);
-            adj_step<T, MR>(Sb, Sa, P, A, cp[d], seed, Tb, Ib, Db, Tb0p, gd, gdT, gg, ge, down);
-            if (cs > 0 && l == 0) {
-                st_ag(ab_out + 2 * d, Tb[0]);
-                st_ag(ab_out + 2 * d + 1, Ib[0]);
-            }
-        };
-        auto window4 = [&](const State<T, MR> &W0, int dw, int d0s) {
-            State<T, MR> W1 = W0;
-            fstep(W1, dw + 1, d0s);
-            State<T, MR> W2 = W1;
-            fstep(W2, dw + 2, d0s);
-            State<T, MR> W3 = W2;
-            fstep(W3, dw + 3, d0s);
-            State<T, MR> W4 = W3;
-            fstep(W4, dw + 4, d0s);
-            rstep(W3, W4, dw + 4, d0s);
-            rstep(W2, W3, dw + 3, d0s);
-            rstep(W1, W2, dw + 2, d0s);
-            rstep(W0, W1, dw + 1, d0s);
-        };
-        // passes 2 and 3, segment by segment from the last
-        for (int k = nseg - 1; k >= 0; k--) {
-            const int d0s = k * K1;
-            const int nsteps = min(K1, D4 - d0s);
-            const int nwin = (nsteps + 2 * kW - 1) / (2 * kW);
-            stage_feeder(d0s);
-            if (!last) {
-                // columns d0s - 61 .. d0s - 61 + K1 of the rows handed up,
-                // once the strip below has finished their steps
-                const int lo = max(1, d0s - 61), hi = min(H, d0s - 61 + K1);
-                if (live && lo <= hi) {
-                    live = wait_progress(prog + cs + 1, lo, a.spin_limit);
-                    if (!live && l == 0) atomicAdd(a.fault, 1u);
-                }
-                const T poison = live ? T(1) : T(__builtin_nan(""));
-                for (int m = l; m <= K1; m += kL) {
-                    const int c = d0s - 61 + m;
-                    const bool in = c >= 1 && c <= H;  // beyond: no adjoint
-                    bst[2 * m] = in ? ld_ag(ab_in + 2 * c) * fb * poison : T(0);
-                    bst[2 * m + 1] = in ? ld_ag(ab_in + 2 * c + 1) * fb * poison : T(0);
-                }
-            }
-            wave_sync();
-            State<T, MR> S;
-            get_state(ck, NV, k, S);
-            // the window starts; the last window's steps are its own pass 3's
-            for (int wi = 0; wi < nwin; wi++) {
-                put_state(l2, NV, wi, S);
-                if (wi == nwin - 1) break;
-                for (int u = 0; u < 2 * kW; u++) fstep(S, d0s + wi * 2 * kW + u + 1, d0s);
-            }
-            wave_sync();
-            for (int wi = nwin - 1; wi >= 0; wi--) {
-                const int dw = d0s + wi * 2 * kW;
-                const int n = min(2 * kW, nsteps - wi * 2 * kW);
-                State<T, MR> W0;
-                get_state(l2, NV, wi, W0);
-                if (n == 2 * kW) {
-                    State<T, MR> X = W0;
-#pragma unroll
-                    for (int u = 1; u <= kW; u++) fstep(X, dw + u, d0s);
-                    window4(X, dw + kW, d0s);
-                }
-                window4(W0, dw, d0s);
-            }
-            // steps d0s + 1 .. of this strip done: their handed-up columns
-            if (cs > 0) publish_progress(prog + cs, d0s + 1);
-        }
+        T gd[MR], gdT[MR], gg[MR], ge[MR];
+        long_strip_b<MR, false>(a, q, cp, cs, B, shs[cs], last ? 0 : shs[cs + 1], sigma,
+                                AB + (uint64_t)((cs + 1) % (W + 1)) * a.lab_elems,
+                                AB + (uint64_t)(cs % (W + 1)) * a.lab_elems, prog + cs + 1, prog + cs, ck, l2, fst,
+                                bst, live, gd, gdT, gg, ge);
         // the strip's rows into the per-lane totals, after the strip below's
         if (!last && live) {
             live = wait_progress(prog + cs + 1, 0, a.spin_limit);
@@ -1122,24 +1278,12 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     }
     __syncthreads();  // every strip's totals
     if (w != 0) return;
-    double acc[10];
+    double v[10];
 #pragma unroll
-    for (int t = 0; t < 10; t++) {
-        double v = tot[t * kL + lane];
-        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-        acc[t] = v;
-    }
-    if (lane == 0) {
-        const double sS = sigma * Ssum;
-        const double ln10 = 2.302585092994045684;
-        a.out_log10[p] = log10(Ssum) - (1020.0 + (double)E) * 0.30102999566398119521;
-        double *g = a.out_grad + 10 * (uint64_t)p;
-        g[0] = -a.delta * acc[0] / (10.0 * sS);
-        g[1] = -a.gamma * acc[1] / (10.0 * sS);
-        for (int b = 0; b < 8; b++) g[2 + b] = acc[2 + b] / (sS * ln10);
-        a.flags[p] = 0x01;
-    }
+    for (int t = 0; t < 10; t++) v[t] = tot[t * kL + lane];
+    long_outputs(a, p, v, sigma, Ssum, E);
 }
+
 
 template <int W>
 __global__ __launch_bounds__(64 * W, 1) void adj_long_kernel(AdjArgs a) {
@@ -1154,6 +1298,145 @@ __global__ __launch_bounds__(64 * W, 1) void adj_long_kernel(AdjArgs a) {
         const uint32_t it = (uint32_t)uni(*slot);
         if (it >= n) break;
         adj_long_item<kLongMR, W>(a, lds, a.items[it]);
+    }
+}
+
+// ---- a pair's strips over several workgroups (the default), in three
+// launches:
+//  A. pass A of each pair (a.items, longest first) on a workgroup of
+//     kLongWaves waves: its rows to the pair's B rows (a.lbnd + li *
+//     lbnd_stride), its shifts, scaled sum and exponent to a.lsh, a.lsum,
+//     a.lexp;
+//  B. one wave per item (pair, strip) of a.bitems -- pairs longest first,
+//     each pair's strips last to first -- running pass B of the strip: the
+//     adjoints it hands up go through one HBM row per strip, with the steps
+//     done published in an HBM count per strip.  A wave waits (bounded) only
+//     for the strip below, an earlier item of the list, which a running
+//     wave took, so every wait ends.  The rows' gradient contributions go to
+//     a.lpart per strip, row and lane;
+//  C. per pair, the contributions added strip by strip, last first, in the
+//     order of the one-workgroup kernel (the same bits), and the outputs.
+// The longest pair's strips then run on as many CUs as it has strips
+// instead of on one.
+__global__ __launch_bounds__(64 * kLongWaves, 1) void adj_longa_kernel(AdjArgs a) {
+    using T = double;
+    extern __shared__ __align__(16) uint8_t lds[];
+    // LDS: codes | scaled sum | shifts | forward progress | row exponents | item
+    T *ssum = reinterpret_cast<T *>(lds + a.lds_stride);
+    int *shs = reinterpret_cast<int *>(ssum + 1);
+    int *progA = shs + kMaxLongStrips;
+    int *mlog = progA + kMaxLongStrips;
+    int *slot = mlog + kMaxLongStrips;
+    const uint32_t n = a.n_items;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        __syncthreads();  // every wave has read the slot
+        if (threadIdx.x == 0) *slot = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint32_t li = (uint32_t)uni(*slot);
+        if (li >= n) break;
+        const LongQ q = long_q<kLongMR>(a, a.items[li]);
+        long_codes(a, lds, q, 64 * kLongWaves);
+        const uint8_t *cp = lds + (kL - 1) - (int)(threadIdx.x & 63);
+        T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)li * a.lbnd_stride;
+        long_forward<kLongMR, kLongWaves>(a, q, cp, B, shs, progA, mlog, ssum);
+        const T Ssum = *ssum;
+        for (int k = threadIdx.x; k < q.nstr; k += 64 * kLongWaves) a.lsh[(uint64_t)li * a.lstrips + k] = shs[k];
+        if (threadIdx.x == 0) {
+            int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
+            for (int k = 1; k < q.nstr; k++) E += shs[k];
+            a.lsum[li] = Ssum;
+            a.lexp[li] = E;
+            if (!(Ssum > T(0))) {  // underflow even renormalised: no gradient
+                a.out_log10[q.p] = -INFINITY;
+                for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)q.p + t] = 0.0;
+                a.flags[q.p] = 0x03;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(64, 1) void adj_longb_kernel(AdjArgs a) {
+    using T = double;
+    constexpr int MR = kLongMR;
+    constexpr int C = (kL - 1) * MR;
+    extern __shared__ __align__(16) uint8_t lds[];
+    const int l = threadIdx.x & 63;
+    const int K1 = a.ckpt;
+    // LDS: packed codes | level-2 states | feeder inputs of a segment |
+    // adjoints from below of a segment
+    T *l2 = reinterpret_cast<T *>(lds + packed_stride(a.lds_stride));
+    T *fst = l2 + (size_t)((K1 + 7) / 8) * (3 * MR + 3) * kL;
+    T *bst = fst + 3 * K1;
+    T *ck = reinterpret_cast<T *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
+    const uint32_t n = a.n_bitems;
+    // a wave takes at most n items: the bound makes every wave exit
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        uint32_t it = 0;
+        if (l == 0) it = atomicAdd(a.bcounter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n) break;
+        const uint32_t e = a.bitems[it];
+        const uint32_t li = (uint32_t)uni((int)(e >> 8));
+        const int cs = uni((int)(e & 0xffu));
+        const T Ssum = a.lsum[li];
+        if (!__builtin_amdgcn_readfirstlane(Ssum > T(0) ? 1 : 0)) continue;  // no gradient: no strip runs
+        const LongQ q = long_q<MR>(a, a.items[li]);
+        long_codes_packed(a, lds, q);
+        const uint8_t *cp = lds;
+        const T *B = reinterpret_cast<const T *>(a.lbnd) + (uint64_t)li * a.lbnd_stride;
+        const int *sh = a.lsh + (uint64_t)li * a.lstrips;
+        const bool last = cs == q.nstr - 1;
+        T *ab = reinterpret_cast<T *>(a.lab) + (uint64_t)li * a.lstrips * a.lab_elems;
+        int *pr = a.lprog + (uint64_t)li * a.lstrips;
+        bool live = true;
+        T gd[MR], gdT[MR], gg[MR], ge[MR];
+        long_strip_b<MR, true>(a, q, cp, cs, B, sh[cs], last ? 0 : sh[cs + 1], ldexp(1.0, -ilogb(Ssum)),
+                               ab + (uint64_t)(last ? cs : cs + 1) * a.lab_elems, ab + (uint64_t)cs * a.lab_elems,
+                               pr + (last ? cs : cs + 1), pr + cs, ck, l2, fst, bst, live, gd, gdT, gg, ge);
+        // the rows' contributions to the totals, as the one-workgroup kernel
+        // adds them: per row the delta and gamma sums, the error-prior sum
+        // and its quality bin
+        T *part = a.lpart + ((uint64_t)li * a.lstrips + cs) * MR * 4 * kL;
+#pragma unroll
+        for (int s = 0; s < MR; s++) {
+            const int i = cs * C + l * MR + s - q.off + 1;
+            const bool rl = l < kL - 1 && i >= 1 && i <= q.R;
+            const uint32_t qq = a.q[q.ro + (uint64_t)(rl ? i - 1 : 0)] & 127u;
+            part[(s * 4 + 0) * kL + l] = rl ? fma(T(-2), gdT[s], gd[s]) : T(0);
+            part[(s * 4 + 1) * kL + l] = rl ? gg[s] : T(0);
+            part[(s * 4 + 2) * kL + l] = rl ? ge[s] * a.wq[qq] : T(0);
+            part[(s * 4 + 3) * kL + l] = (T)a.qbin[qq];
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void adj_longc_kernel(AdjArgs a) {
+    using T = double;
+    constexpr int MR = kLongMR;
+    const int l = threadIdx.x & 63;
+    for (uint32_t li = blockIdx.x; li < a.n_items; li += gridDim.x) {
+        const T Ssum = a.lsum[li];
+        if (!(Ssum > T(0))) continue;
+        const LongQ q = long_q<MR>(a, a.items[li]);
+        T v[10];
+#pragma unroll
+        for (int t = 0; t < 10; t++) v[t] = T(0);
+        for (int cs = q.nstr - 1; cs >= 0; cs--) {
+            const T *part = a.lpart + ((uint64_t)li * a.lstrips + cs) * MR * 4 * kL;
+#pragma unroll
+            for (int s = 0; s < MR; s++) {
+                const T c0 = part[(s * 4 + 0) * kL + l], c1 = part[(s * 4 + 1) * kL + l];
+                const T c2 = part[(s * 4 + 2) * kL + l];
+                const int b = (int)part[(s * 4 + 3) * kL + l];
+                v[0] += c0;
+                v[1] += c1;
+                // one bin takes c2; the others add +0, which leaves a total
+                // unchanged (no total is -0: it starts at +0)
+#pragma unroll
+                for (int t = 0; t < 8; t++) v[2 + t] += t == b ? c2 : T(0);
+            }
+        }
+        long_outputs(a, q.p, v, ldexp(1.0, -ilogb(Ssum)), Ssum, a.lexp[li]);
     }
 }
 
@@ -1187,6 +1470,24 @@ hipError_t launch_adj_long(int waves, int grid, size_t lds_bytes, hipStream_t st
     void *k = waves == 1 ? reinterpret_cast<void *>(&adj_long_kernel<1>)
                          : reinterpret_cast<void *>(&adj_long_kernel<kLongWaves>);
     return hipLaunchKernel(k, dim3(grid), dim3(64 * (waves == 1 ? 1 : kLongWaves)), args, lds_bytes, st);
+}
+
+// The three launches of the long pairs' strips over several workgroups:
+// LDS of pass A's and pass B's workgroups, and the launches in order.
+size_t adj_longa_lds(uint32_t lds_stride) { return (size_t)lds_stride + 8 + 12ull * kMaxLongStrips + 16; }
+size_t adj_longb_lds(uint32_t lds_stride, int K1) {
+    return (size_t)packed_stride(lds_stride) + (size_t)long_wave_elems(K1, kLongMR) * 8;
+}
+hipError_t launch_adj_long_split(int grid_a, int grid_b, int grid_c, size_t lds_b, hipStream_t st,
+                                 const AdjArgs &a) {
+    void *args[] = {const_cast<AdjArgs *>(&a)};
+    hipError_t e = hipLaunchKernel(reinterpret_cast<void *>(&adj_longa_kernel), dim3(grid_a), dim3(64 * kLongWaves),
+                                   args, adj_longa_lds(a.lds_stride), st);
+    if (e == hipSuccess)
+        e = hipLaunchKernel(reinterpret_cast<void *>(&adj_longb_kernel), dim3(grid_b), dim3(64), args, lds_b, st);
+    if (e == hipSuccess)
+        e = hipLaunchKernel(reinterpret_cast<void *>(&adj_longc_kernel), dim3(grid_c), dim3(64), args, 0, st);
+    return e;
 }
 
 hipError_t launch_adj(void *kern, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a) {

@@ -59,6 +59,8 @@ int adj_long_max_strips();
 int adj_long_waves();
 size_t adj_long_lds(uint32_t lds_stride, int K1, int waves);
 hipError_t launch_adj_long(int waves, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
+size_t adj_longb_lds(uint32_t lds_stride, int K1);
+hipError_t launch_adj_long_split(int grid_a, int grid_b, int grid_c, size_t lds_b, hipStream_t st, const AdjArgs &a);
 }  // namespace rodv
 
 using namespace rodv;
@@ -169,8 +171,9 @@ int rescue_long_waves_hook() {
     return e && e[0] ? (int)std::strtol(e, nullptr, 10) : 0;
 }
 
-// Waves per workgroup of the long-read adjoint: tests compare the one-wave
-// kernel (RODV_PHMM_ADJ_LONG_WAVES=1) with the default, bit for bit.
+// The long-read adjoint on one workgroup per pair of 1 or 4 waves
+// (RODV_PHMM_ADJ_LONG_WAVES=1 or 4) instead of the default, a pair's strips
+// over several workgroups: tests compare them bit for bit.
 int adj_long_waves_hook() {
     const char *e = std::getenv("RODV_PHMM_ADJ_LONG_WAVES");
     return e && e[0] ? (int)std::strtol(e, nullptr, 10) : 0;
@@ -2491,14 +2494,18 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     if (!rc) rc = dev_alloc(&d_grad, 10 * n);
     if (!rc) rc = dev_alloc(&d_flags, n);
     if (!rc) rc = dev_alloc(&d_resc, n);
-    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)nbk + 2);  // + the long-read kernel's and its faults
+    if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)nbk + 3);  // + the long-read kernels' two and their faults
     if (!rc) rc = dev_alloc(reinterpret_cast<double **>(&d_ck), (size_t)grid * ck_elems);
-    // long reads: per block, every strip's last row but the last strip's,
-    // waves + 1 rows of handed-up adjoints and per wave the level-1
-    // checkpoints of a strip.  A workgroup of adj_long_waves() waves takes
-    // the LDS of a CU; one wave (RODV_PHMM_ADJ_LONG_WAVES=1, for tests) two
-    // workgroups per CU.
-    const int lwaves = adj_long_waves_hook() == 1 ? 1 : adj_long_waves();
+    // long reads, by default a pair's strips over several workgroups (three
+    // launches: pass A per pair, pass B per strip, the totals per pair):
+    // per pair every strip's last row but the last strip's, per strip a row
+    // of handed-up adjoints, and per pass-B wave the level-1 checkpoints of a
+    // strip.  RODV_PHMM_ADJ_LONG_WAVES=1 or 4 (tests) runs each pair on one
+    // workgroup of that many waves instead (per block the rows, waves + 1
+    // rows of handed-up adjoints, per wave the checkpoints), the same bits.
+    const int lhook = adj_long_waves_hook();
+    const bool lsplit = lhook != 1 && lhook != adj_long_waves();
+    const int lwaves = lhook == 1 ? 1 : adj_long_waves();
     const int lgrid = (int)std::max<uint64_t>(
         1, std::min<uint64_t>(longs.size(), (uint64_t)ctx->num_cus * (lwaves == 1 ? 2 : 1)));
     // columns of a row: up to H + 2L + 4 read by the forward, + a checkpoint
@@ -2509,29 +2516,65 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     const uint64_t lnv = 3ull * lmr + 3;
     int lK1 = ckpt_stride;
     const uint32_t lstride = lds_stride_for(max_h_long, 64);
-    auto long_lds = [&](int K1) { return adj_long_lds(lstride, K1, lwaves); };
+    auto long_lds = [&](int K1) { return lsplit ? adj_longb_lds(lstride, K1) : adj_long_lds(lstride, K1, lwaves); };
     while (lK1 > 4 && long_lds(lK1) > (size_t)maxlds) lK1 /= 2;
     lK1 = lK1 / 4 * 4;
     const uint64_t ld4max = ((uint64_t)max_h_long + 64 + 3) / 4 * 4;
     const uint64_t lck_elems = (ld4max / (uint64_t)lK1 + 2) * lnv * 64;
+    // pass B of the split launches: one-wave workgroups, as many per CU as
+    // the LDS holds, at most 4 (each wave holds 512 registers)
+    int lds_cu = 65536;
+    (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, ctx->device);
+    const int bwpc = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)lds_cu / long_lds(lK1)));
+    const uint64_t nlong = longs.size();
+    const uint64_t lstrips = (uint64_t)max_strips;
+    std::vector<uint32_t> bitems;  // (pair << 8 | strip): pairs longest first, strips last first
+    if (lsplit)
+        for (uint64_t k = 0; k < nlong; k++) {
+            const uint32_t R = batch->read_lengths[pr[(size_t)longs[k].second]];
+            const int ns = strips_for(R, 64, lmr);
+            for (int cs = ns - 1; cs >= 0; cs--) bitems.push_back((uint32_t)k << 8 | (uint32_t)cs);
+        }
+    const int bgrid = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>(bitems.size(), (uint64_t)ctx->num_cus * (uint64_t)bwpc));
+    int *d_lsh = nullptr, *d_lexp = nullptr, *d_lprog = nullptr;
+    double *d_lsum = nullptr, *d_lpart = nullptr;
+    uint32_t *d_bitems = nullptr;
     if (!longs.empty()) {
         std::vector<int32_t> li(longs.size());
         for (size_t k = 0; k < longs.size(); k++) li[k] = longs[k].second;
         if (!rc) rc = upload(&d_litems, li.data(), li.size());
-        if (!rc) rc = dev_alloc(&d_lck, (size_t)lgrid * lwaves * lck_elems);
-        if (!rc) rc = dev_alloc(&d_lbnd, (size_t)lgrid * lbnd_stride);
-        if (!rc) rc = dev_alloc(&d_lab, (size_t)lgrid * (lwaves + 1) * lab_elems);
         // rows are written before they are read; zeros keep the columns a
         // strip never writes finite
-        if (!rc && (hipMemset(d_lbnd, 0, (size_t)lgrid * lbnd_stride * 8) != hipSuccess ||
-                    hipMemset(d_lab, 0, (size_t)lgrid * (lwaves + 1) * lab_elems * 8) != hipSuccess))
-            rc = RODV_PHMM_ERR_DEVICE;
+        if (lsplit) {
+            if (!rc) rc = dev_alloc(&d_lck, (size_t)bgrid * lck_elems);
+            if (!rc) rc = dev_alloc(&d_lbnd, (size_t)nlong * lbnd_stride);
+            if (!rc) rc = dev_alloc(&d_lab, (size_t)nlong * lstrips * lab_elems);
+            if (!rc) rc = dev_alloc(&d_lsh, (size_t)nlong * lstrips);
+            if (!rc) rc = dev_alloc(&d_lexp, (size_t)nlong);
+            if (!rc) rc = dev_alloc(&d_lsum, (size_t)nlong);
+            if (!rc) rc = dev_alloc(&d_lprog, (size_t)nlong * lstrips);
+            if (!rc) rc = dev_alloc(&d_lpart, (size_t)nlong * lstrips * lmr * 4 * 64);
+            if (!rc) rc = upload(&d_bitems, bitems.data(), bitems.size());
+            // progress counts start above every step: nothing published
+            if (!rc && (hipMemset(d_lbnd, 0, (size_t)nlong * lbnd_stride * 8) != hipSuccess ||
+                        hipMemset(d_lab, 0, (size_t)nlong * lstrips * lab_elems * 8) != hipSuccess ||
+                        hipMemset(d_lprog, 0x7f, (size_t)nlong * lstrips * 4) != hipSuccess))
+                rc = RODV_PHMM_ERR_DEVICE;
+        } else {
+            if (!rc) rc = dev_alloc(&d_lck, (size_t)lgrid * lwaves * lck_elems);
+            if (!rc) rc = dev_alloc(&d_lbnd, (size_t)lgrid * lbnd_stride);
+            if (!rc) rc = dev_alloc(&d_lab, (size_t)lgrid * (lwaves + 1) * lab_elems);
+            if (!rc && (hipMemset(d_lbnd, 0, (size_t)lgrid * lbnd_stride * 8) != hipSuccess ||
+                        hipMemset(d_lab, 0, (size_t)lgrid * (lwaves + 1) * lab_elems * 8) != hipSuccess))
+                rc = RODV_PHMM_ERR_DEVICE;
+        }
     }
     hipStream_t st = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     if (!rc && (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess))
         rc = RODV_PHMM_ERR_DEVICE;
-    if (!rc && launch_reset(d_cnt, 3 * (uint32_t)nbk + 2, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+    if (!rc && launch_reset(d_cnt, 3 * (uint32_t)nbk + 3, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     if (!rc && hipEventRecord(ev0, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
     for (size_t bi = 0; bi < nbk && !rc; bi++) {
         const int L = shapes[bi].first, mr = shapes[bi].second, G = 64 / L;
@@ -2636,7 +2679,23 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         a.lab_elems = lab_elems;
         a.spin_limit = spin_limit_hook();
         a.fault = d_cnt + 3 * nbk + 1;
-        if (launch_adj_long(lwaves, lgrid, long_lds(lK1), st, a) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
+        if (lsplit) {
+            a.lsh = d_lsh;
+            a.lsum = d_lsum;
+            a.lexp = d_lexp;
+            a.lprog = d_lprog;
+            a.lpart = d_lpart;
+            a.bitems = d_bitems;
+            a.n_bitems = (uint32_t)bitems.size();
+            a.bcounter = d_cnt + 3 * nbk + 2;
+            a.lstrips = (uint32_t)lstrips;
+            const int cgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(nlong, (uint64_t)ctx->num_cus * 4));
+            if (launch_adj_long_split((int)std::min<uint64_t>(nlong, (uint64_t)ctx->num_cus), bgrid, cgrid,
+                                      long_lds(lK1), st, a) != hipSuccess)
+                rc = RODV_PHMM_ERR_DEVICE;
+        } else if (launch_adj_long(lwaves, lgrid, long_lds(lK1), st, a) != hipSuccess) {
+            rc = RODV_PHMM_ERR_DEVICE;
+        }
     }
     std::vector<double> out(n), g(10 * n);
     if (!rc && hipEventRecord(ev1, st) != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
@@ -2665,8 +2724,8 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         }
         if (out_log10) std::memcpy(out_log10, out.data(), n * sizeof(double));
     }
-    void *bufs[] = {d_xf, d_yf, d_xd, d_yd, d_wq, d_qbin, d_items, d_out, d_grad, d_flags,
-                    d_resc, d_cnt, d_ck, d_litems, d_lck, d_lbnd, d_lab};
+    void *bufs[] = {d_xf, d_yf, d_xd, d_yd, d_wq, d_qbin, d_items, d_out, d_grad, d_flags, d_resc, d_cnt,
+                    d_ck, d_litems, d_lck, d_lbnd, d_lab, d_lsh, d_lexp, d_lsum, d_lprog, d_lpart, d_bitems};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     rodv_phmm_plan_destroy(p);

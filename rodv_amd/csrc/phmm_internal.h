@@ -233,6 +233,20 @@ struct AdjArgs {
     // wait gives up, and the count of those that did
     uint32_t spin_limit;
     uint32_t *fault;
+    // a long pair's strips over several workgroups, per long pair li
+    // (lstrips strips' room each): the shifts, the scaled sum and its exponent from
+    // pass A, each strip's published reverse progress, each strip's rows'
+    // gradient contributions (4 x 64 doubles per row); the (pair << 8 |
+    // strip) items of pass B and their work counter
+    int *lsh;
+    double *lsum;
+    int *lexp;
+    int *lprog;
+    double *lpart;
+    const uint32_t *bitems;
+    uint32_t n_bitems;
+    uint32_t *bcounter;
+    uint32_t lstrips;  // strips' room per long pair in lab, lsh, lprog, lpart
 };
 
 }  // namespace rodv

@@ -204,10 +204,11 @@ def _adjoint_env(engine, b, env, **kw):
 
 
 def test_adjoint_long_reads_waves_same_bits(engine):
-    # the long-read kernel deals a pair's strips to the 4 waves of its
-    # workgroup, which hand adjoints up through published progress and add
-    # their sums in strip order: the bits of one wave taking the strips in
-    # sequence (the default kernel is checked against the oracle above).
+    # by default a long pair's strips run on several workgroups (pass A per
+    # pair, pass B one wave per strip handing adjoints up through HBM
+    # progress counts, the sums replayed in strip order); on one workgroup
+    # of 4 waves (LDS progress) or of one wave they are the same bits (the
+    # default is checked against the oracle above).
     # Reads of 2 (fewer strips than waves) to 30 strips of 252 rows, one
     # and two reads per haplotype; the reads unrelated to their haplotypes
     # (R 1,002) lose more per strip than three strips without a fresh shift
@@ -219,21 +220,25 @@ def test_adjoint_long_reads_waves_same_bits(engine):
     b = Batch.concat(regs + [long_pairs(6)])
     w = rng.normal(size=b.num_pairs())
     one = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 1}, weights=w)
-    four = engine.adjoint(b, THETA, weights=w)
-    np.testing.assert_array_equal(four[0].view(np.uint64), one[0].view(np.uint64))
-    np.testing.assert_array_equal(four[1].view(np.uint64), one[1].view(np.uint64))
-    assert np.all(np.isfinite(four[0]))
+    four = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 4}, weights=w)
+    split = engine.adjoint(b, THETA, weights=w)
+    for got in (four, split):
+        np.testing.assert_array_equal(got[0].view(np.uint64), one[0].view(np.uint64))
+        np.testing.assert_array_equal(got[1].view(np.uint64), one[1].view(np.uint64))
+    assert np.all(np.isfinite(split[0]))
 
 
 def test_adjoint_long_wait_bound_is_an_error(engine):
-    # the waits between the waves are bounded: forced to run out from the
-    # host (no poll), every kernel still ends, the call reports a device
-    # error, and the next adjoint on the same context is exact again
+    # the waits between the waves and workgroups are bounded: forced to run
+    # out from the host (no poll), every kernel still ends, the call reports
+    # a device error, and the next adjoint on the same context is exact again
+    # (the default split launches and the one-workgroup kernel alike)
     from rodv_amd.engine import ERR_DEVICE, PhmmError
     b = long_pairs(2)
-    with pytest.raises(PhmmError) as e:
-        _adjoint_env(engine, b, {"RODV_PHMM_SPIN_LIMIT": 0})
-    assert e.value.code == ERR_DEVICE
+    for env in ({"RODV_PHMM_SPIN_LIMIT": 0}, {"RODV_PHMM_SPIN_LIMIT": 0, "RODV_PHMM_ADJ_LONG_WAVES": 4}):
+        with pytest.raises(PhmmError) as e:
+            _adjoint_env(engine, b, env)
+        assert e.value.code == ERR_DEVICE
     ref = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 1})
     out, g = engine.adjoint(b, THETA)
     np.testing.assert_array_equal(out, ref[0])
