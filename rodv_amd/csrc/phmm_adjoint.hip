@@ -706,6 +706,7 @@ constexpr int kNoMax = -0x40000000;  // a strip's last row is all zero
 // other choice of shifts except in subnormal entries.
 constexpr int kShiftLag = 4;
 constexpr int kSafeLog = -500;  // below: pass A again with a shift per strip
+constexpr int kPubSegs = 4;     // pass B publishes its progress every 4 checkpoint segments
 
 // LDS of a long-read workgroup: hap codes (lds_stride bytes) | per wave:
 // level-2 states, feeder inputs of a segment, adjoints from below of a
@@ -719,13 +720,15 @@ __host__ __device__ constexpr uint64_t long_lds_bytes(uint32_t lds_stride, int K
     return lds_stride + ((uint64_t)W * long_wave_elems(K1, MR) + 10ull * kL + 1) * 8 + 16ull * kMaxLongStrips + 16;
 }
 
-// Waits until the published progress *c is <= need; false if `limit` polls
-// (of about half a microsecond each) did not see it.  Wave-uniform.
-__device__ __forceinline__ bool wait_progress(const int *c, int need, uint32_t limit) {
+// Waits until the published progress *c is <= need, the value seen to
+// `seen`; false if `limit` polls (of about half a microsecond each) did not
+// see it.  Wave-uniform.
+__device__ __forceinline__ bool wait_progress(const int *c, int need, uint32_t limit, int &seen) {
     for (uint32_t k = 0; k < limit; k++) {
         const int v = uni(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
         if (v <= need) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // the handed-up rows after the count
+            seen = v;
             return true;
         }
         __builtin_amdgcn_s_sleep(16);
@@ -976,11 +979,12 @@ __device__ void long_forward(const AdjArgs &a, const LongQ &q, const uint8_t *cp
 
 // Progress of the strips of one pair spread over several workgroups
 // (HBM counters, agent scope): as wait_progress / publish_progress.
-__device__ __forceinline__ bool wait_progress_x(const int *c, int need, uint32_t limit) {
+__device__ __forceinline__ bool wait_progress_x(const int *c, int need, uint32_t limit, int &seen) {
     for (uint32_t k = 0; k < limit; k++) {
         const int v = uni(__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (v <= need) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the handed-up rows after the count
+            seen = v;
             return true;
         }
         __builtin_amdgcn_s_sleep(16);
@@ -1125,6 +1129,7 @@ __device__ void long_strip_b(const AdjArgs &a, const LongQ &q, const uint8_t *cp
         rstep(W0, W1, dw + 1, d0s);
     };
     // passes 2 and 3, segment by segment from the last
+    int seen = kProgNone;  // the strip below's progress as last seen
     for (int k = nseg - 1; k >= 0; k--) {
         const int d0s = k * K1;
         const int nsteps = min(K1, D4 - d0s);
@@ -1134,9 +1139,9 @@ __device__ void long_strip_b(const AdjArgs &a, const LongQ &q, const uint8_t *cp
             // columns d0s - 61 .. d0s - 61 + K1 of the rows handed up,
             // once the strip below has finished their steps
             const int lo = max(1, d0s - 61), hi = min(H, d0s - 61 + K1);
-            if (live && lo <= hi) {
-                live = XWG ? wait_progress_x(prog_below, lo, a.spin_limit)
-                           : wait_progress(prog_below, lo, a.spin_limit);
+            if (live && lo <= hi && seen > lo) {  // (a count seen earlier may cover them)
+                live = XWG ? wait_progress_x(prog_below, lo, a.spin_limit, seen)
+                           : wait_progress(prog_below, lo, a.spin_limit, seen);
                 if (!live && l == 0) atomicAdd(a.fault, 1u);
             }
             const T poison = live ? T(1) : T(__builtin_nan(""));
@@ -1170,8 +1175,10 @@ __device__ void long_strip_b(const AdjArgs &a, const LongQ &q, const uint8_t *cp
             }
             window4(W0, dw, d0s);
         }
-        // steps d0s + 1 .. of this strip done: their handed-up columns
-        if (cs > 0) {
+        // steps d0s + 1 .. of this strip done: their handed-up columns,
+        // published every kPubSegs segments (each publication's release
+        // costs a write-back of the L2 in the XWG case) and at the end
+        if (cs > 0 && k % kPubSegs == 0) {
             if constexpr (XWG)
                 publish_progress_x(prog_self, d0s + 1);
             else
@@ -1260,7 +1267,8 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
                                 bst, live, gd, gdT, gg, ge);
         // the strip's rows into the per-lane totals, after the strip below's
         if (!last && live) {
-            live = wait_progress(prog + cs + 1, 0, a.spin_limit);
+            int seen = kProgNone;
+            live = wait_progress(prog + cs + 1, 0, a.spin_limit, seen);
             if (!live && l == 0) atomicAdd(a.fault, 1u);
         }
 #pragma unroll
