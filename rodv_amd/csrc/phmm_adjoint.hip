@@ -16,8 +16,9 @@
 //     half from a copy advanced 4 steps) and run the adjoint of those steps in
 //     reverse, lanes reading the row below their last row from lane l+1.
 // Values cross lanes through ds_bpermute (LaneXfer), not DPP, so that two
-// waves of a SIMD can co-issue the arithmetic.  Longer reads run strip by
-// strip in one wave in FP64 (adj_long_kernel, below).
+// waves of a SIMD can co-issue the arithmetic.  Longer reads run in FP64
+// strips, each pair's strips over several workgroups (adj_longa/b/c_kernel,
+// below).
 // Adjoint seeds are sigma = 2^-ilogb(S) so that adjoint x forward products
 // stay O(expected counts); the gradient of log10 L follows at the end.
 // Per real cell (i, j) with Tb = p * Mb (p the emission prior):
@@ -587,7 +588,8 @@ __global__ __launch_bounds__(64, RESCUE ? 1 : 2) void adj2_kernel(AdjArgs a) {
 
 // ------------------------------------------------------------ long reads
 // Reads longer than one strip of 63 * MR rows, in FP64 (their FP32 sums
-// underflow), one pair per workgroup, its strips over the waves:
+// underflow), in two passes over the strips (run by the launches described
+// at adj_longa_kernel, or for tests on one workgroup, adj_long_kernel):
 //  A. forward over every strip, storing each strip's last row in HBM; the
 //     row a strip hands to the next is scaled by a power of two (kept per
 //     strip) that takes the largest entry of the row four strips back, in

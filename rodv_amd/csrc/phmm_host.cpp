@@ -179,6 +179,14 @@ int adj_long_waves_hook() {
     return e && e[0] ? (int)std::strtol(e, nullptr, 10) : 0;
 }
 
+// Long pairs per chunk of the split long-read adjoint
+// (RODV_PHMM_ADJ_LONG_CHUNK=n; default: as many as 8 GiB of buffers hold):
+// tests run several small chunks and compare them with one, bit for bit.
+uint64_t adj_long_chunk_hook() {
+    const char *e = std::getenv("RODV_PHMM_ADJ_LONG_CHUNK");
+    return e && e[0] ? (uint64_t)std::strtoull(e, nullptr, 10) : 0;
+}
+
 template <typename T>
 int dev_alloc(T **p, size_t n) {
     *p = nullptr;
@@ -2528,15 +2536,29 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     const int bwpc = (int)std::max<size_t>(1, std::min<size_t>(4, (size_t)lds_cu / long_lds(lK1)));
     const uint64_t nlong = longs.size();
     const uint64_t lstrips = (uint64_t)max_strips;
-    std::vector<uint32_t> bitems;  // (pair << 8 | strip): pairs longest first, strips last first
+    // the split launches run the long pairs in chunks whose per-pair buffers
+    // (rows, handed-up adjoint rows, gradient contributions) fit
+    // kLongChunkBytes; every chunk reuses them
+    constexpr uint64_t kLongChunkBytes = 8ull << 30;
+    const uint64_t pair_bytes = (lbnd_stride + lstrips * lab_elems + lstrips * (uint64_t)lmr * 4 * 64) * 8 +
+                                lstrips * 8 + 16;
+    const uint64_t chook = adj_long_chunk_hook();
+    const uint64_t lchunk =
+        std::max<uint64_t>(1, std::min<uint64_t>(nlong, chook ? chook : kLongChunkBytes / pair_bytes));
+    // (pair in its chunk << 8 | strip): pairs longest first, strips last
+    // first; chunk c's items from bbegin[c]
+    std::vector<uint32_t> bitems, bbegin;
     if (lsplit)
         for (uint64_t k = 0; k < nlong; k++) {
+            if (k % lchunk == 0) bbegin.push_back((uint32_t)bitems.size());
             const uint32_t R = batch->read_lengths[pr[(size_t)longs[k].second]];
             const int ns = strips_for(R, 64, lmr);
-            for (int cs = ns - 1; cs >= 0; cs--) bitems.push_back((uint32_t)k << 8 | (uint32_t)cs);
+            for (int cs = ns - 1; cs >= 0; cs--) bitems.push_back((uint32_t)(k % lchunk) << 8 | (uint32_t)cs);
         }
-    const int bgrid = (int)std::max<uint64_t>(
-        1, std::min<uint64_t>(bitems.size(), (uint64_t)ctx->num_cus * (uint64_t)bwpc));
+    bbegin.push_back((uint32_t)bitems.size());
+    uint64_t bmax = 1;  // items of the largest chunk
+    for (size_t c = 0; c + 1 < bbegin.size(); c++) bmax = std::max<uint64_t>(bmax, bbegin[c + 1] - bbegin[c]);
+    const int bgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bmax, (uint64_t)ctx->num_cus * (uint64_t)bwpc));
     int *d_lsh = nullptr, *d_lexp = nullptr, *d_lprog = nullptr;
     double *d_lsum = nullptr, *d_lpart = nullptr;
     uint32_t *d_bitems = nullptr;
@@ -2548,18 +2570,18 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         // strip never writes finite
         if (lsplit) {
             if (!rc) rc = dev_alloc(&d_lck, (size_t)bgrid * lck_elems);
-            if (!rc) rc = dev_alloc(&d_lbnd, (size_t)nlong * lbnd_stride);
-            if (!rc) rc = dev_alloc(&d_lab, (size_t)nlong * lstrips * lab_elems);
-            if (!rc) rc = dev_alloc(&d_lsh, (size_t)nlong * lstrips);
-            if (!rc) rc = dev_alloc(&d_lexp, (size_t)nlong);
-            if (!rc) rc = dev_alloc(&d_lsum, (size_t)nlong);
-            if (!rc) rc = dev_alloc(&d_lprog, (size_t)nlong * lstrips);
-            if (!rc) rc = dev_alloc(&d_lpart, (size_t)nlong * lstrips * lmr * 4 * 64);
+            if (!rc) rc = dev_alloc(&d_lbnd, (size_t)lchunk * lbnd_stride);
+            if (!rc) rc = dev_alloc(&d_lab, (size_t)lchunk * lstrips * lab_elems);
+            if (!rc) rc = dev_alloc(&d_lsh, (size_t)lchunk * lstrips);
+            if (!rc) rc = dev_alloc(&d_lexp, (size_t)lchunk);
+            if (!rc) rc = dev_alloc(&d_lsum, (size_t)lchunk);
+            if (!rc) rc = dev_alloc(&d_lprog, (size_t)lchunk * lstrips);
+            if (!rc) rc = dev_alloc(&d_lpart, (size_t)lchunk * lstrips * lmr * 4 * 64);
             if (!rc) rc = upload(&d_bitems, bitems.data(), bitems.size());
             // progress counts start above every step: nothing published
-            if (!rc && (hipMemset(d_lbnd, 0, (size_t)nlong * lbnd_stride * 8) != hipSuccess ||
-                        hipMemset(d_lab, 0, (size_t)nlong * lstrips * lab_elems * 8) != hipSuccess ||
-                        hipMemset(d_lprog, 0x7f, (size_t)nlong * lstrips * 4) != hipSuccess))
+            if (!rc && (hipMemset(d_lbnd, 0, (size_t)lchunk * lbnd_stride * 8) != hipSuccess ||
+                        hipMemset(d_lab, 0, (size_t)lchunk * lstrips * lab_elems * 8) != hipSuccess ||
+                        hipMemset(d_lprog, 0x7f, (size_t)lchunk * lstrips * 4) != hipSuccess))
                 rc = RODV_PHMM_ERR_DEVICE;
         } else {
             if (!rc) rc = dev_alloc(&d_lck, (size_t)lgrid * lwaves * lck_elems);
@@ -2685,14 +2707,29 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
             a.lexp = d_lexp;
             a.lprog = d_lprog;
             a.lpart = d_lpart;
-            a.bitems = d_bitems;
-            a.n_bitems = (uint32_t)bitems.size();
             a.bcounter = d_cnt + 3 * nbk + 2;
             a.lstrips = (uint32_t)lstrips;
-            const int cgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(nlong, (uint64_t)ctx->num_cus * 4));
-            if (launch_adj_long_split((int)std::min<uint64_t>(nlong, (uint64_t)ctx->num_cus), bgrid, cgrid,
-                                      long_lds(lK1), st, a) != hipSuccess)
-                rc = RODV_PHMM_ERR_DEVICE;
+            for (uint64_t c0 = 0, ci = 0; c0 < nlong && !rc; c0 += lchunk, ci++) {
+                const uint64_t cn = std::min<uint64_t>(lchunk, nlong - c0);
+                a.items = d_litems + c0;
+                a.n_items = (uint32_t)cn;
+                a.bitems = d_bitems + bbegin[ci];
+                a.n_bitems = bbegin[ci + 1] - bbegin[ci];
+                // a later chunk starts as the first: work counters from 0,
+                // progress counts above every step, rows zero
+                if (ci && (launch_reset(a.counter, 1, st) != hipSuccess ||
+                           launch_reset(a.bcounter, 1, st) != hipSuccess ||
+                           hipMemsetAsync(d_lbnd, 0, (size_t)cn * lbnd_stride * 8, st) != hipSuccess ||
+                           hipMemsetAsync(d_lab, 0, (size_t)cn * lstrips * lab_elems * 8, st) != hipSuccess ||
+                           hipMemsetAsync(d_lprog, 0x7f, (size_t)cn * lstrips * 4, st) != hipSuccess)) {
+                    rc = RODV_PHMM_ERR_DEVICE;
+                    break;
+                }
+                const int cgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(cn, (uint64_t)ctx->num_cus * 4));
+                if (launch_adj_long_split((int)std::min<uint64_t>(cn, (uint64_t)ctx->num_cus), bgrid, cgrid,
+                                          long_lds(lK1), st, a) != hipSuccess)
+                    rc = RODV_PHMM_ERR_DEVICE;
+            }
         } else if (launch_adj_long(lwaves, lgrid, long_lds(lK1), st, a) != hipSuccess) {
             rc = RODV_PHMM_ERR_DEVICE;
         }

@@ -208,7 +208,8 @@ def test_adjoint_long_reads_waves_same_bits(engine):
     # pair, pass B one wave per strip handing adjoints up through HBM
     # progress counts, the sums replayed in strip order); on one workgroup
     # of 4 waves (LDS progress) or of one wave they are the same bits (the
-    # default is checked against the oracle above).
+    # default is checked against the oracle above), and so are the split
+    # launches run a few pairs at a time.
     # Reads of 2 (fewer strips than waves) to 30 strips of 252 rows, one
     # and two reads per haplotype; the reads unrelated to their haplotypes
     # (R 1,002) lose more per strip than three strips without a fresh shift
@@ -222,7 +223,9 @@ def test_adjoint_long_reads_waves_same_bits(engine):
     one = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 1}, weights=w)
     four = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 4}, weights=w)
     split = engine.adjoint(b, THETA, weights=w)
-    for got in (four, split):
+    # the split launches over chunks of 3 long pairs that reuse their buffers
+    chunked = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_CHUNK": 3}, weights=w)
+    for got in (four, split, chunked):
         np.testing.assert_array_equal(got[0].view(np.uint64), one[0].view(np.uint64))
         np.testing.assert_array_equal(got[1].view(np.uint64), one[1].view(np.uint64))
     assert np.all(np.isfinite(split[0]))
