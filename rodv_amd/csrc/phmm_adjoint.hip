@@ -571,6 +571,9 @@ __global__ __launch_bounds__(64, RESCUE ? 1 : 2) void adj2_kernel(AdjArgs a) {
     constexpr int G = 64 / L;
     const uint32_t n = RESCUE ? (*a.n_items_dev + G - 1) / G : a.n_items;
     const uint32_t nlist = RESCUE ? *a.n_items_dev : 0u;
+    // an empty FP64 list (the usual case) ends every workgroup before it
+    // queues on the shared counter's atomic
+    if (__builtin_amdgcn_readfirstlane(n) == 0) return;
     for (uint32_t iter = 0; iter <= n; iter++) {
         uint32_t it = 0;
         if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);

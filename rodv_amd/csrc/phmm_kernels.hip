@@ -1019,6 +1019,8 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int G = 64 / L;
     const uint32_t n = RESCUE ? *a.n_items_dev : a.n_items;
+    // an empty list ends every wave before it queues on the counter's atomic
+    if (__builtin_amdgcn_readfirstlane(n) == 0) return;
     // a wave takes at most n items: the bound makes every wave exit
     for (uint32_t iter = 0; iter <= n; iter++) {
         uint32_t it = 0;
@@ -1468,6 +1470,7 @@ __global__ __launch_bounds__(64 * kLW) void fwd_long_kernel(FwdArgs a) {
 __global__ __launch_bounds__(64) void fwd_rescue_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t n = *a.n_items_dev;
+    if (__builtin_amdgcn_readfirstlane(n) == 0) return;  // empty list: no atomic
     for (uint32_t iter = 0; iter <= n; iter++) {
         uint32_t it = 0;
         if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
@@ -1484,6 +1487,7 @@ __global__ __launch_bounds__(64) void fwd_rescue_kernel(FwdArgs a) {
 __global__ __launch_bounds__(64) void fwd_rescue_route_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t n = *a.n_items_dev;
+    if (__builtin_amdgcn_readfirstlane(n) == 0) return;  // empty list: no atomic
     for (uint32_t iter = 0; iter <= n; iter++) {
         uint32_t it = 0;
         if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
@@ -1797,6 +1801,7 @@ __global__ __launch_bounds__(64) void fwd_rescue_tab_kernel(FwdArgs a) {
         ni[k] = (cnt[k] + 1) / 2;
         tot += ni[k];
     }
+    if (tot == 0) return;  // empty classes: no atomic
     for (uint32_t iter = 0; iter <= tot; iter++) {
         uint32_t it = 0;
         if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
@@ -1975,6 +1980,7 @@ __device__ void ext_pair(const FwdArgs &a, uint8_t *lds, int p) {
 __global__ __launch_bounds__(64) void fwd_ext_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t n = *a.n_items_dev;
+    if (__builtin_amdgcn_readfirstlane(n) == 0) return;  // empty list: no atomic
     for (uint32_t iter = 0; iter <= n; iter++) {
         uint32_t it = 0;
         if (threadIdx.x == 0) it = atomicAdd(a.counter, 1u);
