@@ -17,8 +17,8 @@
 //     reverse, lanes reading the row below their last row from lane l+1.
 // Values cross lanes through ds_bpermute (LaneXfer), not DPP, so that two
 // waves of a SIMD can co-issue the arithmetic.  Longer reads run in FP64
-// strips, each pair's strips over several workgroups (adj_longa/b/c_kernel,
-// below).
+// strips, each pair's strips over several workgroups (adj_longab_kernel and
+// adj_longc_kernel, below).
 // Adjoint seeds are sigma = 2^-ilogb(S) so that adjoint x forward products
 // stay O(expected counts); the gradient of log10 L follows at the end.
 // Per real cell (i, j) with Tb = p * Mb (p the emission prior):
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(64, RESCUE ? 1 : 2) void adj2_kernel(AdjArgs a) {
 // ------------------------------------------------------------ long reads
 // Reads longer than one strip of 63 * MR rows, in FP64 (their FP32 sums
 // underflow), in two passes over the strips (run by the launches described
-// at adj_longa_kernel, or for tests on one workgroup, adj_long_kernel):
+// before adj_longa_kernel, or for tests on one workgroup, adj_long_kernel):
 //  A. forward over every strip, storing each strip's last row in HBM; the
 //     row a strip hands to the next is scaled by a power of two (kept per
 //     strip) that takes the largest entry of the row four strips back, in
@@ -809,27 +809,27 @@ __device__ __forceinline__ void long_codes(const AdjArgs &a, uint8_t *lds, const
 // Pass B's codes, two per byte: column j at nibble L - 1 + j (low nibble
 // first), the one-hot codes (N = 0xf) without the bit that makes a read N
 // match, which long_strip_b puts back on every nonzero code (the zeros past
-// the haplotype stay zero, as in the byte codes); by one wave, between
-// workgroup barriers.  Half the LDS of the byte codes, so that four pass-B
-// waves fit a CU.
+// the haplotype stay zero, as in the byte codes); by one wave on its own
+// (one wave's LDS operations complete in order).  Half the LDS of the byte
+// codes, so that four pass-B waves fit a CU.
 __host__ __device__ constexpr uint32_t packed_stride(uint32_t lds_stride) { return ((lds_stride + 1) / 2 + 15) & ~15u; }
 __device__ __forceinline__ uint32_t pack_nibbles(uint32_t x) {  // 4 codes < 16 -> 2 bytes
     return (x & 0xfu) | ((x >> 4) & 0xf0u) | ((x >> 8) & 0xf00u) | ((x >> 12) & 0xf000u);
 }
 __device__ __forceinline__ void long_codes_packed(const AdjArgs &a, uint8_t *lds, const LongQ &q) {
-    __syncthreads();  // the previous item's LDS reads are done
+    const uint32_t l = threadIdx.x & 63;
+    wave_sync();  // the previous item's LDS reads are done
     const uint32_t ps = packed_stride(a.lds_stride);
-    for (uint32_t k = threadIdx.x * 16; k < ps; k += 64 * 16)
-        *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
+    for (uint32_t k = l * 16; k < ps; k += 64 * 16) *reinterpret_cast<uint4 *>(lds + k) = make_uint4(0, 0, 0, 0);
     const uint4 *src = reinterpret_cast<const uint4 *>(a.hapc + a.hapc_off[q.h]);
     const int nch = (q.H + 15) >> 4;
-    __syncthreads();
-    for (int c = threadIdx.x; c < nch; c += 64) {  // columns 16c + 1 .. 16c + 16 at bytes 32 + 8c ..
+    wave_sync();
+    for (int c = (int)l; c < nch; c += 64) {  // columns 16c + 1 .. 16c + 16 at bytes 32 + 8c ..
         const uint4 v = src[c];
         *reinterpret_cast<uint2 *>(lds + kL / 2 + 8 * c) =
             make_uint2(pack_nibbles(v.x) | pack_nibbles(v.y) << 16, pack_nibbles(v.z) | pack_nibbles(v.w) << 16);
     }
-    __syncthreads();
+    wave_sync();
 }
 
 // Pass A over the strips from strip `first` in steps of `stride`, the
@@ -1331,56 +1331,124 @@ __global__ __launch_bounds__(64 * W, 1) void adj_long_kernel(AdjArgs a) {
 //     order of the one-workgroup kernel (the same bits), and the outputs.
 // The longest pair's strips then run on as many CUs as it has strips
 // instead of on one.
-__global__ __launch_bounds__(64 * kLongWaves, 1) void adj_longa_kernel(AdjArgs a) {
+// LDS of pass A's workgroup: codes | scaled sum | shifts | forward progress
+// | row exponents | item
+struct LongALds {
+    double *ssum;
+    int *shs, *progA, *mlog, *slot;
+    __device__ LongALds(uint8_t *lds, uint32_t lds_stride) {
+        ssum = reinterpret_cast<double *>(lds + lds_stride);
+        shs = reinterpret_cast<int *>(ssum + 1);
+        progA = shs + kMaxLongStrips;
+        mlog = progA + kMaxLongStrips;
+        slot = mlog + kMaxLongStrips;
+    }
+};
+
+// Pass A of long pair li by the workgroup's kLongWaves waves: its rows to
+// the pair's B rows (a.lbnd + li * lbnd_stride), its shifts, scaled sum and
+// exponent to a.lsh, a.lsum, a.lexp.  Ends with a workgroup barrier.
+__device__ void long_a_item(const AdjArgs &a, uint8_t *lds, const LongALds &m, uint32_t li) {
     using T = double;
-    extern __shared__ __align__(16) uint8_t lds[];
-    // LDS: codes | scaled sum | shifts | forward progress | row exponents | item
-    T *ssum = reinterpret_cast<T *>(lds + a.lds_stride);
-    int *shs = reinterpret_cast<int *>(ssum + 1);
-    int *progA = shs + kMaxLongStrips;
-    int *mlog = progA + kMaxLongStrips;
-    int *slot = mlog + kMaxLongStrips;
-    const uint32_t n = a.n_items;
-    for (uint32_t iter = 0; iter <= n; iter++) {
-        __syncthreads();  // every wave has read the slot
-        if (threadIdx.x == 0) *slot = (int)atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const uint32_t li = (uint32_t)uni(*slot);
-        if (li >= n) break;
-        const LongQ q = long_q<kLongMR>(a, a.items[li]);
-        long_codes(a, lds, q, 64 * kLongWaves);
-        const uint8_t *cp = lds + (kL - 1) - (int)(threadIdx.x & 63);
-        T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)li * a.lbnd_stride;
-        long_forward<kLongMR, kLongWaves>(a, q, cp, B, shs, progA, mlog, ssum);
-        const T Ssum = *ssum;
-        for (int k = threadIdx.x; k < q.nstr; k += 64 * kLongWaves) a.lsh[(uint64_t)li * a.lstrips + k] = shs[k];
-        if (threadIdx.x == 0) {
-            int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
-            for (int k = 1; k < q.nstr; k++) E += shs[k];
-            a.lsum[li] = Ssum;
-            a.lexp[li] = E;
-            if (!(Ssum > T(0))) {  // underflow even renormalised: no gradient
-                a.out_log10[q.p] = -INFINITY;
-                for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)q.p + t] = 0.0;
-                a.flags[q.p] = 0x03;
-            }
+    const LongQ q = long_q<kLongMR>(a, a.items[li]);
+    long_codes(a, lds, q, 64 * kLongWaves);
+    const uint8_t *cp = lds + (kL - 1) - (int)(threadIdx.x & 63);
+    T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)li * a.lbnd_stride;
+    long_forward<kLongMR, kLongWaves>(a, q, cp, B, m.shs, m.progA, m.mlog, m.ssum);
+    const T Ssum = *m.ssum;
+    for (int k = threadIdx.x; k < q.nstr; k += 64 * kLongWaves) a.lsh[(uint64_t)li * a.lstrips + k] = m.shs[k];
+    if (threadIdx.x == 0) {
+        int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
+        for (int k = 1; k < q.nstr; k++) E += m.shs[k];
+        a.lsum[li] = Ssum;
+        a.lexp[li] = E;
+        if (!(Ssum > T(0))) {  // underflow even renormalised: no gradient
+            a.out_log10[q.p] = -INFINITY;
+            for (int t = 0; t < 10; t++) a.out_grad[10 * (uint64_t)q.p + t] = 0.0;
+            a.flags[q.p] = 0x03;
         }
     }
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(64, 1) void adj_longb_kernel(AdjArgs a) {
+// Pass B of strip cs of long pair li by one wave (LDS from `lds`: packed
+// codes | level-2 states | feeder inputs of a segment | adjoints from below
+// of a segment; level-1 checkpoints in ck).  The adjoints it hands up go
+// through one HBM row per strip, with the steps done published in an HBM
+// count per strip; it waits (bounded) only for the strip below.  The rows'
+// gradient contributions go to a.lpart per strip, row and lane.
+__device__ void long_b_item(const AdjArgs &a, uint8_t *lds, uint32_t li, int cs, double *ck) {
     using T = double;
     constexpr int MR = kLongMR;
     constexpr int C = (kL - 1) * MR;
-    extern __shared__ __align__(16) uint8_t lds[];
     const int l = threadIdx.x & 63;
     const int K1 = a.ckpt;
-    // LDS: packed codes | level-2 states | feeder inputs of a segment |
-    // adjoints from below of a segment
     T *l2 = reinterpret_cast<T *>(lds + packed_stride(a.lds_stride));
     T *fst = l2 + (size_t)((K1 + 7) / 8) * (3 * MR + 3) * kL;
     T *bst = fst + 3 * K1;
-    T *ck = reinterpret_cast<T *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
+    const T Ssum = a.lsum[li];
+    if (!__builtin_amdgcn_readfirstlane(Ssum > T(0) ? 1 : 0)) return;  // no gradient: no strip runs
+    const LongQ q = long_q<MR>(a, a.items[li]);
+    long_codes_packed(a, lds, q);
+    const uint8_t *cp = lds;
+    const T *B = reinterpret_cast<const T *>(a.lbnd) + (uint64_t)li * a.lbnd_stride;
+    const int *sh = a.lsh + (uint64_t)li * a.lstrips;
+    const bool last = cs == q.nstr - 1;
+    T *ab = reinterpret_cast<T *>(a.lab) + (uint64_t)li * a.lstrips * a.lab_elems;
+    int *pr = a.lprog + (uint64_t)li * a.lstrips;
+    bool live = true;
+    T gd[MR], gdT[MR], gg[MR], ge[MR];
+    long_strip_b<MR, true>(a, q, cp, cs, B, sh[cs], last ? 0 : sh[cs + 1], ldexp(1.0, -ilogb(Ssum)),
+                           ab + (uint64_t)(last ? cs : cs + 1) * a.lab_elems, ab + (uint64_t)cs * a.lab_elems,
+                           pr + (last ? cs : cs + 1), pr + cs, ck, l2, fst, bst, live, gd, gdT, gg, ge);
+    // the rows' contributions to the totals, as the one-workgroup kernel
+    // adds them: per row the delta and gamma sums, the error-prior sum and
+    // its quality bin
+    T *part = a.lpart + ((uint64_t)li * a.lstrips + cs) * MR * 4 * kL;
+#pragma unroll
+    for (int s = 0; s < MR; s++) {
+        const int i = cs * C + l * MR + s - q.off + 1;
+        const bool rl = l < kL - 1 && i >= 1 && i <= q.R;
+        const uint32_t qq = a.q[q.ro + (uint64_t)(rl ? i - 1 : 0)] & 127u;
+        part[(s * 4 + 0) * kL + l] = rl ? fma(T(-2), gdT[s], gd[s]) : T(0);
+        part[(s * 4 + 1) * kL + l] = rl ? gg[s] : T(0);
+        part[(s * 4 + 2) * kL + l] = rl ? ge[s] * a.wq[qq] : T(0);
+        part[(s * 4 + 3) * kL + l] = (T)a.qbin[qq];
+    }
+}
+
+// The long pairs' strips over several workgroups, by default in two
+// launches (adj_longab_kernel, then adj_longc_kernel), for tests in three
+// (adj_longa_kernel, adj_longb_kernel, adj_longc_kernel):
+//  A. pass A of each pair (a.items, longest first) on a workgroup of
+//     kLongWaves waves (long_a_item);
+//  B. one wave per item (pair, strip) -- pairs longest first, each pair's
+//     strips last to first -- running pass B of the strip (long_b_item).  A
+//     wave waits (bounded) only for the strip below, an earlier item that a
+//     running wave took, so every wait ends;
+//  C. per pair, the contributions added strip by strip, last first, in the
+//     order of the one-workgroup kernel (the same bits), and the outputs.
+// The longest pair's strips then run on as many CUs as it has strips
+// instead of on one.
+__global__ __launch_bounds__(64 * kLongWaves, 1) void adj_longa_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const LongALds m(lds, a.lds_stride);
+    const uint32_t n = a.n_items;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        __syncthreads();  // every wave has read the slot
+        if (threadIdx.x == 0) *m.slot = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint32_t li = (uint32_t)uni(*m.slot);
+        if (li >= n) break;
+        long_a_item(a, lds, m, li);
+    }
+}
+
+// B over the list a.bitems of (pair << 8 | strip), one-wave workgroups.
+__global__ __launch_bounds__(64, 1) void adj_longb_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const int l = threadIdx.x & 63;
+    double *ck = reinterpret_cast<double *>(a.ck) + (uint64_t)blockIdx.x * a.ck_elems;
     const uint32_t n = a.n_bitems;
     // a wave takes at most n items: the bound makes every wave exit
     for (uint32_t iter = 0; iter <= n; iter++) {
@@ -1389,37 +1457,92 @@ __global__ __launch_bounds__(64, 1) void adj_longb_kernel(AdjArgs a) {
         it = __builtin_amdgcn_readfirstlane(it);
         if (it >= n) break;
         const uint32_t e = a.bitems[it];
-        const uint32_t li = (uint32_t)uni((int)(e >> 8));
-        const int cs = uni((int)(e & 0xffu));
-        const T Ssum = a.lsum[li];
-        if (!__builtin_amdgcn_readfirstlane(Ssum > T(0) ? 1 : 0)) continue;  // no gradient: no strip runs
-        const LongQ q = long_q<MR>(a, a.items[li]);
-        long_codes_packed(a, lds, q);
-        const uint8_t *cp = lds;
-        const T *B = reinterpret_cast<const T *>(a.lbnd) + (uint64_t)li * a.lbnd_stride;
-        const int *sh = a.lsh + (uint64_t)li * a.lstrips;
-        const bool last = cs == q.nstr - 1;
-        T *ab = reinterpret_cast<T *>(a.lab) + (uint64_t)li * a.lstrips * a.lab_elems;
-        int *pr = a.lprog + (uint64_t)li * a.lstrips;
-        bool live = true;
-        T gd[MR], gdT[MR], gg[MR], ge[MR];
-        long_strip_b<MR, true>(a, q, cp, cs, B, sh[cs], last ? 0 : sh[cs + 1], ldexp(1.0, -ilogb(Ssum)),
-                               ab + (uint64_t)(last ? cs : cs + 1) * a.lab_elems, ab + (uint64_t)cs * a.lab_elems,
-                               pr + (last ? cs : cs + 1), pr + cs, ck, l2, fst, bst, live, gd, gdT, gg, ge);
-        // the rows' contributions to the totals, as the one-workgroup kernel
-        // adds them: per row the delta and gamma sums, the error-prior sum
-        // and its quality bin
-        T *part = a.lpart + ((uint64_t)li * a.lstrips + cs) * MR * 4 * kL;
-#pragma unroll
-        for (int s = 0; s < MR; s++) {
-            const int i = cs * C + l * MR + s - q.off + 1;
-            const bool rl = l < kL - 1 && i >= 1 && i <= q.R;
-            const uint32_t qq = a.q[q.ro + (uint64_t)(rl ? i - 1 : 0)] & 127u;
-            part[(s * 4 + 0) * kL + l] = rl ? fma(T(-2), gdT[s], gd[s]) : T(0);
-            part[(s * 4 + 1) * kL + l] = rl ? gg[s] : T(0);
-            part[(s * 4 + 2) * kL + l] = rl ? ge[s] * a.wq[qq] : T(0);
-            part[(s * 4 + 3) * kL + l] = (T)a.qbin[qq];
+        long_b_item(a, lds, (uint32_t)uni((int)(e >> 8)), uni((int)(e & 0xffu)), ck);
+    }
+}
+
+// LDS of one pass-B wave (rounded to 16 bytes) and of a workgroup of
+// adj_longab_kernel: pass A's, or kLongWaves waves' of pass B.
+__host__ __device__ constexpr uint64_t longb_wave_lds(uint32_t lds_stride, int K1) {
+    return ((uint64_t)packed_stride(lds_stride) + long_wave_elems(K1, kLongMR) * 8 + 15) & ~15ull;
+}
+__host__ __device__ constexpr uint64_t longa_lds(uint32_t lds_stride) {
+    return (uint64_t)lds_stride + 8 + 12ull * kMaxLongStrips + 16;
+}
+
+// A and B in one launch: each workgroup runs pass A of pairs from a.counter
+// (longest first) until none is left, then each of its waves on its own
+// runs pass B of strips: of the first pair (longest first) whose pass A is
+// done (a.lready) with strips left (a.lnext counts those taken, last strip
+// first; a.lnstr the pair's strips).  So pass B of the shorter pairs fills
+// the CUs whose pass A ended while the longest pairs' pass A runs.  A wave
+// finds no strip only while some pair's pass A runs on a workgroup that
+// took it, so its polls end (bounded all the same: a.spin_limit polls, then
+// a fault count).  Pass A publishes a pair with every wave's agent-scope
+// release (its rows, shifts and sum) before the flag.
+__global__ __launch_bounds__(64 * kLongWaves, 1) void adj_longab_kernel(AdjArgs a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const LongALds m(lds, a.lds_stride);
+    const uint32_t n = a.n_items;
+    for (uint32_t iter = 0; iter <= n; iter++) {
+        __syncthreads();  // every wave has read the slot
+        if (threadIdx.x == 0) *m.slot = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint32_t li = (uint32_t)uni(*m.slot);
+        if (li >= n) break;
+        long_a_item(a, lds, m, li);
+        __threadfence();  // this wave's rows and shifts, at agent scope
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (!(a.lsum[li] > 0.0))  // no strip runs: every one taken
+                __hip_atomic_store(a.lnext + li, 0x40000000u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            __hip_atomic_store(a.lready + li, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    }
+    __syncthreads();  // pass A's LDS is free
+    const int w = uni((int)(threadIdx.x >> 6));
+    const int l = threadIdx.x & 63;
+    uint8_t *lw = lds + (uint64_t)w * longb_wave_lds(a.lds_stride, a.ckpt);
+    double *ck = reinterpret_cast<double *>(a.ck) + ((uint64_t)blockIdx.x * kLongWaves + w) * a.ck_elems;
+    uint32_t k0 = 0;     // the pairs before k0 have no strip left
+    uint32_t polls = 0;  // polls since this wave last found a strip
+    for (;;) {
+        int li = -1, cs = 0;
+        bool pending = false;  // a pair with strips left whose pass A runs
+        for (uint32_t k = k0; k < n; k++) {
+            const uint32_t ns = a.lnstr[k];
+            const uint32_t used = uni((int)__hip_atomic_load(a.lnext + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (used >= ns) {
+                if (k == k0) k0++;
+                continue;
+            }
+            if (!uni(__hip_atomic_load(a.lready + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                pending = true;
+                continue;
+            }
+            uint32_t t = 0;
+            if (l == 0) t = atomicAdd(a.lnext + k, 1u);
+            t = __builtin_amdgcn_readfirstlane(t);
+            if (t < ns) {
+                li = (int)k;
+                cs = (int)(ns - 1 - t);
+                break;
+            }
+            if (k == k0) k0++;
+        }
+        if (li >= 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // pass A's rows, shifts and sum
+            long_b_item(a, lw, (uint32_t)li, cs, ck);
+            polls = 0;
+            continue;
+        }
+        if (!pending) break;  // every strip taken
+        if (++polls > a.spin_limit) {
+            if (l == 0) atomicAdd(a.fault, 1u);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(32);
     }
 }
 
@@ -1487,10 +1610,14 @@ hipError_t launch_adj_long(int waves, int grid, size_t lds_bytes, hipStream_t st
 
 // The three launches of the long pairs' strips over several workgroups:
 // LDS of pass A's and pass B's workgroups, and the launches in order.
-size_t adj_longa_lds(uint32_t lds_stride) { return (size_t)lds_stride + 8 + 12ull * kMaxLongStrips + 16; }
-size_t adj_longb_lds(uint32_t lds_stride, int K1) {
-    return (size_t)packed_stride(lds_stride) + (size_t)long_wave_elems(K1, kLongMR) * 8;
+size_t adj_longa_lds(uint32_t lds_stride) { return (size_t)longa_lds(lds_stride); }
+size_t adj_longb_lds(uint32_t lds_stride, int K1) { return (size_t)longb_wave_lds(lds_stride, K1); }
+size_t adj_longab_lds(uint32_t lds_stride, int K1) {
+    const uint64_t b = (uint64_t)kLongWaves * longb_wave_lds(lds_stride, K1);
+    return (size_t)(b > longa_lds(lds_stride) ? b : longa_lds(lds_stride));
 }
+// pass-B waves of one adj_longab_kernel workgroup
+int adj_longab_waves() { return kLongWaves; }
 hipError_t launch_adj_long_split(int grid_a, int grid_b, int grid_c, size_t lds_b, hipStream_t st,
                                  const AdjArgs &a) {
     void *args[] = {const_cast<AdjArgs *>(&a)};
@@ -1498,6 +1625,15 @@ hipError_t launch_adj_long_split(int grid_a, int grid_b, int grid_c, size_t lds_
                                    args, adj_longa_lds(a.lds_stride), st);
     if (e == hipSuccess)
         e = hipLaunchKernel(reinterpret_cast<void *>(&adj_longb_kernel), dim3(grid_b), dim3(64), args, lds_b, st);
+    if (e == hipSuccess)
+        e = hipLaunchKernel(reinterpret_cast<void *>(&adj_longc_kernel), dim3(grid_c), dim3(64), args, 0, st);
+    return e;
+}
+
+hipError_t launch_adj_long_fused(int grid_ab, int grid_c, size_t lds_ab, hipStream_t st, const AdjArgs &a) {
+    void *args[] = {const_cast<AdjArgs *>(&a)};
+    hipError_t e = hipLaunchKernel(reinterpret_cast<void *>(&adj_longab_kernel), dim3(grid_ab),
+                                   dim3(64 * kLongWaves), args, lds_ab, st);
     if (e == hipSuccess)
         e = hipLaunchKernel(reinterpret_cast<void *>(&adj_longc_kernel), dim3(grid_c), dim3(64), args, 0, st);
     return e;

@@ -61,6 +61,9 @@ size_t adj_long_lds(uint32_t lds_stride, int K1, int waves);
 hipError_t launch_adj_long(int waves, int grid, size_t lds_bytes, hipStream_t st, const AdjArgs &a);
 size_t adj_longb_lds(uint32_t lds_stride, int K1);
 hipError_t launch_adj_long_split(int grid_a, int grid_b, int grid_c, size_t lds_b, hipStream_t st, const AdjArgs &a);
+size_t adj_longab_lds(uint32_t lds_stride, int K1);
+int adj_longab_waves();
+hipError_t launch_adj_long_fused(int grid_ab, int grid_c, size_t lds_ab, hipStream_t st, const AdjArgs &a);
 }  // namespace rodv
 
 using namespace rodv;
@@ -185,6 +188,14 @@ int adj_long_waves_hook() {
 uint64_t adj_long_chunk_hook() {
     const char *e = std::getenv("RODV_PHMM_ADJ_LONG_CHUNK");
     return e && e[0] ? (uint64_t)std::strtoull(e, nullptr, 10) : 0;
+}
+
+// The split long-read adjoint in three launches (RODV_PHMM_ADJ_LONG_LAUNCHES=3:
+// pass A, pass B, the totals) instead of two (passes A and B in one): tests
+// compare them bit for bit.
+int adj_long_launches_hook() {
+    const char *e = std::getenv("RODV_PHMM_ADJ_LONG_LAUNCHES");
+    return e && e[0] ? (int)std::strtol(e, nullptr, 10) : 0;
 }
 
 template <typename T>
@@ -2504,8 +2515,9 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     if (!rc) rc = dev_alloc(&d_resc, n);
     if (!rc) rc = dev_alloc(&d_cnt, 3 * (size_t)nbk + 3);  // + the long-read kernels' two and their faults
     if (!rc) rc = dev_alloc(reinterpret_cast<double **>(&d_ck), (size_t)grid * ck_elems);
-    // long reads, by default a pair's strips over several workgroups (three
-    // launches: pass A per pair, pass B per strip, the totals per pair):
+    // long reads, by default a pair's strips over several workgroups (pass A
+    // per pair, pass B per strip, in one launch or, for tests or where LDS
+    // is short, two; then the totals per pair):
     // per pair every strip's last row but the last strip's, per strip a row
     // of handed-up adjoints, and per pass-B wave the level-1 checkpoints of a
     // strip.  RODV_PHMM_ADJ_LONG_WAVES=1 or 4 (tests) runs each pair on one
@@ -2546,22 +2558,29 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     const uint64_t lchunk =
         std::max<uint64_t>(1, std::min<uint64_t>(nlong, chook ? chook : kLongChunkBytes / pair_bytes));
     // (pair in its chunk << 8 | strip): pairs longest first, strips last
-    // first; chunk c's items from bbegin[c]
-    std::vector<uint32_t> bitems, bbegin;
+    // first; chunk c's items from bbegin[c]; per pair its strips (lnstr)
+    std::vector<uint32_t> bitems, bbegin, lnstr;
     if (lsplit)
         for (uint64_t k = 0; k < nlong; k++) {
             if (k % lchunk == 0) bbegin.push_back((uint32_t)bitems.size());
             const uint32_t R = batch->read_lengths[pr[(size_t)longs[k].second]];
             const int ns = strips_for(R, 64, lmr);
+            lnstr.push_back((uint32_t)ns);
             for (int cs = ns - 1; cs >= 0; cs--) bitems.push_back((uint32_t)(k % lchunk) << 8 | (uint32_t)cs);
         }
     bbegin.push_back((uint32_t)bitems.size());
     uint64_t bmax = 1;  // items of the largest chunk
     for (size_t c = 0; c + 1 < bbegin.size(); c++) bmax = std::max<uint64_t>(bmax, bbegin[c + 1] - bbegin[c]);
     const int bgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(bmax, (uint64_t)ctx->num_cus * (uint64_t)bwpc));
-    int *d_lsh = nullptr, *d_lexp = nullptr, *d_lprog = nullptr;
+    // passes A and B in one launch (adj_longab_kernel) where a workgroup's
+    // LDS holds 4 pass-B waves: one workgroup per CU, or fewer for few strips
+    const bool lfused = lsplit && adj_long_launches_hook() != 3 && adj_longab_lds(lstride, lK1) <= (size_t)maxlds;
+    const int abw = adj_longab_waves();
+    const int abgrid = (int)std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)ctx->num_cus, std::max<uint64_t>(lchunk, (bmax + abw - 1) / abw)));
+    int *d_lsh = nullptr, *d_lexp = nullptr, *d_lprog = nullptr, *d_lready = nullptr;
     double *d_lsum = nullptr, *d_lpart = nullptr;
-    uint32_t *d_bitems = nullptr;
+    uint32_t *d_bitems = nullptr, *d_lnstr = nullptr, *d_lnext = nullptr;
     if (!longs.empty()) {
         std::vector<int32_t> li(longs.size());
         for (size_t k = 0; k < longs.size(); k++) li[k] = longs[k].second;
@@ -2569,7 +2588,8 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         // rows are written before they are read; zeros keep the columns a
         // strip never writes finite
         if (lsplit) {
-            if (!rc) rc = dev_alloc(&d_lck, (size_t)bgrid * lck_elems);
+            const uint64_t nck = lfused ? (uint64_t)abgrid * abw : (uint64_t)bgrid;  // pass-B waves
+            if (!rc) rc = dev_alloc(&d_lck, (size_t)nck * lck_elems);
             if (!rc) rc = dev_alloc(&d_lbnd, (size_t)lchunk * lbnd_stride);
             if (!rc) rc = dev_alloc(&d_lab, (size_t)lchunk * lstrips * lab_elems);
             if (!rc) rc = dev_alloc(&d_lsh, (size_t)lchunk * lstrips);
@@ -2578,10 +2598,16 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
             if (!rc) rc = dev_alloc(&d_lprog, (size_t)lchunk * lstrips);
             if (!rc) rc = dev_alloc(&d_lpart, (size_t)lchunk * lstrips * lmr * 4 * 64);
             if (!rc) rc = upload(&d_bitems, bitems.data(), bitems.size());
-            // progress counts start above every step: nothing published
+            if (!rc) rc = upload(&d_lnstr, lnstr.data(), lnstr.size());
+            if (!rc) rc = dev_alloc(&d_lready, (size_t)lchunk);
+            if (!rc) rc = dev_alloc(&d_lnext, (size_t)lchunk);
+            // progress counts start above every step: nothing published; no
+            // pair's pass A done, no strip taken
             if (!rc && (hipMemset(d_lbnd, 0, (size_t)lchunk * lbnd_stride * 8) != hipSuccess ||
                         hipMemset(d_lab, 0, (size_t)lchunk * lstrips * lab_elems * 8) != hipSuccess ||
-                        hipMemset(d_lprog, 0x7f, (size_t)lchunk * lstrips * 4) != hipSuccess))
+                        hipMemset(d_lprog, 0x7f, (size_t)lchunk * lstrips * 4) != hipSuccess ||
+                        hipMemset(d_lready, 0, (size_t)lchunk * 4) != hipSuccess ||
+                        hipMemset(d_lnext, 0, (size_t)lchunk * 4) != hipSuccess))
                 rc = RODV_PHMM_ERR_DEVICE;
         } else {
             if (!rc) rc = dev_alloc(&d_lck, (size_t)lgrid * lwaves * lck_elems);
@@ -2715,20 +2741,27 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                 a.n_items = (uint32_t)cn;
                 a.bitems = d_bitems + bbegin[ci];
                 a.n_bitems = bbegin[ci + 1] - bbegin[ci];
+                a.lnstr = d_lnstr + c0;
+                a.lready = d_lready;
+                a.lnext = d_lnext;
                 // a later chunk starts as the first: work counters from 0,
                 // progress counts above every step, rows zero
                 if (ci && (launch_reset(a.counter, 1, st) != hipSuccess ||
                            launch_reset(a.bcounter, 1, st) != hipSuccess ||
                            hipMemsetAsync(d_lbnd, 0, (size_t)cn * lbnd_stride * 8, st) != hipSuccess ||
                            hipMemsetAsync(d_lab, 0, (size_t)cn * lstrips * lab_elems * 8, st) != hipSuccess ||
-                           hipMemsetAsync(d_lprog, 0x7f, (size_t)cn * lstrips * 4, st) != hipSuccess)) {
+                           hipMemsetAsync(d_lprog, 0x7f, (size_t)cn * lstrips * 4, st) != hipSuccess ||
+                           hipMemsetAsync(d_lready, 0, (size_t)cn * 4, st) != hipSuccess ||
+                           hipMemsetAsync(d_lnext, 0, (size_t)cn * 4, st) != hipSuccess)) {
                     rc = RODV_PHMM_ERR_DEVICE;
                     break;
                 }
                 const int cgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(cn, (uint64_t)ctx->num_cus * 4));
-                if (launch_adj_long_split((int)std::min<uint64_t>(cn, (uint64_t)ctx->num_cus), bgrid, cgrid,
-                                          long_lds(lK1), st, a) != hipSuccess)
-                    rc = RODV_PHMM_ERR_DEVICE;
+                const hipError_t le =
+                    lfused ? launch_adj_long_fused(abgrid, cgrid, adj_longab_lds(lstride, lK1), st, a)
+                           : launch_adj_long_split((int)std::min<uint64_t>(cn, (uint64_t)ctx->num_cus), bgrid,
+                                                   cgrid, long_lds(lK1), st, a);
+                if (le != hipSuccess) rc = RODV_PHMM_ERR_DEVICE;
             }
         } else if (launch_adj_long(lwaves, lgrid, long_lds(lK1), st, a) != hipSuccess) {
             rc = RODV_PHMM_ERR_DEVICE;
@@ -2762,7 +2795,8 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         if (out_log10) std::memcpy(out_log10, out.data(), n * sizeof(double));
     }
     void *bufs[] = {d_xf, d_yf, d_xd, d_yd, d_wq, d_qbin, d_items, d_out, d_grad, d_flags, d_resc, d_cnt,
-                    d_ck, d_litems, d_lck, d_lbnd, d_lab, d_lsh, d_lexp, d_lsum, d_lprog, d_lpart, d_bitems};
+                    d_ck, d_litems, d_lck, d_lbnd, d_lab, d_lsh, d_lexp, d_lsum, d_lprog, d_lpart, d_bitems,
+                    d_lnstr, d_lready, d_lnext};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     rodv_phmm_plan_destroy(p);

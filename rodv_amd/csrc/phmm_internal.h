@@ -247,6 +247,11 @@ struct AdjArgs {
     uint32_t n_bitems;
     uint32_t *bcounter;
     uint32_t lstrips;  // strips' room per long pair in lab, lsh, lprog, lpart
+    // passes A and B in one launch: per long pair its strips, whether its
+    // pass A is done (published after its rows), and the strips taken
+    const uint32_t *lnstr;
+    int *lready;
+    uint32_t *lnext;
 };
 
 }  // namespace rodv

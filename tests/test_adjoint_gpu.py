@@ -206,10 +206,12 @@ def _adjoint_env(engine, b, env, **kw):
 def test_adjoint_long_reads_waves_same_bits(engine):
     # by default a long pair's strips run on several workgroups (pass A per
     # pair, pass B one wave per strip handing adjoints up through HBM
-    # progress counts, the sums replayed in strip order); on one workgroup
-    # of 4 waves (LDS progress) or of one wave they are the same bits (the
-    # default is checked against the oracle above), and so are the split
-    # launches run a few pairs at a time.
+    # progress counts, in one launch that starts a pair's pass B once its
+    # pass A is done; the sums replayed in strip order by a second); in
+    # three launches (A, B, the sums), on one workgroup of 4 waves (LDS
+    # progress) or of one wave they are the same bits (the default is
+    # checked against the oracle above), and so are the launches run a few
+    # pairs at a time.
     # Reads of 2 (fewer strips than waves) to 30 strips of 252 rows, one
     # and two reads per haplotype; the reads unrelated to their haplotypes
     # (R 1,002) lose more per strip than three strips without a fresh shift
@@ -223,9 +225,10 @@ def test_adjoint_long_reads_waves_same_bits(engine):
     one = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 1}, weights=w)
     four = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 4}, weights=w)
     split = engine.adjoint(b, THETA, weights=w)
+    three = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_LAUNCHES": 3}, weights=w)
     # the split launches over chunks of 3 long pairs that reuse their buffers
     chunked = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_CHUNK": 3}, weights=w)
-    for got in (four, split, chunked):
+    for got in (four, split, three, chunked):
         np.testing.assert_array_equal(got[0].view(np.uint64), one[0].view(np.uint64))
         np.testing.assert_array_equal(got[1].view(np.uint64), one[1].view(np.uint64))
     assert np.all(np.isfinite(split[0]))
@@ -235,10 +238,12 @@ def test_adjoint_long_wait_bound_is_an_error(engine):
     # the waits between the waves and workgroups are bounded: forced to run
     # out from the host (no poll), every kernel still ends, the call reports
     # a device error, and the next adjoint on the same context is exact again
-    # (the default split launches and the one-workgroup kernel alike)
+    # (the default launches, the three launches and the one-workgroup kernel
+    # alike)
     from rodv_amd.engine import ERR_DEVICE, PhmmError
     b = long_pairs(2)
-    for env in ({"RODV_PHMM_SPIN_LIMIT": 0}, {"RODV_PHMM_SPIN_LIMIT": 0, "RODV_PHMM_ADJ_LONG_WAVES": 4}):
+    for env in ({"RODV_PHMM_SPIN_LIMIT": 0}, {"RODV_PHMM_SPIN_LIMIT": 0, "RODV_PHMM_ADJ_LONG_LAUNCHES": 3},
+                {"RODV_PHMM_SPIN_LIMIT": 0, "RODV_PHMM_ADJ_LONG_WAVES": 4}):
         with pytest.raises(PhmmError) as e:
             _adjoint_env(engine, b, env)
         assert e.value.code == ERR_DEVICE
