@@ -832,16 +832,27 @@ __device__ __forceinline__ void long_codes_packed(const AdjArgs &a, uint8_t *lds
     wave_sync();
 }
 
+// Elements of one strip's level-1 checkpoints (every K1 steps, the state at
+// step 0 first) for a haplotype of H columns: an upper bound of the steps a
+// strip's reverse covers, H + 63 rounded up to 4.
+__host__ __device__ constexpr uint64_t long_ck_elems(uint32_t H, int K1, int MR) {
+    return ((((uint64_t)H + 64 + 3) / 4 * 4) / (uint64_t)K1 + 2) * (3ull * MR + 3) * kL;
+}
+
 // Pass A over the strips from strip `first` in steps of `stride`, the
 // shifts taken `lag` strips back; a strip's feeder waits for the strip
 // above to publish the columns of each block (LDS progress, bounded waits).
 // Each strip's last row to B (row cs at cs * lrow), its shift to shs[cs],
 // the exponent of its largest entry to mlog[cs]; the last strip's sum to
-// *ssum.
+// *ssum.  ck1 (may be null): each strip's level-1 checkpoints for pass B,
+// strip cs's from ck1 + cs * long_ck_elems(H): the lane state at step 0 and
+// every K1 steps below the last step pass B's reverse covers, the states
+// pass B's own forward would checkpoint (the same step function, inputs and
+// order), so pass B skips that forward.
 template <int MR>
 __device__ void long_pass_a(const AdjArgs &a, const LongQ &q, const uint8_t *cp, double *B, int *shs, int *progA,
                             int *mlog, double *ssum, const int first, const int stride, const int lag,
-                            bool &liveA) {
+                            bool &liveA, double *ck1) {
     using T = double;
     const int l = threadIdx.x & 63;
     const int lane = l;
@@ -885,6 +896,14 @@ __device__ void long_pass_a(const AdjArgs &a, const LongQ &q, const uint8_t *cp,
         } else if (lane == 0) {
             shs[0] = 0;
         }
+        // pass B's checkpoints of this strip: up to the step before its
+        // reverse's last (a multiple of 4)
+        constexpr int NV = 3 * MR + 3;
+        const int K1 = a.ckpt;
+        const int D4b = ((last ? H + lR : H + kL - 2) + kW - 1) / kW * kW;
+        T *cks = ck1 ? ck1 + (uint64_t)cs * long_ck_elems((uint32_t)H, K1, MR) : nullptr;
+        int nck = K1;  // the next checkpoint's step
+        if (cks) put_state(cks, NV, 0, S);
         // a strip that is not the last stores its last row up to column
         // H + 69, past every column the recomputations' feeders read (D4 + 1)
         const int Dend = last ? H + lR : H + 2 * kL + 3;
@@ -929,6 +948,10 @@ __device__ void long_pass_a(const AdjArgs &a, const LongQ &q, const uint8_t *cp,
                         S.D[MR - 1] = bd;
                     }
                 }
+                if (cks && d == nck) {
+                    if (d < D4b) put_state(cks, NV, d / K1, S);
+                    nck += K1;
+                }
             };
             int d = d0;
             for (; d + 1 <= dlim; d += 2) {
@@ -957,7 +980,7 @@ __device__ void long_pass_a(const AdjArgs &a, const LongQ &q, const uint8_t *cp,
 // workgroup barrier.
 template <int MR, int W>
 __device__ void long_forward(const AdjArgs &a, const LongQ &q, const uint8_t *cp, double *B, int *shs, int *progA,
-                             int *mlog, double *ssum) {
+                             int *mlog, double *ssum, double *ck1) {
     static_assert(kShiftLag % W == 0, "strip cs - kShiftLag ran on this wave");
     const int w = uni((int)(threadIdx.x >> 6));
     for (int k = threadIdx.x; k < kMaxLongStrips; k += kL * W) {
@@ -966,7 +989,7 @@ __device__ void long_forward(const AdjArgs &a, const LongQ &q, const uint8_t *cp
     }
     __syncthreads();
     bool liveA = true;  // this wave's waits have not run out
-    long_pass_a<MR>(a, q, cp, B, shs, progA, mlog, ssum, w, W, kShiftLag, liveA);
+    long_pass_a<MR>(a, q, cp, B, shs, progA, mlog, ssum, w, W, kShiftLag, liveA, ck1);
     __syncthreads();
     bool again = !(*ssum > 0.0) || ilogb(*ssum) < kSafeLog;
     for (int k = 0; k + 1 < q.nstr; k++) again = again || mlog[k] < kSafeLog;
@@ -977,7 +1000,7 @@ __device__ void long_forward(const AdjArgs &a, const LongQ &q, const uint8_t *cp
             mlog[k] = kNoMax;
         }
         __syncthreads();
-        if (w == 0) long_pass_a<MR>(a, q, cp, B, shs, progA, mlog, ssum, 0, 1, 1, liveA);
+        if (w == 0) long_pass_a<MR>(a, q, cp, B, shs, progA, mlog, ssum, 0, 1, 1, liveA, ck1);
     }
     __syncthreads();
 }
@@ -1015,7 +1038,7 @@ __device__ void long_strip_b(const AdjArgs &a, const LongQ &q, const uint8_t *cp
                              const int sh, const int sh_below, const double sigma, const double *ab_in,
                              double *ab_out, const int *prog_below, int *prog_self, double *ck, double *l2,
                              double *fst, double *bst, bool &live, double (&gd)[MR], double (&gdT)[MR],
-                             double (&gg)[MR], double (&ge)[MR]) {
+                             double (&gg)[MR], double (&ge)[MR], const double *cka = nullptr) {
     using T = double;
     constexpr int NV = 3 * MR + 3;
     const int l = threadIdx.x & 63;
@@ -1080,8 +1103,9 @@ __device__ void long_strip_b(const AdjArgs &a, const LongQ &q, const uint8_t *cp
             }
         }
     };
-    // pass 1: the strip's forward with level-1 checkpoints
-    {
+    // pass 1: the strip's forward with level-1 checkpoints, unless pass A
+    // left them in cka
+    if (!cka) {
         State<T, MR> S = S0;
         put_state(ck, NV, 0, S);
         for (int k = 0; k < nseg; k++) {
@@ -1159,7 +1183,7 @@ __device__ void long_strip_b(const AdjArgs &a, const LongQ &q, const uint8_t *cp
         }
         wave_sync();
         State<T, MR> S;
-        get_state(ck, NV, k, S);
+        get_state(cka ? cka : ck, NV, k, S);
         // the window starts; the last window's steps are its own pass 3's
         for (int wi = 0; wi < nwin; wi++) {
             put_state(l2, NV, wi, S);
@@ -1244,7 +1268,7 @@ __device__ void adj_long_item(const AdjArgs &a, uint8_t *lds, int p_in) {
     T *ck = reinterpret_cast<T *>(a.ck) + ((uint64_t)blockIdx.x * W + w) * a.ck_elems;
 
     // ---- A
-    long_forward<MR, W>(a, q, cp, B, shs, progA, mlog, ssum);
+    long_forward<MR, W>(a, q, cp, B, shs, progA, mlog, ssum, nullptr);
     const T Ssum = *ssum;
     int E = 0;  // entries of the last strip: true values x 2^1020 x 2^E
     for (int k = 1; k < nstr; k++) E += shs[k];
@@ -1354,7 +1378,8 @@ __device__ void long_a_item(const AdjArgs &a, uint8_t *lds, const LongALds &m, u
     long_codes(a, lds, q, 64 * kLongWaves);
     const uint8_t *cp = lds + (kL - 1) - (int)(threadIdx.x & 63);
     T *B = reinterpret_cast<T *>(a.lbnd) + (uint64_t)li * a.lbnd_stride;
-    long_forward<kLongMR, kLongWaves>(a, q, cp, B, m.shs, m.progA, m.mlog, m.ssum);
+    double *ck1 = a.lck1 ? a.lck1 + a.lck1_off[li] : nullptr;
+    long_forward<kLongMR, kLongWaves>(a, q, cp, B, m.shs, m.progA, m.mlog, m.ssum, ck1);
     const T Ssum = *m.ssum;
     for (int k = threadIdx.x; k < q.nstr; k += 64 * kLongWaves) a.lsh[(uint64_t)li * a.lstrips + k] = m.shs[k];
     if (threadIdx.x == 0) {
@@ -1398,9 +1423,11 @@ __device__ void long_b_item(const AdjArgs &a, uint8_t *lds, uint32_t li, int cs,
     int *pr = a.lprog + (uint64_t)li * a.lstrips;
     bool live = true;
     T gd[MR], gdT[MR], gg[MR], ge[MR];
+    const double *cka =
+        a.lck1 ? a.lck1 + a.lck1_off[li] + (uint64_t)cs * long_ck_elems((uint32_t)q.H, K1, MR) : nullptr;
     long_strip_b<MR, true>(a, q, cp, cs, B, sh[cs], last ? 0 : sh[cs + 1], ldexp(1.0, -ilogb(Ssum)),
                            ab + (uint64_t)(last ? cs : cs + 1) * a.lab_elems, ab + (uint64_t)cs * a.lab_elems,
-                           pr + (last ? cs : cs + 1), pr + cs, ck, l2, fst, bst, live, gd, gdT, gg, ge);
+                           pr + (last ? cs : cs + 1), pr + cs, ck, l2, fst, bst, live, gd, gdT, gg, ge, cka);
     // the rows' contributions to the totals, as the one-workgroup kernel
     // adds them: per row the delta and gamma sums, the error-prior sum and
     // its quality bin
@@ -1618,6 +1645,8 @@ size_t adj_longab_lds(uint32_t lds_stride, int K1) {
 }
 // pass-B waves of one adj_longab_kernel workgroup
 int adj_longab_waves() { return kLongWaves; }
+// elements of one strip's pass-A checkpoints for pass B
+uint64_t adj_long_ck_elems(uint32_t H, int K1) { return long_ck_elems(H, K1, kLongMR); }
 hipError_t launch_adj_long_split(int grid_a, int grid_b, int grid_c, size_t lds_b, hipStream_t st,
                                  const AdjArgs &a) {
     void *args[] = {const_cast<AdjArgs *>(&a)};

@@ -63,6 +63,7 @@ size_t adj_longb_lds(uint32_t lds_stride, int K1);
 hipError_t launch_adj_long_split(int grid_a, int grid_b, int grid_c, size_t lds_b, hipStream_t st, const AdjArgs &a);
 size_t adj_longab_lds(uint32_t lds_stride, int K1);
 int adj_longab_waves();
+uint64_t adj_long_ck_elems(uint32_t H, int K1);
 hipError_t launch_adj_long_fused(int grid_ab, int grid_c, size_t lds_ab, hipStream_t st, const AdjArgs &a);
 }  // namespace rodv
 
@@ -2578,9 +2579,28 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     const int abw = adj_longab_waves();
     const int abgrid = (int)std::max<uint64_t>(
         1, std::min<uint64_t>((uint64_t)ctx->num_cus, std::max<uint64_t>(lchunk, (bmax + abw - 1) / abw)));
+    // pass A leaves each strip's level-1 checkpoints in HBM for pass B (which
+    // then skips its own forward) where a chunk's fit kLongCkBytes: per pair
+    // adj_long_ck_elems(H) per strip, at chunk-relative offsets
+    constexpr uint64_t kLongCkBytes = 16ull << 30;
+    std::vector<uint64_t> ck1off;
+    uint64_t ck1max = 0;  // elements of the largest chunk's
+    if (lsplit) {
+        uint64_t acc = 0;
+        for (uint64_t k = 0; k < nlong; k++) {
+            if (k % lchunk == 0) acc = 0;
+            ck1off.push_back(acc);
+            const uint32_t H = batch->hap_lengths[ph[(size_t)longs[k].second]];
+            acc += (uint64_t)lnstr[k] * adj_long_ck_elems(H, lK1);
+            ck1max = std::max(ck1max, acc);
+        }
+    }
+    const bool lck1 = lsplit && ck1max * 8 <= kLongCkBytes;
     int *d_lsh = nullptr, *d_lexp = nullptr, *d_lprog = nullptr, *d_lready = nullptr;
     double *d_lsum = nullptr, *d_lpart = nullptr;
     uint32_t *d_bitems = nullptr, *d_lnstr = nullptr, *d_lnext = nullptr;
+    double *d_lck1 = nullptr;
+    uint64_t *d_lck1_off = nullptr;
     if (!longs.empty()) {
         std::vector<int32_t> li(longs.size());
         for (size_t k = 0; k < longs.size(); k++) li[k] = longs[k].second;
@@ -2589,7 +2609,9 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
         // strip never writes finite
         if (lsplit) {
             const uint64_t nck = lfused ? (uint64_t)abgrid * abw : (uint64_t)bgrid;  // pass-B waves
-            if (!rc) rc = dev_alloc(&d_lck, (size_t)nck * lck_elems);
+            if (!rc) rc = dev_alloc(&d_lck, lck1 ? (size_t)1 : (size_t)nck * lck_elems);
+            if (!rc && lck1) rc = dev_alloc(&d_lck1, (size_t)ck1max);
+            if (!rc && lck1) rc = upload(&d_lck1_off, ck1off.data(), ck1off.size());
             if (!rc) rc = dev_alloc(&d_lbnd, (size_t)lchunk * lbnd_stride);
             if (!rc) rc = dev_alloc(&d_lab, (size_t)lchunk * lstrips * lab_elems);
             if (!rc) rc = dev_alloc(&d_lsh, (size_t)lchunk * lstrips);
@@ -2744,6 +2766,8 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
                 a.lnstr = d_lnstr + c0;
                 a.lready = d_lready;
                 a.lnext = d_lnext;
+                a.lck1 = d_lck1;
+                a.lck1_off = d_lck1_off ? d_lck1_off + c0 : nullptr;
                 // a later chunk starts as the first: work counters from 0,
                 // progress counts above every step, rows zero
                 if (ci && (launch_reset(a.counter, 1, st) != hipSuccess ||
@@ -2796,7 +2820,7 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
     }
     void *bufs[] = {d_xf, d_yf, d_xd, d_yd, d_wq, d_qbin, d_items, d_out, d_grad, d_flags, d_resc, d_cnt,
                     d_ck, d_litems, d_lck, d_lbnd, d_lab, d_lsh, d_lexp, d_lsum, d_lprog, d_lpart, d_bitems,
-                    d_lnstr, d_lready, d_lnext};
+                    d_lnstr, d_lready, d_lnext, d_lck1, d_lck1_off};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     rodv_phmm_plan_destroy(p);

@@ -252,6 +252,10 @@ struct AdjArgs {
     const uint32_t *lnstr;
     int *lready;
     uint32_t *lnext;
+    // pass A's level-1 checkpoints for pass B (null: pass B recomputes
+    // them): long pair li's strips from lck1 + lck1_off[li]
+    double *lck1;
+    const uint64_t *lck1_off;
 };
 
 }  // namespace rodv
