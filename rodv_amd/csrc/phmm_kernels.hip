@@ -998,7 +998,8 @@ __device__ uint32_t process_chain(const FwdArgs &a, uint8_t *lds, uint32_t it) {
 // a haplotype read codes left there, always a valid block 0..3).
 // Up to 12 rows per lane the kernel is held to 168 VGPRs, three waves per
 // SIMD where the LDS allows (a few set-up values spill; the step loop keeps
-// its registers): 0.7 % faster on the batched workload than at 2 waves.
+// its registers): 0.7 % faster on the batched workload than at 2 waves, and
+// 0.3 % faster than 12 rows at 2 waves without its 100 bytes of spills.
 template <int L, int MR, bool INTER, int NS, bool SH>
 __global__ __launch_bounds__(64, MR <= 12 ? 3 : 2) void fwd_chain_kernel(FwdArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
