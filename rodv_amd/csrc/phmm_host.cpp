@@ -199,6 +199,14 @@ int adj_long_launches_hook() {
     return e && e[0] ? (int)std::strtol(e, nullptr, 10) : 0;
 }
 
+// Pass B of the split long-read adjoint recomputes its level-1 checkpoints
+// instead of reading pass A's (RODV_PHMM_ADJ_LONG_PASSA_CKPT=0), as it does
+// where they would not fit: tests compare the two bit for bit.
+bool adj_long_passa_ckpt_hook() {
+    const char *e = std::getenv("RODV_PHMM_ADJ_LONG_PASSA_CKPT");
+    return !(e && e[0] == '0');
+}
+
 template <typename T>
 int dev_alloc(T **p, size_t n) {
     *p = nullptr;
@@ -2595,7 +2603,7 @@ static int adjoint_impl(rodv_phmm_ctx *ctx, const rodv_phmm_batch *batch,
             ck1max = std::max(ck1max, acc);
         }
     }
-    const bool lck1 = lsplit && ck1max * 8 <= kLongCkBytes;
+    const bool lck1 = lsplit && ck1max * 8 <= kLongCkBytes && adj_long_passa_ckpt_hook();
     int *d_lsh = nullptr, *d_lexp = nullptr, *d_lprog = nullptr, *d_lready = nullptr;
     double *d_lsum = nullptr, *d_lpart = nullptr;
     uint32_t *d_bitems = nullptr, *d_lnstr = nullptr, *d_lnext = nullptr;

@@ -226,9 +226,11 @@ def test_adjoint_long_reads_waves_same_bits(engine):
     four = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_WAVES": 4}, weights=w)
     split = engine.adjoint(b, THETA, weights=w)
     three = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_LAUNCHES": 3}, weights=w)
+    # pass B recomputing its checkpoints instead of reading pass A's
+    own = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_PASSA_CKPT": 0}, weights=w)
     # the split launches over chunks of 3 long pairs that reuse their buffers
     chunked = _adjoint_env(engine, b, {"RODV_PHMM_ADJ_LONG_CHUNK": 3}, weights=w)
-    for got in (four, split, three, chunked):
+    for got in (four, split, three, own, chunked):
         np.testing.assert_array_equal(got[0].view(np.uint64), one[0].view(np.uint64))
         np.testing.assert_array_equal(got[1].view(np.uint64), one[1].view(np.uint64))
     assert np.all(np.isfinite(split[0]))
